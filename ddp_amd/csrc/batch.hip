@@ -1,0 +1,565 @@
+// Batched A*: many (s, t) queries in one persistent launch.
+//
+// The reference answers one query per process run (GA_star/GA_v2.cu:298-577)
+// and its applications loop over queries on the host (WSN/wsn_2011.cu:355-456,
+// k-short/GA_merge.cu:702-731); its global PQ symbol (GA_v2.cu:26) makes GA*
+// non-reentrant.  Here each workgroup owns one query slot -- a private
+// (f, parent) array and B lane-owned heaps -- and pulls query indices from a
+// device work counter until the batch is exhausted.  Within a workgroup one
+// GA* iteration pops one node per lane-queue and expands them cooperatively;
+// an improved node is pushed into the heap of the lane that relaxed it, so the
+// search never leaves the workgroup (no grid barrier, no cross-CU traffic).
+// The slot's touched nodes are logged on first touch and reset when the query
+// ends, so slots are reused without clearing N entries per query.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+#include "../../include/ddp_batch.h"
+#include "search_expand.h"
+
+namespace {
+
+enum : uint32_t { QERR_HEAP = 1, QERR_ITERS = 2, QERR_BADQ = 4, QERR_PATH = 8 };
+
+struct BatchArgs {
+    GraphView g;
+    HeurView h;
+    const int32_t *qs, *qt, *order;
+    int32_t nq;
+    uint32_t *next;
+    uint64_t *gp;        // slots * N
+    int32_t *touched;    // slots * N
+    uint64_t *home;      // slots * cap * B
+    int32_t *cost, *plen, *path;
+    int32_t path_cap;
+    uint32_t *qerr;
+    uint64_t *qtime;     // 2 per query: start/end, 100 MHz device clock
+    unsigned long long *stats;
+    int32_t N, lcap, cap, maxit;
+};
+
+template <int B>
+struct SlotPush {
+    const LaneHeap *H;
+    int32_t *size;
+    uint32_t *s_F;
+    int32_t *touched;
+    int32_t *s_ntouch;
+    uint32_t err = 0;
+
+    __device__ void target(uint32_t f) { atomicMin(s_F, f); }
+    __device__ void touch(int32_t c, uint64_t prev)
+    {
+        if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(s_ntouch, 1)] = c;
+    }
+    __device__ void operator()(bool has, uint64_t key)
+    {
+        if (has && !heap_push(*H, *size, key)) err |= QERR_HEAP;
+    }
+};
+
+template <int HK>
+__device__ __forceinline__ void set_target(HeurView &hv, int32_t t)
+{
+    if constexpr (HK == DDP_H_MANHATTAN) {
+        hv.ty = t / hv.width;
+        hv.tx = t - hv.ty * hv.width;
+    } else if constexpr (HK == DDP_H_EUCLID) {
+        hv.tx = hv.xy[2 * t];
+        hv.ty = hv.xy[2 * t + 1];
+    }
+}
+
+template <int B, int HK>
+__global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
+{
+    extern __shared__ uint64_t s_heap[];  // lcap * B
+    __shared__ ExpandLds<B> L;
+    __shared__ uint32_t s_red[16];
+    __shared__ int32_t s_q, s_ntouch, s_plen;
+    __shared__ uint32_t s_F, s_err;
+
+    const int lane = threadIdx.x;
+    const int64_t slot = blockIdx.x;
+    const int32_t N = a.N;
+    uint64_t *gp = a.gp + slot * N;
+    int32_t *touched = a.touched + slot * N;
+    const LaneHeap H{s_heap + lane, a.home + slot * a.cap * B + lane, B, a.lcap, (int64_t)B, a.cap};
+    ExpandCounters C;
+    uint64_t stale = 0;
+
+    for (;;) {
+        if (lane == 0) {
+            s_q = (int32_t)atomicAdd(a.next, 1u);
+            s_F = DDP_INF_F;
+            s_ntouch = 0;
+            s_err = 0;
+        }
+        __syncthreads();
+        const int32_t qi = s_q;
+        if (qi >= a.nq) break;
+        const int32_t q = a.order ? a.order[qi] : qi;
+        const int32_t s = a.qs[q], t = a.qt[q];
+        if (lane == 0) a.qtime[2 * (int64_t)q] = dev_clock_100mhz();
+        HeurView hv = a.h;
+        bool run = s >= 0 && s < N && t >= 0 && t < N;
+        int32_t size = 0;
+        if (run) {
+            set_target<HK>(hv, t);
+            const int32_t hs = heur_eval<HK>(hv, s);
+            if (HK == DDP_H_ARRAY && hs < 0) {
+                run = false;  // h(s) = -1: s cannot reach t
+            } else if (s == t) {
+                run = false;
+                if (lane == 0) s_F = (uint32_t)hs;
+            } else if (lane == 0) {
+                gp[s] = ((uint64_t)(uint32_t)hs << 32) | 0xFFFFFFFFull;
+                touched[0] = s;
+                s_ntouch = 1;
+                heap_push(H, size, ((uint64_t)(uint32_t)hs << 32) | (uint32_t)s);
+            }
+        } else if (lane == 0) {
+            s_err = QERR_BADQ;
+        }
+        block_sync_global();
+        SlotPush<B> push{&H, &size, &s_F, touched, &s_ntouch};
+        if (run) {
+            for (int32_t it = 0;; ++it) {
+                const uint32_t mytop = size > 0 ? (uint32_t)(s_heap[lane] >> 32) : 0xFFFFFFFFu;
+                const uint32_t topmin = block_min_u32<B>(mytop, s_red);
+                if constexpr (B == 64) __syncthreads();
+                const uint32_t F = s_F;
+                if (F <= topmin) break;  // no queued node can improve t
+                if (it >= a.maxit) {
+                    if (lane == 0) atomicOr(&s_err, (uint32_t)QERR_ITERS);
+                    break;
+                }
+                int32_t u = -1;
+                uint32_t fu = 0;
+                while (size > 0) {
+                    uint64_t key;
+                    heap_pop(H, size, key);
+                    const int32_t node = (int32_t)(uint32_t)key;
+                    const uint32_t f = (uint32_t)(key >> 32);
+                    if ((uint32_t)(ld_gp(gp + node) >> 32) == f) {
+                        u = node;
+                        fu = f;
+                        break;
+                    }
+                    ++stale;
+                }
+                expand_block<B, HK>(a.g, hv, gp, t, u, fu, L, C, push);
+                if (push.err) {
+                    atomicOr(&s_err, push.err);
+                    push.err = 0;
+                }
+            }
+        }
+        block_sync_global();
+        const uint32_t F = s_F;
+        // -- path t -> s, written s..t
+        if (a.path_cap > 0) {
+            if (lane == 0) {
+                int32_t n = 0;
+                if (F != DDP_INF_F) {
+                    int32_t *out = a.path + (int64_t)q * a.path_cap;
+                    int32_t v = t;
+                    for (int32_t step = 0; step <= N; ++step) {
+                        if (n < a.path_cap) out[n] = v;
+                        ++n;
+                        if (v == s) break;
+                        const uint32_t p = (uint32_t)ld_gp(gp + v);
+                        if (p >= (uint32_t)N) { n = 0; atomicOr(&s_err, (uint32_t)QERR_PATH); break; }
+                        v = (int32_t)p;
+                    }
+                }
+                s_plen = n;
+            }
+            if (lane == 0) {
+                // reverse t..s into s..t (this lane wrote every entry)
+                const int32_t n = s_plen;
+                if (n > 0 && n <= a.path_cap) {
+                    int32_t *out = a.path + (int64_t)q * a.path_cap;
+                    for (int32_t i = 0; i < n / 2; ++i) {
+                        const int32_t x = out[i];
+                        out[i] = out[n - 1 - i];
+                        out[n - 1 - i] = x;
+                    }
+                }
+                a.plen[q] = (n > a.path_cap) ? -n : n;
+            }
+        }
+        // -- reset the touched entries of this slot (only after lane 0's walk
+        //    above has read every parent it needs)
+        __syncthreads();
+        const int32_t nt = s_ntouch;
+        for (int32_t i = lane; i < nt; i += B)
+            gp[__hip_atomic_load(touched + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = DDP_GP_INF;
+        if (lane == 0) {
+            a.cost[q] = (int32_t)F;
+            a.qerr[q] = s_err;
+            a.qtime[2 * (int64_t)q + 1] = dev_clock_100mhz();
+        }
+        block_sync_global();
+    }
+    const uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
+                   v3 = wave_sum_u64(stale);
+    if ((lane & 63) == 0) {
+        atomicAdd(a.stats + 0, (unsigned long long)v0);
+        atomicAdd(a.stats + 1, (unsigned long long)v1);
+        atomicAdd(a.stats + 2, (unsigned long long)v2);
+        atomicAdd(a.stats + 3, (unsigned long long)v3);
+    }
+}
+
+__global__ void batch_fill_u64(uint64_t *p, int64_t n, uint64_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Resets the work counter and statistics before each launch (a kernel, not a
+// memset node, so the sequence can be captured into a hipGraph).
+__global__ void batch_reset(uint32_t *next, unsigned long long *stats)
+{
+    if (threadIdx.x == 0) *next = 0;
+    if (threadIdx.x < 4) stats[threadIdx.x] = 0;
+}
+
+using BatchFn = void (*)(BatchArgs);
+
+template <int B>
+BatchFn pick_b(int hk)
+{
+    switch (hk) {
+    case DDP_H_ZERO: return batch_kernel<B, DDP_H_ZERO>;
+    case DDP_H_ARRAY: return batch_kernel<B, DDP_H_ARRAY>;
+    case DDP_H_MANHATTAN: return batch_kernel<B, DDP_H_MANHATTAN>;
+    default: return batch_kernel<B, DDP_H_EUCLID>;
+    }
+}
+
+BatchFn pick_batch(int B, int hk)
+{
+    switch (B) {
+    case 64: return pick_b<64>(hk);
+    case 128: return pick_b<128>(hk);
+    case 256: return pick_b<256>(hk);
+    default: return pick_b<512>(hk);
+    }
+}
+
+}  // namespace
+
+struct ddp_batch {
+    ddp_ctx *ctx = nullptr;
+    int device = 0;  // destroy uses this, never ctx (the context may be gone)
+    const ddp_graph *g = nullptr;
+    ddp_heur_t heur{};
+    HeurView hv{};
+    int32_t maxq = 0, B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0, path_cap = 0, schedule = 0;
+    int32_t nq = 0;
+    // device
+    int32_t *qs = nullptr, *qt = nullptr, *order = nullptr, *cost = nullptr, *plen = nullptr, *path = nullptr,
+            *touched = nullptr;
+    uint32_t *next = nullptr, *qerr = nullptr;
+    uint64_t *gp = nullptr, *home = nullptr, *qtime = nullptr;
+    unsigned long long *stats = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t graph_nq = -1;
+    float last_ms = 0;
+};
+
+static void batch_free(ddp_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    void *ptrs[] = {b->qs, b->qt, b->order, b->cost, b->plen, b->path, b->touched, b->next,
+                    b->qerr, b->gp, b->home, b->qtime, b->stats};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+}
+
+extern "C" void ddp_batch_destroy(ddp_batch *b)
+{
+    if (!b) return;
+    hipSetDevice(b->device);  // hipFree waits for work that uses the memory
+    batch_free(b);
+    delete b;
+}
+
+extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t max_queries,
+                                const ddp_batch_opts_t *o, ddp_batch **out)
+{
+    if (!ctx || !g || !out) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_batch_create: null argument");
+    *out = nullptr;
+    DDP_CHECK(max_queries > 0, ctx, DDP_ERR_ARG, "ddp_batch_create: max_queries must be > 0");
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    HeurView hv;
+    int rc = ddp_make_heur(g, h, -1, &hv);
+    if (rc) return rc;
+    int32_t B = (o && o->block) ? o->block : 256;
+    DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512, ctx, DDP_ERR_ARG,
+              "block must be 64, 128, 256 or 512 (got %d)", B);
+    int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(4, std::min(32, 4096 / B));
+    DDP_CHECK((int64_t)lcap * B * 8 <= 64 * 1024, ctx, DDP_ERR_ARG, "lds_cap %d x block %d exceeds 64 KiB", lcap, B);
+    int32_t cap = (o && o->heap_cap) ? o->heap_cap : std::max(lcap, 2048);
+    cap = std::max(cap, lcap);
+    int per_cu = 0;
+    DDP_HIP(ctx, hipFuncSetAttribute((const void *)pick_batch(B, hv.kind), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)((size_t)lcap * B * 8)));
+    DDP_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pick_batch(B, hv.kind), B,
+                                                             (size_t)lcap * B * 8));
+    DDP_CHECK(per_cu > 0, ctx, DDP_ERR_ARG, "batch kernel does not fit on a CU with block %d", B);
+    int64_t slots = (o && o->slots) ? o->slots : (int64_t)per_cu * ctx->prop.multiProcessorCount;
+    slots = std::min<int64_t>(slots, max_queries);
+    // admission by free HBM: keep 10% headroom
+    size_t fr = 0, tot = 0;
+    DDP_HIP(ctx, hipMemGetInfo(&fr, &tot));
+    const size_t per_slot = (size_t)g->N * 12 + (size_t)cap * B * 8;
+    const size_t fixed = (size_t)max_queries * (7 * 4 + 16 + (size_t)std::max(0, o ? o->path_cap : 0) * 4) + 4096;
+    const size_t budget = fr > fixed ? (size_t)((fr - fixed) * 0.9) : 0;
+    slots = std::min<int64_t>(slots, (int64_t)(budget / per_slot));
+    DDP_CHECK(slots >= 1, ctx, DDP_ERR_NOMEM, "not enough free HBM for one query slot (%zu bytes needed, %zu free)",
+              per_slot + fixed, fr);
+    ddp_batch *b = new ddp_batch();
+    b->ctx = ctx;
+    b->device = ctx->device;
+    b->g = g;
+    b->heur = h;
+    b->hv = hv;
+    b->maxq = max_queries;
+    b->B = B;
+    b->slots = (int32_t)slots;
+    b->lcap = lcap;
+    b->cap = cap;
+    b->maxit = (o && o->max_iters) ? o->max_iters : (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
+    b->path_cap = o ? std::max(0, o->path_cap) : 0;
+    b->schedule = o ? o->schedule : 0;
+    const size_t M = (size_t)max_queries;
+    hipError_t e = hipSuccess;
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 8));
+    };
+    al((void **)&b->qs, M * 4);
+    al((void **)&b->qt, M * 4);
+    al((void **)&b->order, M * 4);
+    al((void **)&b->cost, M * 4);
+    al((void **)&b->plen, M * 4);
+    al((void **)&b->qerr, M * 4);
+    al((void **)&b->qtime, M * 16);
+    al((void **)&b->path, M * (size_t)b->path_cap * 4);
+    al((void **)&b->next, 256);
+    al((void **)&b->stats, 64);
+    al((void **)&b->gp, (size_t)slots * g->N * 8);
+    al((void **)&b->touched, (size_t)slots * g->N * 4);
+    al((void **)&b->home, (size_t)slots * cap * B * 8);
+    if (e != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return ddp_hip_err(ctx, e, "batch allocation", __FILE__, __LINE__);
+    }
+    // one-time clear of every slot's (f, parent) array
+    hipLaunchKernelGGL(batch_fill_u64, dim3(4096), dim3(256), 0, ctx->stream, b->gp, (int64_t)slots * g->N,
+                       (uint64_t)DDP_GP_INF);
+    e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return ddp_hip_err(ctx, e, "batch slot clear", __FILE__, __LINE__);
+    }
+    *out = b;
+    return DDP_OK;
+}
+
+extern "C" int ddp_batch_info(const ddp_batch *b, int32_t *slots, int32_t *block, int32_t *heap_cap,
+                              int32_t *lds_cap)
+{
+    if (!b) return DDP_ERR_ARG;
+    if (slots) *slots = b->slots;
+    if (block) *block = b->B;
+    if (heap_cap) *heap_cap = b->cap;
+    if (lds_cap) *lds_cap = b->lcap;
+    return DDP_OK;
+}
+
+extern "C" int ddp_batch_upload(ddp_batch *b, int32_t nq, const int32_t *src, const int32_t *dst)
+{
+    if (!b) return DDP_ERR_ARG;
+    ddp_ctx *ctx = b->ctx;
+    DDP_CHECK(nq >= 0 && nq <= b->maxq, ctx, DDP_ERR_ARG, "nq=%d outside [0,%d]", nq, b->maxq);
+    DDP_CHECK(nq == 0 || (src && dst), ctx, DDP_ERR_ARG, "ddp_batch_upload: null query arrays");
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> ord(nq);
+    std::iota(ord.begin(), ord.end(), 0);
+    if (b->schedule == 0 && b->hv.kind == DDP_H_EUCLID && !b->g->host_xy.empty()) {
+        // longest-estimate-first: straight-line distance orders the search sizes
+        const auto &xy = b->g->host_xy;
+        const int32_t N = b->g->N;
+        std::vector<int64_t> est(nq, 0);
+        for (int32_t i = 0; i < nq; ++i) {
+            if (src[i] < 0 || src[i] >= N || dst[i] < 0 || dst[i] >= N) continue;
+            const int64_t dx = xy[2 * (size_t)src[i]] - xy[2 * (size_t)dst[i]];
+            const int64_t dy = xy[2 * (size_t)src[i] + 1] - xy[2 * (size_t)dst[i] + 1];
+            est[i] = dx * dx + dy * dy;
+        }
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return est[x] > est[y]; });
+    }
+    if (nq) {
+        DDP_HIP(ctx, hipMemcpyAsync(b->qs, src, (size_t)nq * 4, hipMemcpyHostToDevice, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(b->qt, dst, (size_t)nq * 4, hipMemcpyHostToDevice, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(b->order, ord.data(), (size_t)nq * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    b->nq = nq;
+    return DDP_OK;
+}
+
+static int batch_capture(ddp_batch *b)
+{
+    ddp_ctx *ctx = b->ctx;
+    if (b->exec && b->graph_nq == b->nq) return DDP_OK;
+    if (b->exec) {
+        hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    BatchArgs a{};
+    a.g = ddp_view(b->g);
+    a.h = b->hv;
+    a.qs = b->qs;
+    a.qt = b->qt;
+    a.order = b->order;
+    a.nq = b->nq;
+    a.next = b->next;
+    a.gp = b->gp;
+    a.touched = b->touched;
+    a.home = b->home;
+    a.cost = b->cost;
+    a.plen = b->plen;
+    a.path = b->path;
+    a.path_cap = b->path_cap;
+    a.qerr = b->qerr;
+    a.qtime = b->qtime;
+    a.stats = b->stats;
+    a.N = b->g->N;
+    a.lcap = b->lcap;
+    a.cap = b->cap;
+    a.maxit = b->maxit;
+    const int32_t grid = std::max(1, std::min(b->slots, std::max(1, b->nq)));
+    DDP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, ctx->stream, b->next, b->stats);
+    hipLaunchKernelGGL(pick_batch(b->B, b->hv.kind), dim3(grid), dim3(b->B), (size_t)b->lcap * b->B * 8,
+                       ctx->stream, a);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &b->graph);
+    if (e != hipSuccess) return ddp_hip_err(ctx, e, "hipStreamEndCapture", __FILE__, __LINE__);
+    DDP_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_nq = b->nq;
+    return DDP_OK;
+}
+
+extern "C" int ddp_batch_launch(ddp_batch *b)
+{
+    if (!b) return DDP_ERR_ARG;
+    ddp_ctx *ctx = b->ctx;
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = batch_capture(b);
+    if (rc) return rc;
+    DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    DDP_HIP(ctx, hipGraphLaunch(b->exec, ctx->stream));
+    DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    return DDP_OK;
+}
+
+extern "C" int ddp_batch_wait(ddp_batch *b, ddp_batch_stats_t *st)
+{
+    if (!b) return DDP_ERR_ARG;
+    ddp_ctx *ctx = b->ctx;
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    DDP_HIP(ctx, hipEventSynchronize(ctx->ev1));
+    float ms = 0;
+    DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    b->last_ms = ms;
+    if (st) {
+        std::memset(st, 0, sizeof *st);
+        unsigned long long v[4];
+        DDP_HIP(ctx, hipMemcpy(v, b->stats, 32, hipMemcpyDeviceToHost));
+        st->expansions = (int64_t)v[0];
+        st->edges = (int64_t)v[1];
+        st->improvements = (int64_t)v[2];
+        st->stale_pops = (int64_t)v[3];
+        st->slots = b->slots;
+        st->block = b->B;
+        st->kernel_ms = ms;
+        std::vector<uint32_t> qe(b->nq);
+        if (b->nq) DDP_HIP(ctx, hipMemcpy(qe.data(), b->qerr, (size_t)b->nq * 4, hipMemcpyDeviceToHost));
+        for (uint32_t x : qe) st->errors += x != 0;
+    }
+    return DDP_OK;
+}
+
+extern "C" int ddp_batch_download(ddp_batch *b, int32_t *costs, int32_t *path_len, int32_t *paths,
+                                  uint32_t *qerr, double *latency_ms)
+{
+    if (!b) return DDP_ERR_ARG;
+    ddp_ctx *ctx = b->ctx;
+    const size_t n = (size_t)b->nq;
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    if (costs && n) DDP_HIP(ctx, hipMemcpyAsync(costs, b->cost, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (path_len && n) {
+        if (b->path_cap > 0) DDP_HIP(ctx, hipMemcpyAsync(path_len, b->plen, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        else std::memset(path_len, 0, n * 4);
+    }
+    if (paths && n && b->path_cap > 0)
+        DDP_HIP(ctx, hipMemcpyAsync(paths, b->path, n * b->path_cap * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (qerr && n) DDP_HIP(ctx, hipMemcpyAsync(qerr, b->qerr, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint64_t> tt;
+    if (latency_ms && n) {
+        tt.resize(2 * n);
+        DDP_HIP(ctx, hipMemcpyAsync(tt.data(), b->qtime, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (latency_ms && n) {
+        uint64_t t0 = ~0ull;
+        for (size_t i = 0; i < n; ++i) t0 = std::min(t0, tt[2 * i]);
+        // completion time of each query measured from the first query's start
+        for (size_t i = 0; i < n; ++i) latency_ms[i] = (double)(tt[2 * i + 1] - t0) * 1e-5;
+    }
+    return DDP_OK;
+}
+
+extern "C" int ddp_astar_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
+                               const int32_t *dst, const ddp_batch_opts_t *opts, int32_t *costs,
+                               int32_t *path_len, int32_t *paths, ddp_batch_stats_t *stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !g || !costs) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_astar_batch: null argument");
+    if (nq == 0) return DDP_OK;
+    ddp_batch *b = nullptr;
+    int rc = ddp_batch_create(ctx, g, h, nq, opts, &b);
+    if (rc) return rc;
+    std::vector<uint32_t> qe(nq);
+    if (!(rc = ddp_batch_upload(b, nq, src, dst)) && !(rc = ddp_batch_launch(b)) && !(rc = ddp_batch_wait(b, stats)))
+        rc = ddp_batch_download(b, costs, path_len, paths, qe.data(), nullptr);
+    ddp_batch_destroy(b);
+    if (rc) return rc;
+    int32_t nerr = 0, first = -1;
+    uint32_t kind = 0;
+    for (int32_t i = 0; i < nq; ++i)
+        if (qe[i]) {
+            if (first < 0) { first = i; kind = qe[i]; }
+            ++nerr;
+        }
+    if (stats)
+        stats->total_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    DDP_CHECK(nerr == 0, ctx, (kind & QERR_HEAP) ? DDP_ERR_OVERFLOW : (kind & QERR_BADQ) ? DDP_ERR_ARG : DDP_ERR_TIMEOUT,
+              "%d of %d queries failed (first: query %d, flags 0x%x: 1=heap_cap, 2=iterations, 4=bad id, 8=path)",
+              nerr, nq, first, kind);
+    return DDP_OK;
+}

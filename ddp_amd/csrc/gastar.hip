@@ -1,0 +1,470 @@
+// Single-query GA*: K priority queues over one persistent cooperative grid.
+//
+// Reference loop (GA_star/GA_v2.cu:488-559): per iteration the host launches
+// extractMin, A_star_expand, keepHeapPQ, setNV (an O(N) flag scan), insertPQ,
+// copies flags and all K queue sizes back, and runs checkMIN.  Here the whole
+// loop is one kernel:
+//   * queue q is a binary heap owned by one lane (lane q % B of workgroup
+//     q / B); its first lcap entries sit in LDS, the rest spill to HBM;
+//   * each iteration every queue pops its best valid entry, the workgroup
+//     expands the popped nodes cooperatively (search_expand.h), and every
+//     improvement CASes a packed (f << 32 | parent) word -- no per-node locks;
+//   * improved nodes are appended to a push buffer with one atomic per wave
+//     (ballot + mbcnt), replacing setNV's O(N) scan; after the grid barrier
+//     queue q takes entries p with (rr + p) % K == q, the round-robin
+//     distribution of insertPQ (GA_v2.cu:241-273) made continuous across
+//     iterations so queue sizes stay balanced;
+//   * termination is decided on the device: once the target's f is <= the
+//     smallest queue top at the previous iteration's start and every f pushed
+//     during it, no open node can improve it (GA_v2.cu:277-289, 547-557).
+//     The decision reads only values frozen by the barrier, so all workgroups
+//     agree without another barrier.
+// Stale heap entries (a node improved after it was queued) are skipped when
+// popped; this replaces keepHeapPQ's in-place decrease-key repair.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "search_expand.h"
+
+namespace {
+
+// Control block layout (uint32 words).
+enum : int {
+    SLOT_WORDS = 8,          // per slot: pushcnt, topmin, pushmin, tmin, err
+    S_PUSHCNT = 0, S_TOPMIN = 1, S_PUSHMIN = 2, S_TMIN = 3, S_ERR = 4,
+    C_RES_IT = 32, C_RES_RR = 33, C_RES_F = 34, C_RES_ERR = 35,
+    C_BAR = 64,              // bar[0] count, bar[1] generation (own cache line)
+    C_WORDS = 96,
+};
+
+enum : uint32_t { ERR_HEAP = 1, ERR_PBUF = 2, ERR_ITERS = 4, ERR_BARRIER = 8 };
+
+struct GridArgs {
+    GraphView g;
+    HeurView h;
+    uint64_t *gp;
+    uint64_t *home;      // heap spill / home: entry i of queue q at home[i*K + q]
+    int32_t *hsize;      // K
+    uint64_t *pbuf;      // 2 * pcap
+    uint32_t *ctrl;      // C_WORDS
+    unsigned long long *stats;  // expanded, edges, improved, stale
+    int32_t K, lcap, cap, pcap, t, maxit;
+    uint32_t F0;
+    int32_t rr0;
+    uint64_t barrier_ticks;
+};
+
+template <int B>
+struct GridPush {
+    uint64_t *buf;
+    uint32_t *pushcnt;
+    int32_t pcap;
+    uint32_t tmin = 0xFFFFFFFFu, pushmin = 0xFFFFFFFFu;
+    uint32_t err = 0;
+
+    __device__ void target(uint32_t f) { tmin = f < tmin ? f : tmin; }
+    __device__ void touch(int32_t, uint64_t) {}
+    __device__ void operator()(bool has, uint64_t key)
+    {
+        const uint64_t mask = __ballot(has);
+        if (mask == 0) return;
+        const int lane = threadIdx.x & 63;
+        const int leader = __builtin_ctzll(mask);
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(pushcnt, (uint32_t)__popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (has) {
+            const uint32_t idx = base + lane_rank(mask);
+            if (idx < (uint32_t)pcap) buf[idx] = key;
+            else err |= ERR_PBUF;
+            const uint32_t f = (uint32_t)(key >> 32);
+            pushmin = f < pushmin ? f : pushmin;
+        }
+    }
+};
+
+template <int B, int HK>
+__global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
+{
+    extern __shared__ uint64_t s_heap[];  // lcap * B
+    __shared__ ExpandLds<B> L;
+    __shared__ uint32_t s_red[16];
+    __shared__ int32_t s_flag;
+
+    const int lane = threadIdx.x;
+    const int32_t K = a.K;
+    const int32_t q = blockIdx.x * B + lane;
+    uint32_t *ctrl = a.ctrl;
+    const LaneHeap H{s_heap + lane, a.home + q, B, a.lcap, (int64_t)K, a.cap};
+    int32_t size = a.hsize[q];
+    for (int32_t i = 0; i < size && i < a.lcap; ++i) s_heap[i * B + lane] = a.home[(int64_t)i * K + q];
+
+    uint32_t gen = ld_u32(ctrl + C_BAR + 1);
+    uint32_t F = a.F0;
+    int32_t rr = a.rr0;
+    uint32_t err_local = 0, err_seen = 0;
+    ExpandCounters C;
+    uint64_t stale = 0;
+    int32_t it = 0;
+    for (;; ++it) {
+        // -- drain the previous iteration's pushes into this lane's queue
+        const int ps = (it + 2) % 3;
+        const uint32_t n = ld_u32(ctrl + ps * SLOT_WORDS + S_PUSHCNT);
+        const uint32_t topm = ld_u32(ctrl + ps * SLOT_WORDS + S_TOPMIN);
+        const uint32_t pm = ld_u32(ctrl + ps * SLOT_WORDS + S_PUSHMIN);
+        const uint32_t tm = ld_u32(ctrl + ps * SLOT_WORDS + S_TMIN);
+        err_seen |= ld_u32(ctrl + ps * SLOT_WORDS + S_ERR);
+        F = tm < F ? tm : F;
+        const uint32_t navail = n < (uint32_t)a.pcap ? n : (uint32_t)a.pcap;
+        const uint64_t *buf = a.pbuf + (int64_t)((it + 1) & 1) * a.pcap;
+        for (int64_t p = ((int64_t)q - rr) % K + (q < rr ? K : 0); p < (int64_t)navail; p += K) {
+            const uint64_t e = __hip_atomic_load(buf + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (!heap_push(H, size, e)) err_local |= ERR_HEAP;
+        }
+        rr = (int32_t)(((int64_t)rr + n) % K);
+        // -- termination (uniform: every input was frozen by the last barrier)
+        const uint32_t open_min = topm < pm ? topm : pm;
+        if (it > 0 && F <= open_min) break;
+        if (err_seen) break;
+        if (it >= a.maxit) {
+            err_local |= ERR_ITERS;
+            break;
+        }
+        const int cs = it % 3;
+        if (blockIdx.x == 0 && lane == 0) {
+            uint32_t *ns = ctrl + ((it + 1) % 3) * SLOT_WORDS;
+            st_u32(ns + S_PUSHCNT, 0);
+            st_u32(ns + S_TOPMIN, 0xFFFFFFFFu);
+            st_u32(ns + S_PUSHMIN, 0xFFFFFFFFu);
+            st_u32(ns + S_TMIN, 0xFFFFFFFFu);
+            st_u32(ns + S_ERR, 0);
+        }
+        // -- publish the smallest queue top (stale tops only make this smaller)
+        const uint32_t mytop = size > 0 ? (uint32_t)(s_heap[lane] >> 32) : 0xFFFFFFFFu;
+        const uint32_t bmin = block_min_u32<B>(mytop, s_red);
+        if (lane == 0 && bmin != 0xFFFFFFFFu) atomicMin(ctrl + cs * SLOT_WORDS + S_TOPMIN, bmin);
+        // -- pop the best entry whose f is still current
+        int32_t u = -1;
+        uint32_t fu = 0;
+        while (size > 0) {
+            uint64_t key;
+            heap_pop(H, size, key);
+            const int32_t node = (int32_t)(uint32_t)key;
+            const uint32_t f = (uint32_t)(key >> 32);
+            if ((uint32_t)(ld_gp(a.gp + node) >> 32) == f) {
+                u = node;
+                fu = f;
+                break;
+            }
+            ++stale;
+        }
+        // -- expand and push
+        GridPush<B> push{a.pbuf + (int64_t)(it & 1) * a.pcap, ctrl + cs * SLOT_WORDS + S_PUSHCNT, a.pcap};
+        expand_block<B, HK>(a.g, a.h, a.gp, a.t, u, fu, L, C, push);
+        err_local |= push.err;
+        const uint32_t wt = wave_min_u32(push.tmin);
+        const uint32_t wp = wave_min_u32(push.pushmin);
+        if ((lane & 63) == 0) {
+            if (wt != 0xFFFFFFFFu) atomicMin(ctrl + cs * SLOT_WORDS + S_TMIN, wt);
+            if (wp != 0xFFFFFFFFu) atomicMin(ctrl + cs * SLOT_WORDS + S_PUSHMIN, wp);
+        }
+        if (err_local) atomicOr(ctrl + cs * SLOT_WORDS + S_ERR, err_local);
+        if (!grid_barrier(ctrl + C_BAR, gridDim.x, gen, a.barrier_ticks, &s_flag)) {
+            atomicOr(ctrl + C_RES_ERR, (uint32_t)ERR_BARRIER);
+            break;
+        }
+    }
+    // -- persist queues for a later resume (DA*) and report
+    for (int32_t i = 0; i < size && i < a.lcap; ++i) a.home[(int64_t)i * K + q] = s_heap[i * B + lane];
+    a.hsize[q] = size;
+    if (err_local) atomicOr(ctrl + C_RES_ERR, err_local);
+    if (err_seen) atomicOr(ctrl + C_RES_ERR, err_seen);
+    if (blockIdx.x == 0 && lane == 0) {
+        ctrl[C_RES_IT] = (uint32_t)it;
+        ctrl[C_RES_RR] = (uint32_t)rr;
+        ctrl[C_RES_F] = F;
+    }
+    uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
+             v3 = wave_sum_u64(stale);
+    if ((lane & 63) == 0) {
+        atomicAdd(a.stats + 0, (unsigned long long)v0);
+        atomicAdd(a.stats + 1, (unsigned long long)v1);
+        atomicAdd(a.stats + 2, (unsigned long long)v2);
+        atomicAdd(a.stats + 3, (unsigned long long)v3);
+    }
+}
+
+__global__ void fill_u64_kernel(uint64_t *p, int64_t n, uint64_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_i32_kernel(int32_t *p, int64_t n, int32_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Fresh search: gp[s] = (h(s), parent -1); s is the only push of "iteration -1".
+__global__ void seed_kernel(uint64_t *gp, uint64_t *pbuf1, uint32_t *ctrl, int32_t s, uint32_t hs)
+{
+    if (threadIdx.x != 0) return;
+    gp[s] = ((uint64_t)hs << 32) | 0xFFFFFFFFull;
+    pbuf1[0] = ((uint64_t)hs << 32) | (uint32_t)s;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t *sl = ctrl + k * SLOT_WORDS;
+        sl[S_PUSHCNT] = 0;
+        sl[S_TOPMIN] = 0xFFFFFFFFu;
+        sl[S_PUSHMIN] = 0xFFFFFFFFu;
+        sl[S_TMIN] = 0xFFFFFFFFu;
+        sl[S_ERR] = 0;
+    }
+    ctrl[2 * SLOT_WORDS + S_PUSHCNT] = 1;
+    ctrl[2 * SLOT_WORDS + S_PUSHMIN] = hs;
+    ctrl[C_RES_IT] = 0;
+    ctrl[C_RES_RR] = 0;
+    ctrl[C_RES_F] = DDP_INF_F;
+    ctrl[C_RES_ERR] = 0;
+    ctrl[C_BAR] = 0;
+    ctrl[C_BAR + 1] = 0;
+}
+
+__global__ void unpack_parent_kernel(const uint64_t *gp, int32_t *par, int32_t N)
+{
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        par[i] = (int32_t)(uint32_t)gp[i];
+}
+
+using KernelFn = void (*)(GridArgs);
+
+template <int B>
+KernelFn pick_kernel(int hk)
+{
+    switch (hk) {
+    case DDP_H_ZERO: return ga_grid_kernel<B, DDP_H_ZERO>;
+    case DDP_H_ARRAY: return ga_grid_kernel<B, DDP_H_ARRAY>;
+    case DDP_H_MANHATTAN: return ga_grid_kernel<B, DDP_H_MANHATTAN>;
+    default: return ga_grid_kernel<B, DDP_H_EUCLID>;
+    }
+}
+
+KernelFn pick(int B, int hk)
+{
+    switch (B) {
+    case 64: return pick_kernel<64>(hk);
+    case 256: return pick_kernel<256>(hk);
+    default: return pick_kernel<1024>(hk);
+    }
+}
+
+}  // namespace
+
+// Shared with other translation units (DA* resume).
+struct GaGridPlan {
+    int32_t K, B, nblocks, lcap, cap, pcap, maxit;
+};
+
+int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p)
+{
+    int32_t B = (o && o->block) ? o->block : 0;
+    int32_t K = (o && o->K) ? o->K : 1024;
+    DDP_CHECK(K >= 1, ctx, DDP_ERR_ARG, "K must be >= 1 (got %d)", K);
+    if (B == 0) B = K >= 4096 ? 256 : 64;
+    DDP_CHECK(B == 64 || B == 256 || B == 1024, ctx, DDP_ERR_ARG,
+              "block must be 64, 256 or 1024 (got %d)", B);
+    K = ((K + B - 1) / B) * B;
+    const int32_t nblocks = K / B;
+    int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(1, std::min(32, 8192 / B));
+    DDP_CHECK((int64_t)lcap * B * 8 <= 96 * 1024, ctx, DDP_ERR_ARG,
+              "lds_cap %d x block %d exceeds the LDS budget", lcap, B);
+    int64_t cap = (o && o->heap_cap) ? o->heap_cap : (2 * (int64_t)g->N + K - 1) / K + 256;
+    cap = std::max<int64_t>(cap, lcap);
+    DDP_CHECK(cap < (1 << 30), ctx, DDP_ERR_ARG, "heap_cap too large");
+    int64_t pcap = (int64_t)K * std::max(1, g->maxdeg) + K;
+    pcap = std::min<int64_t>(pcap, (int64_t)g->E + 2 * (int64_t)K + 64);
+    pcap = std::max<int64_t>(pcap, 1024);
+    DDP_CHECK(pcap < (1LL << 31), ctx, DDP_ERR_ARG, "push buffer too large");
+    int32_t maxit = (o && o->max_iters) ? o->max_iters : (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
+    *p = GaGridPlan{K, B, nblocks, lcap, (int32_t)cap, (int32_t)pcap, maxit};
+    // co-residency check for the cooperative launch
+    int per_cu = 0;
+    DDP_HIP(ctx, hipFuncSetAttribute((const void *)pick(B, DDP_H_ZERO), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)((size_t)lcap * B * 8)));
+    DDP_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pick(B, DDP_H_ZERO), B,
+                                                             (size_t)lcap * B * 8));
+    DDP_CHECK((int64_t)per_cu * ctx->prop.multiProcessorCount >= nblocks, ctx, DDP_ERR_ARG,
+              "K=%d needs %d co-resident workgroups of %d lanes; the device holds %d", K, nblocks, B,
+              per_cu * ctx->prop.multiProcessorCount);
+    return DDP_OK;
+}
+
+// Buffers of one GA* search state.
+struct GaGridState {
+    uint64_t *gp, *home, *pbuf;
+    int32_t *hsize;
+    uint32_t *ctrl;
+    unsigned long long *stats;
+};
+
+int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridState *S)
+{
+    void *p;
+    int rc;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_GP, (size_t)g->N * 8, &p))) return rc;
+    S->gp = (uint64_t *)p;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_HEAP, (size_t)P.cap * P.K * 8, &p))) return rc;
+    S->home = (uint64_t *)p;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_HSIZE, (size_t)P.K * 4, &p))) return rc;
+    S->hsize = (int32_t *)p;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_PBUF, (size_t)P.pcap * 16, &p))) return rc;
+    S->pbuf = (uint64_t *)p;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_CTRL, C_WORDS * 4 + 64, &p))) return rc;
+    S->ctrl = (uint32_t *)p;
+    S->stats = (unsigned long long *)((char *)p + C_WORDS * 4);
+    return DDP_OK;
+}
+
+// Runs the GA* loop from the current state (fresh or resumed).
+int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+               const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
+               int32_t *rr_out, int32_t *iters_out, float *kernel_ms)
+{
+    GridArgs a{};
+    a.g = gv;
+    a.h = hv;
+    a.gp = S.gp;
+    a.home = S.home;
+    a.hsize = S.hsize;
+    a.pbuf = S.pbuf;
+    a.ctrl = S.ctrl;
+    a.stats = S.stats;
+    a.K = P.K;
+    a.lcap = P.lcap;
+    a.cap = P.cap;
+    a.pcap = P.pcap;
+    a.t = t;
+    a.maxit = P.maxit;
+    a.F0 = F0;
+    a.rr0 = rr0;
+    a.barrier_ticks = 100ull * 1000 * 1000 * 20;  // 20 s of the 100 MHz clock
+    void *args[] = {&a};
+    KernelFn fn = pick(P.B, hv.kind);
+    DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)((size_t)P.lcap * P.B * 8)));
+    // a previous launch that timed out may have left the barrier mid-count
+    DDP_HIP(ctx, hipMemsetAsync(S.ctrl + C_BAR, 0, 8, ctx->stream));
+    DDP_HIP(ctx, hipMemsetAsync(S.ctrl + C_RES_ERR, 0, 4, ctx->stream));
+    DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    DDP_HIP(ctx, hipLaunchCooperativeKernel((const void *)fn, dim3(P.nblocks), dim3(P.B), args,
+                                            (size_t)P.lcap * P.B * 8, ctx->stream));
+    DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    uint32_t res[4];
+    DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + C_RES_IT, 16, hipMemcpyDeviceToHost, ctx->stream));
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    if (iters_out) *iters_out = (int32_t)res[0];
+    if (rr_out) *rr_out = (int32_t)res[1];
+    if (F_out) *F_out = res[2];
+    const uint32_t err = res[3];
+    DDP_CHECK(!(err & ERR_BARRIER), ctx, DDP_ERR_TIMEOUT, "GA*: grid barrier wait timed out");
+    DDP_CHECK(!(err & ERR_ITERS), ctx, DDP_ERR_TIMEOUT, "GA*: iteration limit %d reached", P.maxit);
+    DDP_CHECK(!(err & ERR_HEAP), ctx, DDP_ERR_OVERFLOW,
+              "GA*: a queue exceeded heap_cap=%d entries; raise heap_cap", P.cap);
+    DDP_CHECK(!(err & ERR_PBUF), ctx, DDP_ERR_OVERFLOW, "GA*: push buffer (%d entries) overflowed", P.pcap);
+    return DDP_OK;
+}
+
+extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t s, int32_t t,
+                         const ddp_astar_opts_t *opts, int32_t *cost, int32_t *parent_out,
+                         int32_t *path_out, int32_t path_cap, int32_t *path_len,
+                         ddp_astar_stats_t *stats)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!ctx || !g || !cost) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_astar: null argument");
+    DDP_CHECK(s >= 0 && s < g->N && t >= 0 && t < g->N, ctx, DDP_ERR_ARG,
+              "ddp_astar: s=%d t=%d outside [0,%d)", s, t, g->N);
+    DDP_CHECK(!path_out || path_len, ctx, DDP_ERR_ARG, "ddp_astar: path_out needs path_len");
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    HeurView hv;
+    int rc = ddp_make_heur(g, h, t, &hv);
+    if (rc) return rc;
+    GaGridPlan P;
+    if ((rc = ddp_ga_plan(ctx, g, opts, &P))) return rc;
+    GaGridState S;
+    if ((rc = ddp_ga_alloc(ctx, g, P, &S))) return rc;
+    int32_t hs = 0;
+    if ((rc = ddp_heur_at(g, hv, s, &hs))) return rc;
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->K = P.K;
+        stats->block = P.B;
+        stats->blocks = P.nblocks;
+    }
+    const int fill_blocks = 1024;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(fill_blocks), dim3(256), 0, ctx->stream, S.gp,
+                       (int64_t)g->N, (uint64_t)DDP_GP_INF);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(std::max(1, std::min(fill_blocks, P.K / 256 + 1))), dim3(256), 0,
+                       ctx->stream, S.hsize, (int64_t)P.K, 0);
+    DDP_HIP(ctx, hipMemsetAsync(S.stats, 0, 64, ctx->stream));
+    if (hv.kind == DDP_H_ARRAY && hs < 0) {
+        // h(s) = -1: the heuristic says s cannot reach t.
+        *cost = DDP_INF_COST;
+        if (path_len) *path_len = 0;
+    } else if (s == t) {
+        *cost = hs;
+        if (path_out && path_cap >= 1) {
+            path_out[0] = s;
+            *path_len = 1;
+        } else if (path_len) {
+            *path_len = -1;
+        }
+    } else {
+        hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(64), 0, ctx->stream, S.gp, S.pbuf + P.pcap, S.ctrl, s,
+                           (uint32_t)hs);
+        DDP_HIP(ctx, hipGetLastError());
+        uint32_t F = 0;
+        int32_t iters = 0;
+        float kms = 0;
+        rc = ddp_ga_run(ctx, ddp_view(g), hv, P, S, t, DDP_INF_F, 0, &F, nullptr, &iters, &kms);
+        if (rc) return rc;
+        *cost = (int32_t)F;
+        if (stats) {
+            unsigned long long st[4];
+            DDP_HIP(ctx, hipMemcpy(st, S.stats, 32, hipMemcpyDeviceToHost));
+            stats->expansions = (int64_t)st[0];
+            stats->edges = (int64_t)st[1];
+            stats->improvements = (int64_t)st[2];
+            stats->stale_pops = (int64_t)st[3];
+            stats->iterations = iters;
+            stats->kernel_ms = kms;
+        }
+        if (path_out) {
+            if (F == DDP_INF_F) *path_len = 0;
+            else if ((rc = ddp_extract_path(ctx, S.gp, g->N, s, t, path_out, path_cap, path_len))) return rc;
+        } else if (path_len) {
+            *path_len = 0;
+        }
+    }
+    if (parent_out) {
+        int32_t *dpar = nullptr;
+        void *p;
+        // reuse the push buffer region as a staging area when large enough
+        if ((size_t)P.pcap * 16 >= (size_t)g->N * 4) dpar = (int32_t *)S.pbuf;
+        else {
+            if ((rc = ddp_ws(ctx, WS_PATH, (size_t)g->N * 4 + 256, &p))) return rc;
+            dpar = (int32_t *)p;
+        }
+        hipLaunchKernelGGL(unpack_parent_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.gp, dpar, g->N);
+        DDP_HIP(ctx, hipMemcpyAsync(parent_out, dpar, (size_t)g->N * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->total_ms =
+            std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return DDP_OK;
+}

@@ -1,0 +1,232 @@
+"""GPU parity: reverse BFS, single-query GA* and batched A* through the C ABI
+against the CPU oracle on the same seeded inputs.  Integer results must be
+bit-exact; every returned path must be a real path of the returned cost."""
+import sys
+
+import numpy as np
+import pytest
+
+import ddp_amd
+from helpers import path_cost, random_graph
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def bfs_h_oracle(off, col, w, t, scale):
+    roff, rcol, rw = pyoracle.transpose(off, col, w)
+    hops, lv = pyoracle.bfs_hops(roff, rcol, t)
+    return np.where(hops < 0, -1, hops * scale).astype(np.int32), lv
+
+
+def check_path(off, col, w, path, s, t, g_t):
+    assert len(path) >= 1 and path[0] == s and path[-1] == t
+    assert path_cost(off, col, w, path) == g_t
+
+
+def test_bfs_hops_parity(ctx):
+    rng = np.random.default_rng(101)
+    for k in range(40):
+        N = int(rng.integers(1, 400))
+        off, col, w = random_graph(rng, N, max_deg=int(rng.integers(1, 6)), tombstones=0.05 if k % 4 == 0 else 0)
+        hg = ddp_amd.HostGraph(off, col, w)
+        dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)
+        t = int(rng.integers(0, N))
+        scale = int(rng.integers(1, 4))
+        say("bfs case", k, "N", N, "t", t)
+        want, lv = bfs_h_oracle(off, col, w, t, scale)
+        got, glv = dg.bfs_hops(t, reverse=True, scale=scale)
+        assert np.array_equal(got, want) and glv == lv
+        # reverse=0 on the transposed graph is the reference's own use (bfs_hx on graph_op)
+        roff, rcol, rw = pyoracle.transpose(off, col, w)
+        dgr = ddp_amd.DeviceGraph(ctx, ddp_amd.HostGraph(roff, rcol, rw))
+        got0, _ = dgr.bfs_hops(t, reverse=False, scale=scale)
+        assert np.array_equal(got0, want)
+
+
+def test_bfs_hops_config1(ctx):
+    hg = ddp_amd.HostGraph.random(100000, 6, 1, 100, 1)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)
+    for t in (0, 777, 99999):
+        say("bfs C1 t", t)
+        want, lv = bfs_h_oracle(hg.off, hg.col, hg.w, t, 1)
+        got, glv = dg.bfs_hops(t, reverse=True, scale=1)
+        assert np.array_equal(got, want) and glv == lv
+
+
+@pytest.mark.parametrize("K,block", [(1, 64), (64, 64), (1024, 64), (1024, 256), (4096, 1024)])
+def test_astar_small_graphs(ctx, K, block):
+    rng = np.random.default_rng(1000 + K + block)
+    for k in range(60):
+        N = int(rng.integers(2, 300))
+        off, col, w = random_graph(rng, N, max_deg=int(rng.integers(1, 7)), wmax=int(rng.choice([1, 3, 100])),
+                                   tombstones=0.05 if k % 5 == 0 else 0)
+        s = int(rng.integers(0, N))
+        dist, _ = pyoracle.dijkstra(off, col, w, s)
+        reach = np.flatnonzero(dist >= 0)
+        t = int(rng.choice(reach)) if rng.random() < 0.85 else int(rng.integers(0, N))
+        hg = ddp_amd.HostGraph(off, col, w)
+        dg = ddp_amd.DeviceGraph(ctx, hg)
+        mode = k % 3
+        if mode == 0:
+            h = np.zeros(N, np.int32)
+        else:
+            h, _ = bfs_h_oracle(off, col, w, t, int(w[col >= 0].min(initial=1)) if mode == 1 else 1)
+        dg.set_h(h)
+        say("astar case", K, block, k, "N", N, "s", s, "t", t, "mode", mode)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=K, block=block, want_parent=True)
+        found, cost, parent, _ = pyoracle.astar_ref(off, col, w, h, s, t, quirk=False)
+        if not found or (h[s] < 0):
+            assert r.cost == ddp_amd.INF_COST and len(r.path) == 0
+            continue
+        assert r.cost == cost == dist[t] + h[t]
+        check_path(off, col, w, r.path, s, t, int(dist[t]))
+        # the parent array holds the same chain (reversed)
+        assert pyoracle.chain_from_parent(r.parent, t) == [int(x) for x in r.path[::-1]]
+
+
+def test_astar_config1(ctx):
+    """Config 1: random N=100,000, 6 out-edges, w in [1,100], seed 1,
+    h = reverse-BFS hops x min weight, K = 1024."""
+    hg = ddp_amd.HostGraph.random(100000, 6, 1, 100, 1)
+    src, dst = hg.reachable_pairs(3, 1)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)
+    minw = int(hg.w.min())
+    for s, t in zip(src, dst):
+        s, t = int(s), int(t)
+        h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
+        say("C1 s", s, "t", t)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=1024)
+        found_q, cost_q, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=True)
+        found, cost, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=False)
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s, t)
+        assert found and r.cost == cost == dist[t]
+        if found_q:  # the reference CPU binary's answer (edge E-1 dropped) agrees unless that edge matters
+            assert cost_q >= cost
+        check_path(hg.off, hg.col, hg.w, r.path, s, t, int(dist[t]))
+        say("  stats", r.stats)
+
+
+def test_astar_grid_manhattan(ctx):
+    hg = ddp_amd.HostGraph.grid_maze(128, 96, 300, 2)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg.set_grid(128)
+    s, t = 0, 128 * 96 - 1
+    dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s)
+    assert dist[t] > 0
+    for K in (512, 2048, 8192):
+        say("grid K", K)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_MANHATTAN, K=K)
+        assert r.cost == dist[t]
+        check_path(hg.off, hg.col, hg.w, r.path, s, t, int(dist[t]))
+
+
+def test_astar_euclid_and_zero(ctx):
+    hg = ddp_amd.HostGraph.geometric(20000, 10, 100000, 4)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.geo_pairs(6, 200, 100000, 5)
+    for s, t in zip(src, dst):
+        s, t = int(s), int(t)
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s, t)
+        for kind in (ddp_amd.H_EUCLID, ddp_amd.H_ZERO):
+            say("geo", s, t, kind)
+            r = ddp_amd.astar(ctx, dg, s, t, heur=kind, K=1024)
+            hx = hg.xy[2 * s] - hg.xy[2 * t]
+            assert r.cost == dist[t]
+            check_path(hg.off, hg.col, hg.w, r.path, s, t, int(dist[t]))
+
+
+def test_astar_edge_cases(ctx):
+    off = np.array([0, 1, 2], np.int32)
+    col = np.array([1, 2, 0], np.int32)
+    w = np.array([4, 5, 6], np.uint32)
+    dg = ddp_amd.DeviceGraph(ctx, ddp_amd.HostGraph(off, col, w))
+    r = ddp_amd.astar(ctx, dg, 1, 1, heur=ddp_amd.H_ZERO)  # s == t
+    assert r.cost == 0 and r.path.tolist() == [1]
+    r = ddp_amd.astar(ctx, dg, 2, 0, heur=ddp_amd.H_ZERO)  # uses edge E-1 (no row-end quirk on the GPU)
+    assert r.cost == 6 and r.path.tolist() == [2, 0]
+    dg.set_h(np.array([-1, 0, 0], np.int32))
+    r = ddp_amd.astar(ctx, dg, 0, 2, heur=ddp_amd.H_ARRAY)  # h(s) = -1: cannot reach
+    assert r.cost == ddp_amd.INF_COST
+    with pytest.raises(ddp_amd.DDPError):
+        ddp_amd.astar(ctx, dg, 0, 5, heur=ddp_amd.H_ZERO)
+    iso = ddp_amd.DeviceGraph(ctx, ddp_amd.HostGraph(np.zeros(4, np.int32), np.zeros(0, np.int32),
+                                                      np.zeros(0, np.uint32)))
+    r = ddp_amd.astar(ctx, iso, 0, 3, heur=ddp_amd.H_ZERO)
+    assert r.cost == ddp_amd.INF_COST and len(r.path) == 0
+
+
+def test_batch_parity_geometric(ctx):
+    hg = ddp_amd.HostGraph.geometric(50000, 10, 100000, 4)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.geo_pairs(300, 100, 100000, 6)
+    for block in (64, 256):
+        say("batch block", block)
+        cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=block,
+                                              path_cap=4096)
+        say("  stats", st)
+        for i, (s, t) in enumerate(zip(src, dst)):
+            dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(s), int(t))
+            assert cost[i] == dist[t], (i, s, t)
+            check_path(hg.off, hg.col, hg.w, paths[i], int(s), int(t), int(dist[t]))
+
+
+def test_heap_spill_to_hbm(ctx):
+    """Tiny LDS parts force most heap entries into the HBM spill area."""
+    hg = ddp_amd.HostGraph.geometric(20000, 10, 100000, 14)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.geo_pairs(64, 2000, 100000, 15)
+    want = []
+    for s, t in zip(src, dst):
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(s), int(t))
+        want.append(int(dist[t]))
+    for i in range(0, 64, 16):
+        say("spill single", i)
+        r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_EUCLID, K=256, block=256, lds_cap=1)
+        assert r.cost == want[i]
+    for block, lcap in ((64, 4), (256, 4), (512, 4), (128, 1)):
+        say("spill batch", block, lcap)
+        cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=block, lds_cap=lcap,
+                                              slots=7, path_cap=2048)
+        assert cost.tolist() == want
+        for i in range(64):
+            check_path(hg.off, hg.col, hg.w, paths[i], int(src[i]), int(dst[i]), want[i])
+
+
+def test_batch_same_alone_and_in_the_mix(ctx):
+    hg = ddp_amd.HostGraph.geometric(30000, 10, 100000, 8)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.geo_pairs(200, 100, 100000, 9)
+    mixed, _, _ = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, slots=37)
+    for i in range(0, 200, 20):
+        alone, _, _ = ddp_amd.astar_batch(ctx, dg, src[i:i + 1], dst[i:i + 1], heur=ddp_amd.H_EUCLID)
+        single = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_EUCLID, K=256)
+        assert alone[0] == mixed[i] == single.cost
+
+
+def test_batch_plan_replay(ctx):
+    """The captured hipGraph is replayed several times with host work in
+    between; every replay must give the eager result."""
+    hg = ddp_amd.HostGraph.random(20000, 6, 1, 100, 21)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.reachable_pairs(64, 22)
+    b = ddp_amd.Batch(ctx, dg, 64, heur=ddp_amd.H_ZERO, block=64, path_cap=0)
+    b.upload(src, dst)
+    first = None
+    for rep in range(4):
+        b.launch()
+        st = b.wait()
+        out = b.download()
+        assert st["errors"] == 0
+        if first is None:
+            first = out["cost"].copy()
+            for i in range(0, 64, 8):
+                dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
+                assert first[i] == dist[dst[i]]
+        assert np.array_equal(out["cost"], first)
+    b.close()
