@@ -49,6 +49,9 @@ class Context:
     def _adopt(self, obj):
         self._children.add(obj)
 
+    def synchronize(self):
+        _check(_capi.lib().ddp_ctx_synchronize(self._h), self._h)
+
     def device_info(self):
         info = DeviceInfo()
         _check(_capi.lib().ddp_device_info(self._h, byref(info)), self._h)
@@ -446,6 +449,59 @@ def astar_batch(ctx, dg, src, dst, heur=H_EUCLID, scale=1, block=0, slots=0, hea
         paths = [pb[i * path_cap:i * path_cap + plen[i]].copy() if plen[i] > 0 else np.zeros(0, np.int32)
                  for i in range(n)]
     return cost, paths, {f: getattr(st, f) for f, _ in BatchStats._fields_}
+
+
+class DAStar:
+    """Dynamic A* for one (s, t) query over a graph that receives update
+    batches (ddp_dastar).  apply() returns the new cost and per-batch stats."""
+
+    REPAIR, RECOMPUTE = 0, 1
+
+    def __init__(self, ctx, hg, s, t, h=None, K=0, block=0, heap_cap=0, lds_cap=0):
+        self.ctx, self.N, self.s, self.t = ctx, hg.N, s, t
+        o = AstarOpts(K, block, heap_cap, lds_cap, 0)
+        hh = None if h is None else as_i32(h)
+        handle = c_void_p()
+        cost = c_int32()
+        _check(_capi.lib().ddp_dastar_open(ctx.handle, hg.N, hg.E, ptr_i32(hg.off), ptr_i32(hg.col), ptr_u32(hg.w),
+                                           ptr_i32(hh), s, t, byref(o), byref(handle), byref(cost)), ctx.handle)
+        self._h = handle
+        ctx._adopt(self)
+        self.cost = cost.value
+
+    def apply(self, lines, mode=0):
+        arr = _updates_array(lines)
+        cost = c_int32()
+        st = _capi.DAStats()
+        _check(_capi.lib().ddp_dastar_apply(self._h, arr, len(lines), mode, byref(cost), byref(st)), self.ctx.handle)
+        self.cost = cost.value
+        return cost.value, {f: getattr(st, f) for f, _ in _capi.DAStats._fields_}
+
+    def path(self):
+        buf = np.zeros(self.N, np.int32)
+        n = c_int32()
+        _check(_capi.lib().ddp_dastar_path(self._h, ptr_i32(buf), self.N, byref(n)), self.ctx.handle)
+        return buf[:max(n.value, 0)].copy()
+
+    def graph(self):
+        E = c_int32()
+        _check(_capi.lib().ddp_dastar_num_edges(self._h, byref(E)))
+        off = np.zeros(self.N, np.int32)
+        col = np.zeros(max(E.value, 1), np.int32)
+        w = np.zeros(max(E.value, 1), np.uint32)
+        _check(_capi.lib().ddp_dastar_export(self._h, ptr_i32(off), ptr_i32(col), ptr_u32(w)))
+        return HostGraph(off, col[:E.value], w[:E.value])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.lib().ddp_dastar_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------- facade of the reference library

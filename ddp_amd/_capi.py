@@ -67,6 +67,13 @@ class Update(ctypes.Structure):
     _fields_ = [("flag", c_int32), ("u", c_int32), ("v", c_int32), ("w", c_uint32)]
 
 
+class DAStats(ctypes.Structure):
+    _fields_ = [("inserts", c_int32), ("deletes", c_int32), ("deletes_effective", c_int32), ("roots", c_int32),
+                ("invalidated", c_int32), ("repulled", c_int32), ("pushes", c_int32), ("iterations", c_int32),
+                ("expansions", c_int64), ("edges", c_int64), ("improvements", c_int64), ("host_ms", c_float),
+                ("repair_ms", c_float), ("search_ms", c_float), ("total_ms", c_float)]
+
+
 P_I32 = POINTER(c_int32)
 P_U32 = POINTER(c_uint32)
 P_F64 = POINTER(c_double)
@@ -80,6 +87,7 @@ SIGNATURES = {
     "ddp_last_error": (c_char_p, [c_void_p]),
     "ddp_create_error": (c_char_p, []),
     "ddp_device_info": (c_int, [c_void_p, POINTER(DeviceInfo)]),
+    "ddp_ctx_synchronize": (c_int, [c_void_p]),
     "ddp_graph_upload": (c_int, [c_void_p, c_int32, c_int32, P_I32, P_I32, P_U32, c_uint32, PP]),
     "ddp_graph_destroy": (None, [c_void_p]),
     "ddp_graph_info": (c_int, [c_void_p, P_I32, P_I32, P_U32, P_I32]),
@@ -99,6 +107,14 @@ SIGNATURES = {
     "ddp_batch_download": (c_int, [c_void_p, P_I32, P_I32, P_I32, P_U32, P_F64]),
     "ddp_astar_batch": (c_int, [c_void_p, c_void_p, Heur, c_int32, P_I32, P_I32, POINTER(BatchOpts), P_I32,
                                 P_I32, P_I32, POINTER(BatchStats)]),
+    # ddp_dastar.h
+    "ddp_dastar_open": (c_int, [c_void_p, c_int32, c_int32, P_I32, P_I32, P_U32, P_I32, c_int32, c_int32,
+                                POINTER(AstarOpts), PP, P_I32]),
+    "ddp_dastar_apply": (c_int, [c_void_p, POINTER(Update), c_int32, c_int32, P_I32, POINTER(DAStats)]),
+    "ddp_dastar_path": (c_int, [c_void_p, P_I32, c_int32, P_I32]),
+    "ddp_dastar_num_edges": (c_int, [c_void_p, P_I32]),
+    "ddp_dastar_export": (c_int, [c_void_p, P_I32, P_I32, P_U32]),
+    "ddp_dastar_close": (None, [c_void_p]),
     # ddp_io.h
     "ddp_hgraph_from_arrays": (c_int, [c_int32, c_int32, P_I32, P_I32, P_U32, PP]),
     "ddp_hgraph_free": (None, [c_void_p]),

@@ -147,7 +147,12 @@ extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t rever
               100ull * 1000 * 1000 * 20};
     void *args[] = {&a};
     DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    DDP_HIP(ctx, hipLaunchCooperativeKernel((const void *)bfs_kernel<B>, dim3(grid), dim3(B), args, 0, ctx->stream));
+    // Ordinary launch: the grid barrier is software and only needs every
+    // workgroup resident at once, which the occupancy-sized grid gives on an
+    // idle device (a cooperative launch would also reserve the GPU's global
+    // wave-sync unit, which only one process may hold).  Every barrier wait is
+    // bounded, so a non-resident grid ends with DDP_ERR_TIMEOUT, not a hang.
+    DDP_HIP(ctx, hipLaunchKernel((const void *)bfs_kernel<B>, dim3(grid), dim3(B), args, 0, ctx->stream));
     DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t res[2];
     DDP_HIP(ctx, hipMemcpyAsync(res, ctrl + BC_LEVELS, 8, hipMemcpyDeviceToHost, ctx->stream));

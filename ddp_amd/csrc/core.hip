@@ -116,6 +116,14 @@ extern "C" int ddp_device_info(ddp_ctx *ctx, ddp_device_info_t *info)
     return DDP_OK;
 }
 
+extern "C" int ddp_ctx_synchronize(ddp_ctx *ctx)
+{
+    if (!ctx) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_ctx_synchronize: null context");
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    DDP_HIP(ctx, hipDeviceSynchronize());
+    return DDP_OK;
+}
+
 // ------------------------------------------------------------------ graph
 
 extern "C" void ddp_graph_destroy(ddp_graph *g)

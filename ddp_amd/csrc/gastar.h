@@ -1,0 +1,52 @@
+// Host-side interface of the single-query GA* grid kernel, shared by the
+// entry points that run it (ddp_astar, DA* resume).
+#pragma once
+
+#include "ddp_internal.h"
+
+// Control block layout (uint32 words).  Three rotating slots carry, per GA*
+// iteration, the push count, the smallest queue top, the smallest pushed f,
+// the target's best f and error bits.  Slot 2 plays "iteration -1": whatever
+// is pushed there before a launch is drained into the queues at iteration 0.
+enum : int {
+    GA_SLOT_WORDS = 8,
+    GA_S_PUSHCNT = 0, GA_S_TOPMIN = 1, GA_S_PUSHMIN = 2, GA_S_TMIN = 3, GA_S_ERR = 4,
+    GA_RES_IT = 32, GA_RES_RR = 33, GA_RES_F = 34, GA_RES_ERR = 35,
+    GA_BAR = 64,
+    GA_CTRL_WORDS = 96,
+};
+
+struct GaGridPlan {
+    int32_t K, B, nblocks, lcap, cap, pcap, maxit;
+};
+
+// Device buffers of one GA* search state.
+struct GaGridState {
+    uint64_t *gp = nullptr;      // N packed (f << 32 | parent)
+    uint64_t *home = nullptr;    // cap * K heap entries
+    int32_t *hsize = nullptr;    // K
+    uint64_t *pbuf = nullptr;    // 2 * pcap push entries
+    uint32_t *ctrl = nullptr;    // GA_CTRL_WORDS
+    unsigned long long *stats = nullptr;  // 8 counters after ctrl
+    bool owned = false;          // allocated by ddp_ga_alloc_owned (freed by ddp_ga_free)
+};
+
+// min_pcap: lower bound on the push buffer (DA* repairs push before a launch).
+int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p,
+                int64_t min_pcap = 0);
+// State in the context's reusable scratch (valid until the next ddp_astar on ctx).
+int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridState *S);
+// State owned by the caller (DA*), freed by ddp_ga_free.
+int ddp_ga_alloc_owned(ddp_ctx *ctx, int32_t N, const GaGridPlan &P, GaGridState *S);
+void ddp_ga_free(GaGridState *S);
+// gp = INF everywhere, queues empty, counters zero.
+int ddp_ga_reset(ddp_ctx *ctx, int32_t N, const GaGridPlan &P, const GaGridState &S);
+// Fresh search from s: gp[s] = (h(s), -1) and s as the only pending push.
+int ddp_ga_seed(ddp_ctx *ctx, const GaGridPlan &P, const GaGridState &S, int32_t s, uint32_t hs);
+// Clears the three slots before repair kernels push into slot 2 / buffer 1.
+int ddp_ga_prepare_resume(ddp_ctx *ctx, const GaGridState &S);
+int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+               const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
+               int32_t *rr_out, int32_t *iters_out, float *kernel_ms);
+// Reads the four search counters (expanded, edges, improved, stale).
+int ddp_ga_stats(ddp_ctx *ctx, const GaGridState &S, unsigned long long out[4]);

@@ -25,18 +25,10 @@
 #include <chrono>
 #include <cstring>
 
+#include "gastar.h"
 #include "search_expand.h"
 
 namespace {
-
-// Control block layout (uint32 words).
-enum : int {
-    SLOT_WORDS = 8,          // per slot: pushcnt, topmin, pushmin, tmin, err
-    S_PUSHCNT = 0, S_TOPMIN = 1, S_PUSHMIN = 2, S_TMIN = 3, S_ERR = 4,
-    C_RES_IT = 32, C_RES_RR = 33, C_RES_F = 34, C_RES_ERR = 35,
-    C_BAR = 64,              // bar[0] count, bar[1] generation (own cache line)
-    C_WORDS = 96,
-};
 
 enum : uint32_t { ERR_HEAP = 1, ERR_PBUF = 2, ERR_ITERS = 4, ERR_BARRIER = 8 };
 
@@ -47,7 +39,7 @@ struct GridArgs {
     uint64_t *home;      // heap spill / home: entry i of queue q at home[i*K + q]
     int32_t *hsize;      // K
     uint64_t *pbuf;      // 2 * pcap
-    uint32_t *ctrl;      // C_WORDS
+    uint32_t *ctrl;      // GA_CTRL_WORDS
     unsigned long long *stats;  // expanded, edges, improved, stale
     int32_t K, lcap, cap, pcap, t, maxit;
     uint32_t F0;
@@ -100,7 +92,7 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
     int32_t size = a.hsize[q];
     for (int32_t i = 0; i < size && i < a.lcap; ++i) s_heap[i * B + lane] = a.home[(int64_t)i * K + q];
 
-    uint32_t gen = ld_u32(ctrl + C_BAR + 1);
+    uint32_t gen = ld_u32(ctrl + GA_BAR + 1);
     uint32_t F = a.F0;
     int32_t rr = a.rr0;
     uint32_t err_local = 0, err_seen = 0;
@@ -110,11 +102,11 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
     for (;; ++it) {
         // -- drain the previous iteration's pushes into this lane's queue
         const int ps = (it + 2) % 3;
-        const uint32_t n = ld_u32(ctrl + ps * SLOT_WORDS + S_PUSHCNT);
-        const uint32_t topm = ld_u32(ctrl + ps * SLOT_WORDS + S_TOPMIN);
-        const uint32_t pm = ld_u32(ctrl + ps * SLOT_WORDS + S_PUSHMIN);
-        const uint32_t tm = ld_u32(ctrl + ps * SLOT_WORDS + S_TMIN);
-        err_seen |= ld_u32(ctrl + ps * SLOT_WORDS + S_ERR);
+        const uint32_t n = ld_u32(ctrl + ps * GA_SLOT_WORDS + GA_S_PUSHCNT);
+        const uint32_t topm = ld_u32(ctrl + ps * GA_SLOT_WORDS + GA_S_TOPMIN);
+        const uint32_t pm = ld_u32(ctrl + ps * GA_SLOT_WORDS + GA_S_PUSHMIN);
+        const uint32_t tm = ld_u32(ctrl + ps * GA_SLOT_WORDS + GA_S_TMIN);
+        err_seen |= ld_u32(ctrl + ps * GA_SLOT_WORDS + GA_S_ERR);
         F = tm < F ? tm : F;
         const uint32_t navail = n < (uint32_t)a.pcap ? n : (uint32_t)a.pcap;
         const uint64_t *buf = a.pbuf + (int64_t)((it + 1) & 1) * a.pcap;
@@ -133,17 +125,17 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
         }
         const int cs = it % 3;
         if (blockIdx.x == 0 && lane == 0) {
-            uint32_t *ns = ctrl + ((it + 1) % 3) * SLOT_WORDS;
-            st_u32(ns + S_PUSHCNT, 0);
-            st_u32(ns + S_TOPMIN, 0xFFFFFFFFu);
-            st_u32(ns + S_PUSHMIN, 0xFFFFFFFFu);
-            st_u32(ns + S_TMIN, 0xFFFFFFFFu);
-            st_u32(ns + S_ERR, 0);
+            uint32_t *ns = ctrl + ((it + 1) % 3) * GA_SLOT_WORDS;
+            st_u32(ns + GA_S_PUSHCNT, 0);
+            st_u32(ns + GA_S_TOPMIN, 0xFFFFFFFFu);
+            st_u32(ns + GA_S_PUSHMIN, 0xFFFFFFFFu);
+            st_u32(ns + GA_S_TMIN, 0xFFFFFFFFu);
+            st_u32(ns + GA_S_ERR, 0);
         }
         // -- publish the smallest queue top (stale tops only make this smaller)
         const uint32_t mytop = size > 0 ? (uint32_t)(s_heap[lane] >> 32) : 0xFFFFFFFFu;
         const uint32_t bmin = block_min_u32<B>(mytop, s_red);
-        if (lane == 0 && bmin != 0xFFFFFFFFu) atomicMin(ctrl + cs * SLOT_WORDS + S_TOPMIN, bmin);
+        if (lane == 0 && bmin != 0xFFFFFFFFu) atomicMin(ctrl + cs * GA_SLOT_WORDS + GA_S_TOPMIN, bmin);
         // -- pop the best entry whose f is still current
         int32_t u = -1;
         uint32_t fu = 0;
@@ -160,30 +152,30 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
             ++stale;
         }
         // -- expand and push
-        GridPush<B> push{a.pbuf + (int64_t)(it & 1) * a.pcap, ctrl + cs * SLOT_WORDS + S_PUSHCNT, a.pcap};
+        GridPush<B> push{a.pbuf + (int64_t)(it & 1) * a.pcap, ctrl + cs * GA_SLOT_WORDS + GA_S_PUSHCNT, a.pcap};
         expand_block<B, HK>(a.g, a.h, a.gp, a.t, u, fu, L, C, push);
         err_local |= push.err;
         const uint32_t wt = wave_min_u32(push.tmin);
         const uint32_t wp = wave_min_u32(push.pushmin);
         if ((lane & 63) == 0) {
-            if (wt != 0xFFFFFFFFu) atomicMin(ctrl + cs * SLOT_WORDS + S_TMIN, wt);
-            if (wp != 0xFFFFFFFFu) atomicMin(ctrl + cs * SLOT_WORDS + S_PUSHMIN, wp);
+            if (wt != 0xFFFFFFFFu) atomicMin(ctrl + cs * GA_SLOT_WORDS + GA_S_TMIN, wt);
+            if (wp != 0xFFFFFFFFu) atomicMin(ctrl + cs * GA_SLOT_WORDS + GA_S_PUSHMIN, wp);
         }
-        if (err_local) atomicOr(ctrl + cs * SLOT_WORDS + S_ERR, err_local);
-        if (!grid_barrier(ctrl + C_BAR, gridDim.x, gen, a.barrier_ticks, &s_flag)) {
-            atomicOr(ctrl + C_RES_ERR, (uint32_t)ERR_BARRIER);
+        if (err_local) atomicOr(ctrl + cs * GA_SLOT_WORDS + GA_S_ERR, err_local);
+        if (!grid_barrier(ctrl + GA_BAR, gridDim.x, gen, a.barrier_ticks, &s_flag)) {
+            atomicOr(ctrl + GA_RES_ERR, (uint32_t)ERR_BARRIER);
             break;
         }
     }
     // -- persist queues for a later resume (DA*) and report
     for (int32_t i = 0; i < size && i < a.lcap; ++i) a.home[(int64_t)i * K + q] = s_heap[i * B + lane];
     a.hsize[q] = size;
-    if (err_local) atomicOr(ctrl + C_RES_ERR, err_local);
-    if (err_seen) atomicOr(ctrl + C_RES_ERR, err_seen);
+    if (err_local) atomicOr(ctrl + GA_RES_ERR, err_local);
+    if (err_seen) atomicOr(ctrl + GA_RES_ERR, err_seen);
     if (blockIdx.x == 0 && lane == 0) {
-        ctrl[C_RES_IT] = (uint32_t)it;
-        ctrl[C_RES_RR] = (uint32_t)rr;
-        ctrl[C_RES_F] = F;
+        ctrl[GA_RES_IT] = (uint32_t)it;
+        ctrl[GA_RES_RR] = (uint32_t)rr;
+        ctrl[GA_RES_F] = F;
     }
     uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
              v3 = wave_sum_u64(stale);
@@ -214,21 +206,21 @@ __global__ void seed_kernel(uint64_t *gp, uint64_t *pbuf1, uint32_t *ctrl, int32
     gp[s] = ((uint64_t)hs << 32) | 0xFFFFFFFFull;
     pbuf1[0] = ((uint64_t)hs << 32) | (uint32_t)s;
     for (int k = 0; k < 3; ++k) {
-        uint32_t *sl = ctrl + k * SLOT_WORDS;
-        sl[S_PUSHCNT] = 0;
-        sl[S_TOPMIN] = 0xFFFFFFFFu;
-        sl[S_PUSHMIN] = 0xFFFFFFFFu;
-        sl[S_TMIN] = 0xFFFFFFFFu;
-        sl[S_ERR] = 0;
+        uint32_t *sl = ctrl + k * GA_SLOT_WORDS;
+        sl[GA_S_PUSHCNT] = 0;
+        sl[GA_S_TOPMIN] = 0xFFFFFFFFu;
+        sl[GA_S_PUSHMIN] = 0xFFFFFFFFu;
+        sl[GA_S_TMIN] = 0xFFFFFFFFu;
+        sl[GA_S_ERR] = 0;
     }
-    ctrl[2 * SLOT_WORDS + S_PUSHCNT] = 1;
-    ctrl[2 * SLOT_WORDS + S_PUSHMIN] = hs;
-    ctrl[C_RES_IT] = 0;
-    ctrl[C_RES_RR] = 0;
-    ctrl[C_RES_F] = DDP_INF_F;
-    ctrl[C_RES_ERR] = 0;
-    ctrl[C_BAR] = 0;
-    ctrl[C_BAR + 1] = 0;
+    ctrl[2 * GA_SLOT_WORDS + GA_S_PUSHCNT] = 1;
+    ctrl[2 * GA_SLOT_WORDS + GA_S_PUSHMIN] = hs;
+    ctrl[GA_RES_IT] = 0;
+    ctrl[GA_RES_RR] = 0;
+    ctrl[GA_RES_F] = DDP_INF_F;
+    ctrl[GA_RES_ERR] = 0;
+    ctrl[GA_BAR] = 0;
+    ctrl[GA_BAR + 1] = 0;
 }
 
 __global__ void unpack_parent_kernel(const uint64_t *gp, int32_t *par, int32_t N)
@@ -261,12 +253,7 @@ KernelFn pick(int B, int hk)
 
 }  // namespace
 
-// Shared with other translation units (DA* resume).
-struct GaGridPlan {
-    int32_t K, B, nblocks, lcap, cap, pcap, maxit;
-};
-
-int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p)
+int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p, int64_t min_pcap)
 {
     int32_t B = (o && o->block) ? o->block : 0;
     int32_t K = (o && o->K) ? o->K : 1024;
@@ -284,7 +271,7 @@ int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaG
     DDP_CHECK(cap < (1 << 30), ctx, DDP_ERR_ARG, "heap_cap too large");
     int64_t pcap = (int64_t)K * std::max(1, g->maxdeg) + K;
     pcap = std::min<int64_t>(pcap, (int64_t)g->E + 2 * (int64_t)K + 64);
-    pcap = std::max<int64_t>(pcap, 1024);
+    pcap = std::max<int64_t>(pcap, std::max<int64_t>(1024, min_pcap));
     DDP_CHECK(pcap < (1LL << 31), ctx, DDP_ERR_ARG, "push buffer too large");
     int32_t maxit = (o && o->max_iters) ? o->max_iters : (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
     *p = GaGridPlan{K, B, nblocks, lcap, (int32_t)cap, (int32_t)pcap, maxit};
@@ -300,14 +287,6 @@ int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaG
     return DDP_OK;
 }
 
-// Buffers of one GA* search state.
-struct GaGridState {
-    uint64_t *gp, *home, *pbuf;
-    int32_t *hsize;
-    uint32_t *ctrl;
-    unsigned long long *stats;
-};
-
 int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridState *S)
 {
     void *p;
@@ -320,9 +299,85 @@ int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridSt
     S->hsize = (int32_t *)p;
     if ((rc = ddp_ws(ctx, WS_ASTAR_PBUF, (size_t)P.pcap * 16, &p))) return rc;
     S->pbuf = (uint64_t *)p;
-    if ((rc = ddp_ws(ctx, WS_ASTAR_CTRL, C_WORDS * 4 + 64, &p))) return rc;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_CTRL, GA_CTRL_WORDS * 4 + 64, &p))) return rc;
     S->ctrl = (uint32_t *)p;
-    S->stats = (unsigned long long *)((char *)p + C_WORDS * 4);
+    S->stats = (unsigned long long *)((char *)p + GA_CTRL_WORDS * 4);
+    S->owned = false;
+    return DDP_OK;
+}
+
+int ddp_ga_alloc_owned(ddp_ctx *ctx, int32_t N, const GaGridPlan &P, GaGridState *S)
+{
+    *S = GaGridState{};
+    S->owned = true;
+    hipError_t e = hipSuccess;
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 64));
+    };
+    al((void **)&S->gp, (size_t)N * 8);
+    al((void **)&S->home, (size_t)P.cap * P.K * 8);
+    al((void **)&S->hsize, (size_t)P.K * 4);
+    al((void **)&S->pbuf, (size_t)P.pcap * 16);
+    al((void **)&S->ctrl, GA_CTRL_WORDS * 4 + 64);
+    if (e != hipSuccess) {
+        ddp_ga_free(S);
+        return ddp_hip_err(ctx, e, "GA* state allocation", __FILE__, __LINE__);
+    }
+    S->stats = (unsigned long long *)((char *)S->ctrl + GA_CTRL_WORDS * 4);
+    return DDP_OK;
+}
+
+void ddp_ga_free(GaGridState *S)
+{
+    if (!S->owned) return;
+    void *ptrs[] = {S->gp, S->home, S->hsize, S->pbuf, S->ctrl};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    *S = GaGridState{};
+}
+
+int ddp_ga_reset(ddp_ctx *ctx, int32_t N, const GaGridPlan &P, const GaGridState &S)
+{
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.gp, (int64_t)N,
+                       (uint64_t)DDP_GP_INF);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(std::max(1, std::min(1024, P.K / 256 + 1))), dim3(256), 0,
+                       ctx->stream, S.hsize, (int64_t)P.K, 0);
+    DDP_HIP(ctx, hipMemsetAsync(S.stats, 0, 64, ctx->stream));
+    DDP_HIP(ctx, hipGetLastError());
+    return DDP_OK;
+}
+
+int ddp_ga_seed(ddp_ctx *ctx, const GaGridPlan &P, const GaGridState &S, int32_t s, uint32_t hs)
+{
+    hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(64), 0, ctx->stream, S.gp, S.pbuf + P.pcap, S.ctrl, s, hs);
+    DDP_HIP(ctx, hipGetLastError());
+    return DDP_OK;
+}
+
+__global__ void prepare_resume_kernel(uint32_t *ctrl)
+{
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t *sl = ctrl + k * GA_SLOT_WORDS;
+        sl[GA_S_PUSHCNT] = 0;
+        sl[GA_S_TOPMIN] = 0xFFFFFFFFu;
+        sl[GA_S_PUSHMIN] = 0xFFFFFFFFu;
+        sl[GA_S_TMIN] = 0xFFFFFFFFu;
+        sl[GA_S_ERR] = 0;
+    }
+}
+
+int ddp_ga_prepare_resume(ddp_ctx *ctx, const GaGridState &S)
+{
+    hipLaunchKernelGGL(prepare_resume_kernel, dim3(1), dim3(64), 0, ctx->stream, S.ctrl);
+    DDP_HIP(ctx, hipGetLastError());
+    return DDP_OK;
+}
+
+int ddp_ga_stats(ddp_ctx *ctx, const GaGridState &S, unsigned long long out[4])
+{
+    DDP_HIP(ctx, hipMemcpyAsync(out, S.stats, 32, hipMemcpyDeviceToHost, ctx->stream));
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return DDP_OK;
 }
 
@@ -354,14 +409,17 @@ int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGr
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)((size_t)P.lcap * P.B * 8)));
     // a previous launch that timed out may have left the barrier mid-count
-    DDP_HIP(ctx, hipMemsetAsync(S.ctrl + C_BAR, 0, 8, ctx->stream));
-    DDP_HIP(ctx, hipMemsetAsync(S.ctrl + C_RES_ERR, 0, 4, ctx->stream));
+    DDP_HIP(ctx, hipMemsetAsync(S.ctrl + GA_BAR, 0, 8, ctx->stream));
+    DDP_HIP(ctx, hipMemsetAsync(S.ctrl + GA_RES_ERR, 0, 4, ctx->stream));
     DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    DDP_HIP(ctx, hipLaunchCooperativeKernel((const void *)fn, dim3(P.nblocks), dim3(P.B), args,
-                                            (size_t)P.lcap * P.B * 8, ctx->stream));
+    // Ordinary launch: ddp_ga_plan sized the grid to what the device holds at
+    // once and the grid barrier is software with bounded waits (a cooperative
+    // launch would also reserve the global wave-sync unit, one process only).
+    DDP_HIP(ctx, hipLaunchKernel((const void *)fn, dim3(P.nblocks), dim3(P.B), args, (size_t)P.lcap * P.B * 8,
+                                 ctx->stream));
     DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t res[4];
-    DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + C_RES_IT, 16, hipMemcpyDeviceToHost, ctx->stream));
+    DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + GA_RES_IT, 16, hipMemcpyDeviceToHost, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -404,12 +462,7 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
         stats->block = P.B;
         stats->blocks = P.nblocks;
     }
-    const int fill_blocks = 1024;
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(fill_blocks), dim3(256), 0, ctx->stream, S.gp,
-                       (int64_t)g->N, (uint64_t)DDP_GP_INF);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(std::max(1, std::min(fill_blocks, P.K / 256 + 1))), dim3(256), 0,
-                       ctx->stream, S.hsize, (int64_t)P.K, 0);
-    DDP_HIP(ctx, hipMemsetAsync(S.stats, 0, 64, ctx->stream));
+    if ((rc = ddp_ga_reset(ctx, g->N, P, S))) return rc;
     if (hv.kind == DDP_H_ARRAY && hs < 0) {
         // h(s) = -1: the heuristic says s cannot reach t.
         *cost = DDP_INF_COST;
@@ -423,9 +476,7 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
             *path_len = -1;
         }
     } else {
-        hipLaunchKernelGGL(seed_kernel, dim3(1), dim3(64), 0, ctx->stream, S.gp, S.pbuf + P.pcap, S.ctrl, s,
-                           (uint32_t)hs);
-        DDP_HIP(ctx, hipGetLastError());
+        if ((rc = ddp_ga_seed(ctx, P, S, s, (uint32_t)hs))) return rc;
         uint32_t F = 0;
         int32_t iters = 0;
         float kms = 0;
@@ -434,7 +485,7 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
         *cost = (int32_t)F;
         if (stats) {
             unsigned long long st[4];
-            DDP_HIP(ctx, hipMemcpy(st, S.stats, 32, hipMemcpyDeviceToHost));
+            if ((rc = ddp_ga_stats(ctx, S, st))) return rc;
             stats->expansions = (int64_t)st[0];
             stats->edges = (int64_t)st[1];
             stats->improvements = (int64_t)st[2];

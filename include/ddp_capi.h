@@ -65,6 +65,9 @@ typedef struct {
     char arch[64];
 } ddp_device_info_t;
 int ddp_device_info(ddp_ctx *ctx, ddp_device_info_t *info);
+/* Waits for all work on the context's device (the reference's
+ * cudaDeviceSynchronize after each launch, GA_star/GA_v2.cu:495). */
+int ddp_ctx_synchronize(ddp_ctx *ctx);
 
 /* -------------------------------------------------------------------- graph */
 

@@ -1,0 +1,139 @@
+"""GPU parity of DA*: after every update batch the repaired cost must equal a
+from-scratch search on the updated graph (oracle Dijkstra on the oracle's own
+application of the batch), the returned path must be a path of that cost in
+the updated graph, and the library's graph version must equal the oracle's
+edge for edge."""
+import sys
+
+import numpy as np
+import pytest
+
+import ddp_amd
+from helpers import full_offsets, path_cost
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def union_h(hg, batches, t):
+    """Reverse-BFS hops x min weight on base + every insert: admissible for
+    every graph version of the run."""
+    o = full_offsets(hg.off, hg.E)
+    src = list(np.repeat(np.arange(hg.N), np.diff(o)))
+    dst = list(hg.col)
+    ws = list(hg.w)
+    for b in batches:
+        for f, u, v, w in b:
+            if f == 1:
+                src.append(u)
+                dst.append(v)
+                ws.append(w)
+    src, dst, ws = np.array(src), np.array(dst), np.array(ws, np.uint32)
+    order = np.argsort(src, kind="stable")
+    off = np.searchsorted(src[order], np.arange(hg.N)).astype(np.int32)
+    roff, rcol, rw = pyoracle.transpose(off, dst[order], ws[order])
+    hops, _ = pyoracle.bfs_hops(roff, rcol, t)
+    minw = int(ws.min())
+    return np.where(hops < 0, -1, hops * minw).astype(np.int32)
+
+
+def run_batches(ctx, hg, batches, s, t, h, K=1024):
+    da = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=K)
+    rc = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=K)
+    cur = (hg.off, hg.col, hg.w)
+    d0, _ = pyoracle.dijkstra(*cur, s, t)
+    assert da.cost == (d0[t] if d0[t] >= 0 else ddp_amd.INF_COST)
+    for bi, lines in enumerate(batches):
+        say("batch", bi, "lines", len(lines))
+        cost, st = da.apply(lines, mode=ddp_amd.DAStar.REPAIR)
+        cost_rc, _ = rc.apply(lines, mode=ddp_amd.DAStar.RECOMPUTE)
+        o2, c2, w2, nd = pyoracle.apply_batch(*cur, lines)
+        cur = (o2, c2, w2)
+        assert st["deletes_effective"] == nd
+        g = da.graph()
+        assert g.off.tolist() == o2.tolist() and g.col.tolist() == c2.tolist() and g.w.tolist() == w2.tolist()
+        dist, _ = pyoracle.dijkstra(*cur, s, t)
+        want = int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST
+        say("  cost", cost, "want", want, "stats", st)
+        assert cost == cost_rc == want
+        if want != ddp_amd.INF_COST:
+            p = da.path()
+            assert p[0] == s and p[-1] == t
+            assert path_cost(*cur, p) == want
+    da.close()
+    rc.close()
+
+
+def test_dastar_random_batches(ctx):
+    hg = ddp_amd.HostGraph.random(20000, 8, 1, 1000, 3)
+    upd = hg.gen_updates(8, 300, 300, 1, 1000, 4)
+    src, dst = hg.reachable_pairs(1, 5)
+    s, t = int(src[0]), int(dst[0])
+    h = union_h(hg, upd.batches, t)
+    run_batches(ctx, hg, upd.batches, s, t, h)
+
+
+def test_dastar_zero_heuristic_and_small_K(ctx):
+    hg = ddp_amd.HostGraph.random(5000, 4, 1, 50, 7)
+    upd = hg.gen_updates(6, 150, 200, 1, 50, 8)
+    src, dst = hg.reachable_pairs(1, 9)
+    run_batches(ctx, hg, upd.batches, int(src[0]), int(dst[0]), None, K=64)
+
+
+def test_dastar_deletes_on_the_path(ctx):
+    """Each batch deletes edges of the current shortest path (forcing subtree
+    invalidation and re-pulls) and inserts a few random shortcuts."""
+    rng = np.random.default_rng(11)
+    hg = ddp_amd.HostGraph.random(8000, 5, 1, 100, 12)
+    src, dst = hg.reachable_pairs(1, 13)
+    s, t = int(src[0]), int(dst[0])
+    batches = []
+    cur = (hg.off, hg.col, hg.w)
+    for b in range(10):
+        dist, _ = pyoracle.dijkstra(*cur, s)
+        lines = []
+        if dist[t] >= 0:
+            # rebuild one shortest path from the distances
+            o = full_offsets(cur[0], len(cur[1]))
+            path = [t]
+            while path[-1] != s:
+                v = path[-1]
+                for u in range(len(cur[0])):
+                    hit = False
+                    for j in range(o[u], o[u + 1]):
+                        if cur[1][j] == v and dist[u] >= 0 and dist[u] + int(cur[2][j]) == dist[v]:
+                            path.append(u)
+                            hit = True
+                            break
+                    if hit:
+                        break
+            path = path[::-1]
+            for k in rng.choice(len(path) - 1, size=min(3, len(path) - 1), replace=False):
+                lines.append((0, int(path[k]), int(path[k + 1]), 0))
+        lines.append((0, 1, 2, 0))  # most likely absent: a no-op delete
+        for _ in range(20):
+            lines.append((1, int(rng.integers(0, hg.N)), int(rng.integers(0, hg.N)), int(rng.integers(1, 100))))
+        batches.append(lines)
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
+    run_batches(ctx, hg, batches, s, t, None)
+
+
+def test_dastar_target_cut_off_and_reconnected(ctx):
+    # chain 0 -> 1 -> 2 -> 3 plus a detour 0 -> 4 -> 3
+    off = np.array([0, 2, 3, 4, 4], np.int32)
+    col = np.array([1, 4, 2, 3, 3], np.int32)
+    w = np.array([1, 5, 1, 1, 5], np.uint32)
+    hg = ddp_amd.HostGraph(off, col, w)
+    batches = [
+        [(0, 1, 2, 0)],                    # forces the detour: cost 10
+        [(0, 0, 4, 0)],                    # t unreachable
+        [(1, 2, 3, 7), (1, 1, 2, 1)],      # reconnected through a new 1->2: cost 3
+        [(1, 0, 3, 4)],                    # a direct 0->3 of weight 4 does not beat 3
+        [(0, 2, 3, 0), (0, 2, 3, 0)],      # both parallel 2->3 edges gone: direct edge, cost 4
+    ]
+    run_batches(ctx, hg, batches, 0, 3, None, K=64)

@@ -1,0 +1,83 @@
+"""The command-line drivers honour the reference's process contract: stdin
+arguments, graph.txt / Hx.txt / Updates.txt in the working directory, the
+reference's stdout grammar.  Results are checked against the oracle and the
+reference's own A_star_cpu where it is built."""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+import ddp_amd
+from helpers import path_cost
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ddp_amd", "bin")
+
+
+def run(exe, cwd, stdin, *args):
+    print("run", exe, args, flush=True)
+    sys.stdout.flush()
+    return subprocess.run([os.path.join(BIN, exe), *args], input=stdin, capture_output=True, text=True, cwd=cwd,
+                          timeout=120, check=True).stdout
+
+
+def test_bfs_hx_and_ga_star_contract():
+    hg = ddp_amd.HostGraph.random(3000, 4, 1, 30, 21)
+    src, dst = hg.reachable_pairs(4, 22)
+    with tempfile.TemporaryDirectory() as d:
+        for s, t in zip(src, dst):
+            s, t = int(s), int(t)
+            # bfs_hx reads the transposed graph as graph.txt (GA_star/bfs_hx.cu:45)
+            roff, rcol, rw = pyoracle.transpose(hg.off, hg.col, hg.w)
+            pyoracle.write_graph_txt(os.path.join(d, "graph.txt"), roff, rcol, rw)
+            out = run("bfs_hx", d, f"{t}\n")
+            hops, _ = pyoracle.bfs_hops(roff, rcol, t)
+            got = np.array(open(os.path.join(d, "Hx.txt")).read().split(), np.int32)
+            assert got.tolist() == hops.tolist()
+            assert f"unreachable {int((hops < 0).sum())} ," in out
+            assert f"max-level: {int(hops.max())}" in out
+            # ga_star on the forward graph with that heuristic
+            hg.write_txt(os.path.join(d, "graph.txt"))
+            out = run("ga_star", d, f"1024\n{s} {t}\n").splitlines()
+            assert out[0] == "completed input"
+            cost = int(out[1].split()[1])
+            path_rev = [int(x) for x in out[2].split()]
+            found, want, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, got, s, t, quirk=False)
+            assert found and cost == want
+            assert path_rev[0] == t and path_rev[-1] == s
+            assert path_cost(hg.off, hg.col, hg.w, path_rev[::-1]) == cost
+            if pyoracle.ref_tool("a_star_cpu"):
+                f2, c2, _ = pyoracle.run_ref_astar(d, s, t)  # the reference's own CPU A*
+                assert f2 and c2 >= cost  # it drops edge E-1; equal unless that edge is on every shortest path
+                if hg.col[-1] not in path_rev:
+                    assert c2 == cost
+
+
+def test_da_star_contract():
+    hg = ddp_amd.HostGraph.random(4000, 6, 1, 100, 31)
+    upd = hg.gen_updates(4, 100, 100, 1, 100, 32)
+    src, dst = hg.reachable_pairs(1, 33)
+    s, t = int(src[0]), int(dst[0])
+    with tempfile.TemporaryDirectory() as d:
+        hg.write_txt(os.path.join(d, "graph.txt"))
+        pyoracle.write_hx_txt(os.path.join(d, "Hx.txt"), np.zeros(hg.N, np.int32))
+        upd.write_txt(os.path.join(d, "Updates.txt"))
+        outs = [run("da_star", d, f"1024\n{s} {t}\n", *a) for a in ((), ("--recompute",))]
+        costs = [[int(l.split()[-1]) for l in o.splitlines() if l.startswith("[OUT] Cost:")] for o in outs]
+        assert costs[0] == costs[1] and len(costs[0]) == 1 + len(upd.batches)
+        cur = (hg.off, hg.col, hg.w)
+        want = []
+        for b in [None] + upd.batches:
+            if b is not None:
+                cur = pyoracle.apply_batch(*cur, b)[:3]
+            dist, _ = pyoracle.dijkstra(*cur, s, t)
+            want.append(int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST)
+        assert costs[0] == want
+        paths = [l for l in outs[0].splitlines() if l.startswith("[OUT] Path(in reverse):")]
+        assert len(paths) == len(want)
+        assert "[INFO] RUNTIME:" in outs[0]
