@@ -504,6 +504,79 @@ class DAStar:
             pass
 
 
+def _result_dict(r):
+    return {f: getattr(r, f) for f, _ in _capi.ServerResult._fields_}
+
+
+class Server:
+    """Continuous-batching A* server on one GPU (ddp_server)."""
+
+    QUERY, UPDATE = 0, 1
+
+    def __init__(self, ctx, dg, heur=H_ZERO, scale=1, block=0, slots=0, heap_cap=0, ring=0, max_inserts=0,
+                 idle_ms=0):
+        self.ctx, self.dg = ctx, dg
+        o = _capi.ServerOpts(block, slots, heap_cap, ring, max_inserts, idle_ms)
+        h = c_void_p()
+        _check(_capi.lib().ddp_server_create(ctx.handle, dg.handle, _heur(heur, scale), byref(o), byref(h)),
+               ctx.handle)
+        self._h = h
+        ctx._adopt(self)
+
+    def info(self):
+        v = [c_int32() for _ in range(3)]
+        _check(_capi.lib().ddp_server_info(self._h, *[byref(x) for x in v]))
+        return dict(zip(["slots", "block", "version"], [x.value for x in v]))
+
+    def submit_query(self, s, t, tag):
+        return _capi.lib().ddp_server_submit_query(self._h, s, t, tag)
+
+    def submit_update(self, lines, tag):
+        arr = _updates_array(lines)
+        return _capi.lib().ddp_server_submit_update(self._h, arr, len(lines), tag)
+
+    def poll(self, max_results=4096):
+        buf = (_capi.ServerResult * max_results)()
+        n = c_int32()
+        _check(_capi.lib().ddp_server_poll(self._h, buf, max_results, byref(n)), self.ctx.handle)
+        return [_result_dict(buf[i]) for i in range(n.value)]
+
+    def drain(self, timeout_ms=600000):
+        _check(_capi.lib().ddp_server_drain(self._h, timeout_ms), self.ctx.handle)
+
+    def replay(self, arrival_ms, types, src, dst, updates):
+        """Open-loop replay; updates: list aligned with requests (None for
+        queries).  Returns one result dict per request."""
+        n = len(arrival_ms)
+        arr = np.ascontiguousarray(arrival_ms, np.float64)
+        ty = as_i32(types)
+        s = as_i32(src)
+        t = as_i32(dst)
+        offs = [0]
+        flat = []
+        for u in updates:
+            if u:
+                flat.extend(u)
+            offs.append(len(flat))
+        uo = as_i32(offs)
+        ua = _updates_array(flat)
+        out = (_capi.ServerResult * max(1, n))()
+        _check(_capi.lib().ddp_server_replay(self._h, n, ptr_f64(arr), ptr_i32(ty), ptr_i32(s), ptr_i32(t),
+                                             ptr_i32(uo), ua, out), self.ctx.handle)
+        return [_result_dict(out[i]) for i in range(n)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.lib().ddp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ------------------------------------------- facade of the reference library
 
 class GPU_Graph(DeviceGraph):

@@ -74,6 +74,16 @@ class DAStats(ctypes.Structure):
                 ("repair_ms", c_float), ("search_ms", c_float), ("total_ms", c_float)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("block", c_int32), ("slots", c_int32), ("heap_cap", c_int32), ("ring", c_int32),
+                ("max_inserts", c_int32), ("idle_ms", c_int32), ("reserved", c_int32 * 2)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("tag", c_uint64), ("type", c_int32), ("cost", c_int32), ("version", c_int32), ("status", c_int32),
+                ("submit_ms", c_double), ("done_ms", c_double), ("device_ms", c_double)]
+
+
 P_I32 = POINTER(c_int32)
 P_U32 = POINTER(c_uint32)
 P_F64 = POINTER(c_double)
@@ -115,6 +125,16 @@ SIGNATURES = {
     "ddp_dastar_num_edges": (c_int, [c_void_p, P_I32]),
     "ddp_dastar_export": (c_int, [c_void_p, P_I32, P_I32, P_U32]),
     "ddp_dastar_close": (None, [c_void_p]),
+    # ddp_server.h
+    "ddp_server_create": (c_int, [c_void_p, c_void_p, Heur, POINTER(ServerOpts), PP]),
+    "ddp_server_info": (c_int, [c_void_p, P_I32, P_I32, P_I32]),
+    "ddp_server_submit_query": (c_int, [c_void_p, c_int32, c_int32, c_uint64]),
+    "ddp_server_submit_update": (c_int, [c_void_p, POINTER(Update), c_int32, c_uint64]),
+    "ddp_server_poll": (c_int, [c_void_p, POINTER(ServerResult), c_int32, P_I32]),
+    "ddp_server_drain": (c_int, [c_void_p, c_int32]),
+    "ddp_server_destroy": (None, [c_void_p]),
+    "ddp_server_replay": (c_int, [c_void_p, c_int32, P_F64, P_I32, P_I32, P_I32, P_I32, POINTER(Update),
+                                  POINTER(ServerResult)]),
     # ddp_io.h
     "ddp_hgraph_from_arrays": (c_int, [c_int32, c_int32, P_I32, P_I32, P_U32, PP]),
     "ddp_hgraph_free": (None, [c_void_p]),

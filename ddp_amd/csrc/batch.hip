@@ -17,11 +17,9 @@
 #include <numeric>
 
 #include "../../include/ddp_batch.h"
-#include "search_expand.h"
+#include "slot_search.h"
 
 namespace {
-
-enum : uint32_t { QERR_HEAP = 1, QERR_ITERS = 2, QERR_BADQ = 4, QERR_PATH = 8 };
 
 struct BatchArgs {
     GraphView g;
@@ -40,46 +38,12 @@ struct BatchArgs {
     int32_t N, lcap, cap, maxit;
 };
 
-template <int B>
-struct SlotPush {
-    const LaneHeap *H;
-    int32_t *size;
-    uint32_t *s_F;
-    int32_t *touched;
-    int32_t *s_ntouch;
-    uint32_t err = 0;
-
-    __device__ void target(uint32_t f) { atomicMin(s_F, f); }
-    __device__ void touch(int32_t c, uint64_t prev)
-    {
-        if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(s_ntouch, 1)] = c;
-    }
-    __device__ void operator()(bool has, uint64_t key)
-    {
-        if (has && !heap_push(*H, *size, key)) err |= QERR_HEAP;
-    }
-};
-
-template <int HK>
-__device__ __forceinline__ void set_target(HeurView &hv, int32_t t)
-{
-    if constexpr (HK == DDP_H_MANHATTAN) {
-        hv.ty = t / hv.width;
-        hv.tx = t - hv.ty * hv.width;
-    } else if constexpr (HK == DDP_H_EUCLID) {
-        hv.tx = hv.xy[2 * t];
-        hv.ty = hv.xy[2 * t + 1];
-    }
-}
-
 template <int B, int HK>
 __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
 {
     extern __shared__ uint64_t s_heap[];  // lcap * B
-    __shared__ ExpandLds<B> L;
-    __shared__ uint32_t s_red[16];
-    __shared__ int32_t s_q, s_ntouch, s_plen;
-    __shared__ uint32_t s_F, s_err;
+    __shared__ SlotLds<B> S;
+    __shared__ int32_t s_q;
 
     const int lane = threadIdx.x;
     const int64_t slot = blockIdx.x;
@@ -89,129 +53,27 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
     const LaneHeap H{s_heap + lane, a.home + slot * a.cap * B + lane, B, a.lcap, (int64_t)B, a.cap};
     ExpandCounters C;
     uint64_t stale = 0;
-
     for (;;) {
-        if (lane == 0) {
-            s_q = (int32_t)atomicAdd(a.next, 1u);
-            s_F = DDP_INF_F;
-            s_ntouch = 0;
-            s_err = 0;
-        }
+        if (lane == 0) s_q = (int32_t)atomicAdd(a.next, 1u);
         __syncthreads();
         const int32_t qi = s_q;
         if (qi >= a.nq) break;
         const int32_t q = a.order ? a.order[qi] : qi;
         const int32_t s = a.qs[q], t = a.qt[q];
         if (lane == 0) a.qtime[2 * (int64_t)q] = dev_clock_100mhz();
-        HeurView hv = a.h;
-        bool run = s >= 0 && s < N && t >= 0 && t < N;
-        int32_t size = 0;
-        if (run) {
-            set_target<HK>(hv, t);
-            const int32_t hs = heur_eval<HK>(hv, s);
-            if (HK == DDP_H_ARRAY && hs < 0) {
-                run = false;  // h(s) = -1: s cannot reach t
-            } else if (s == t) {
-                run = false;
-                if (lane == 0) s_F = (uint32_t)hs;
-            } else if (lane == 0) {
-                gp[s] = ((uint64_t)(uint32_t)hs << 32) | 0xFFFFFFFFull;
-                touched[0] = s;
-                s_ntouch = 1;
-                heap_push(H, size, ((uint64_t)(uint32_t)hs << 32) | (uint32_t)s);
-            }
-        } else if (lane == 0) {
-            s_err = QERR_BADQ;
-        }
-        block_sync_global();
-        SlotPush<B> push{&H, &size, &s_F, touched, &s_ntouch};
-        if (run) {
-            for (int32_t it = 0;; ++it) {
-                const uint32_t mytop = size > 0 ? (uint32_t)(s_heap[lane] >> 32) : 0xFFFFFFFFu;
-                const uint32_t topmin = block_min_u32<B>(mytop, s_red);
-                if constexpr (B == 64) __syncthreads();
-                const uint32_t F = s_F;
-                if (F <= topmin) break;  // no queued node can improve t
-                if (it >= a.maxit) {
-                    if (lane == 0) atomicOr(&s_err, (uint32_t)QERR_ITERS);
-                    break;
-                }
-                int32_t u = -1;
-                uint32_t fu = 0;
-                while (size > 0) {
-                    uint64_t key;
-                    heap_pop(H, size, key);
-                    const int32_t node = (int32_t)(uint32_t)key;
-                    const uint32_t f = (uint32_t)(key >> 32);
-                    if ((uint32_t)(ld_gp(gp + node) >> 32) == f) {
-                        u = node;
-                        fu = f;
-                        break;
-                    }
-                    ++stale;
-                }
-                expand_block<B, HK>(a.g, hv, gp, t, u, fu, L, C, push);
-                if (push.err) {
-                    atomicOr(&s_err, push.err);
-                    push.err = 0;
-                }
-            }
-        }
-        block_sync_global();
-        const uint32_t F = s_F;
-        // -- path t -> s, written s..t
+        slot_search<B, HK>(a.g, a.h, gp, touched, H, s_heap, N, s, t, a.maxit, S, C, stale);
         if (a.path_cap > 0) {
-            if (lane == 0) {
-                int32_t n = 0;
-                if (F != DDP_INF_F) {
-                    int32_t *out = a.path + (int64_t)q * a.path_cap;
-                    int32_t v = t;
-                    for (int32_t step = 0; step <= N; ++step) {
-                        if (n < a.path_cap) out[n] = v;
-                        ++n;
-                        if (v == s) break;
-                        const uint32_t p = (uint32_t)ld_gp(gp + v);
-                        if (p >= (uint32_t)N) { n = 0; atomicOr(&s_err, (uint32_t)QERR_PATH); break; }
-                        v = (int32_t)p;
-                    }
-                }
-                s_plen = n;
-            }
-            if (lane == 0) {
-                // reverse t..s into s..t (this lane wrote every entry)
-                const int32_t n = s_plen;
-                if (n > 0 && n <= a.path_cap) {
-                    int32_t *out = a.path + (int64_t)q * a.path_cap;
-                    for (int32_t i = 0; i < n / 2; ++i) {
-                        const int32_t x = out[i];
-                        out[i] = out[n - 1 - i];
-                        out[n - 1 - i] = x;
-                    }
-                }
-                a.plen[q] = (n > a.path_cap) ? -n : n;
-            }
+            slot_path<B>(gp, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
+            if (lane == 0) a.plen[q] = S.plen;
         }
-        // -- reset the touched entries of this slot (only after lane 0's walk
-        //    above has read every parent it needs)
-        __syncthreads();
-        const int32_t nt = s_ntouch;
-        for (int32_t i = lane; i < nt; i += B)
-            gp[__hip_atomic_load(touched + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = DDP_GP_INF;
+        slot_reset<B>(gp, touched, S);
         if (lane == 0) {
-            a.cost[q] = (int32_t)F;
-            a.qerr[q] = s_err;
+            a.cost[q] = (int32_t)S.F;
+            a.qerr[q] = S.err;
             a.qtime[2 * (int64_t)q + 1] = dev_clock_100mhz();
         }
-        block_sync_global();
     }
-    const uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
-                   v3 = wave_sum_u64(stale);
-    if ((lane & 63) == 0) {
-        atomicAdd(a.stats + 0, (unsigned long long)v0);
-        atomicAdd(a.stats + 1, (unsigned long long)v1);
-        atomicAdd(a.stats + 2, (unsigned long long)v2);
-        atomicAdd(a.stats + 3, (unsigned long long)v3);
-    }
+    slot_flush_stats<B>(a.stats, C, stale);
 }
 
 __global__ void batch_fill_u64(uint64_t *p, int64_t n, uint64_t v)

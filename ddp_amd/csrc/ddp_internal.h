@@ -82,6 +82,17 @@ struct GraphView {
     const int32_t *__restrict__ dcol;
     const uint32_t *__restrict__ dw;
     int32_t N;
+    // Versioned graph (serving): a base edge p is visible iff dver[p] > ver;
+    // inserted edges hang off lhead[u] as a list ordered oldest to newest whose
+    // links pack (entry << 32 | version that inserted it); an entry x is
+    // visible iff its link version <= ver and ldel[x] > ver.
+    const int32_t *__restrict__ dver;
+    int32_t ver;
+    const uint64_t *lhead;
+    const uint64_t *lnext;
+    const int32_t *lv;
+    const uint32_t *lw;
+    const int32_t *ldel;
 };
 
 struct HeurView {

@@ -2,9 +2,13 @@
 //
 // Each lane of the block brings at most one popped node u (its queue's pop).
 // Out-degrees are prefix-summed across the block and the union of all rows is
-// walked B edges at a time, so lanes read consecutive CSR entries and a long
+// walked R*B edges per round, so lanes read consecutive CSR entries and a long
 // row is split across the whole block instead of serialising one lane (the
 // reference gives each thread its own row: GA_star/GA_v2.cu:109-167).
+// Within a round every lane owns R edges and issues their loads stage by
+// stage (row entries, then heuristic inputs and (f, parent) words, then the
+// CAS attempts), so R independent memory round trips overlap instead of
+// running back to back.
 #pragma once
 
 #include "device_common.h"
@@ -24,6 +28,14 @@ struct ExpandCounters {
     uint64_t expanded = 0, edges = 0, improved = 0;
 };
 
+// Edges handled per lane and round: more overlap for small blocks, fewer
+// registers for large ones.
+template <int B>
+constexpr int expand_unroll()
+{
+    return B <= 256 ? 4 : 2;
+}
+
 // Push policy interface:
 //   void target(uint32_t f)                    -- t improved to f (t is never queued)
 //   void touch(int32_t c, uint64_t prev)       -- c's word was prev before the update
@@ -33,6 +45,7 @@ __device__ __forceinline__ void expand_block(const GraphView &G, const HeurView 
                                              int32_t t, int32_t u, uint32_t fu, ExpandLds<B> &L,
                                              ExpandCounters &C, Push &push)
 {
+    constexpr int R = expand_unroll<B>();
     const int lane = threadIdx.x;
     int32_t deg = 0, b = 0, bd = 0, db = 0;
     uint32_t gu = 0;
@@ -57,51 +70,144 @@ __device__ __forceinline__ void expand_block(const GraphView &G, const HeurView 
     L.gu[lane] = gu;
     L.node[lane] = u;
     __syncthreads();
-    for (int32_t base = 0; base < total; base += B) {
-        const int32_t e = base + lane;
-        bool has = false;
-        uint64_t key = 0;
-        if (e < total) {
-            int32_t lo = 0, hi = B - 1;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi) >> 1;
-                if (L.incl[mid] > e) hi = mid;
-                else lo = mid + 1;
+    for (int32_t base = 0; base < total; base += B * R) {
+        int32_t c[R], jj[R];
+        uint32_t wt[R];
+        // stage 1: locate each edge and load its column and weight
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int32_t e = base + r * B + lane;
+            c[r] = -1;
+            jj[r] = 0;
+            wt[r] = 0;
+            if (e < total) {
+                int32_t lo = 0, hi = B - 1;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi) >> 1;
+                    if (L.incl[mid] > e) hi = mid;
+                    else lo = mid + 1;
+                }
+                const int32_t k = e - (lo ? L.incl[lo - 1] : 0);
+                jj[r] = lo;
+                if (k < L.based[lo]) {
+                    const int32_t p = L.rowb[lo] + k;
+                    c[r] = G.col[p];
+                    wt[r] = G.w[p];
+                    if (G.dver && G.dver[p] <= G.ver) c[r] = -1;  // deleted at or before this version
+                } else {
+                    const int32_t p = L.drow[lo] + (k - L.based[lo]);
+                    c[r] = G.dcol[p];
+                    wt[r] = G.dw[p];
+                }
+                C.edges++;
             }
-            const int32_t j = lo;
-            const int32_t k = e - (j ? L.incl[j - 1] : 0);
-            int32_t c;
-            uint32_t wt;
-            if (k < L.based[j]) {
-                const int32_t p = L.rowb[j] + k;
-                c = G.col[p];
-                wt = G.w[p];
-            } else {
-                const int32_t p = L.drow[j] + (k - L.based[j]);
-                c = G.dcol[p];
-                wt = G.dw[p];
+        }
+        // stage 2: heuristic inputs and the current (f, parent) words
+        int32_t hc[R];
+        uint64_t cur[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            hc[r] = 0;
+            cur[r] = 0;
+            if (c[r] >= 0) {  // negative id = deleted edge
+                hc[r] = heur_eval<HK>(H, c[r]);
+                cur[r] = ld_gp(gp + c[r]);
             }
-            C.edges++;
-            if (c >= 0) {  // negative id = deleted edge
-                const int32_t hc = heur_eval<HK>(H, c);
-                if (HK != DDP_H_ARRAY || hc >= 0) {  // h = -1: cannot reach t
-                    // unsigned relax expression of GA_star/GA_v2.cu:137
-                    const uint32_t cand = L.gu[j] + wt + (uint32_t)hc;
+        }
+        // stage 3: one CAS attempt per improving edge, all issued together
+        uint64_t want[R], got[R];
+        bool tried[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            tried[r] = false;
+            want[r] = 0;
+            if (c[r] >= 0 && (HK != DDP_H_ARRAY || hc[r] >= 0)) {  // h = -1: cannot reach t
+                const uint32_t cand = L.gu[jj[r]] + wt[r] + (uint32_t)hc[r];
+                want[r] = ((uint64_t)cand << 32) | (uint32_t)L.node[jj[r]];
+                tried[r] = (uint32_t)(cur[r] >> 32) > cand;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            got[r] = tried[r] ? (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
+                                                   (unsigned long long)want[r])
+                              : 0;
+        // stage 4: settle contended CASes, then hand improvements to the push policy
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            bool has = false;
+            uint64_t key = 0;
+            if (tried[r]) {
+                const uint32_t cand = (uint32_t)(want[r] >> 32);
+                bool ok = got[r] == cur[r];
+                uint64_t prev = cur[r];
+                if (!ok) {
+                    uint64_t now = got[r];
+                    while ((uint32_t)(now >> 32) > cand) {
+                        const uint64_t old = atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)now,
+                                                       (unsigned long long)want[r]);
+                        if (old == now) {
+                            ok = true;
+                            prev = now;
+                            break;
+                        }
+                        now = old;
+                    }
+                }
+                if (ok) {
+                    C.improved++;
+                    push.touch(c[r], prev);
+                    if (c[r] == t) {
+                        push.target(cand);
+                    } else {
+                        has = true;
+                        key = ((uint64_t)cand << 32) | (uint32_t)c[r];
+                    }
+                }
+            }
+            push(has, key);
+        }
+    }
+    if (G.lhead) {
+        // Versioned inserts: each lane walks its own node's list, oldest
+        // first, stopping at the first entry newer than this version (links
+        // carry the inserting version, so newer entries are never read).
+        uint64_t link = u >= 0 ? __hip_atomic_load(G.lhead + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : ~0ull;
+        for (;;) {
+            bool active = false, has = false;
+            uint64_t key = 0;
+            int32_t x = -1;
+            while ((uint32_t)link <= (uint32_t)G.ver) {
+                x = (int32_t)(link >> 32);
+                link = __hip_atomic_load(G.lnext + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (G.ldel[x] > G.ver) {
+                    active = true;
+                    break;
+                }
+            }
+            if (active) {
+                C.edges++;
+                const int32_t cn = G.lv[x];
+                const int32_t hcn = heur_eval<HK>(H, cn);
+                if (HK != DDP_H_ARRAY || hcn >= 0) {
+                    const uint32_t cand = gu + G.lw[x] + (uint32_t)hcn;
                     uint64_t prev;
-                    if (relax_gp(gp, c, cand, (uint32_t)L.node[j], &prev)) {
+                    if (relax_gp(gp, cn, cand, (uint32_t)u, &prev)) {
                         C.improved++;
-                        push.touch(c, prev);
-                        if (c == t) {
+                        push.touch(cn, prev);
+                        if (cn == t) {
                             push.target(cand);
                         } else {
                             has = true;
-                            key = ((uint64_t)cand << 32) | (uint32_t)c;
+                            key = ((uint64_t)cand << 32) | (uint32_t)cn;
                         }
                     }
                 }
             }
+            push(has, key);
+            if (!__syncthreads_or(active)) break;
         }
-        push(has, key);
     }
     __syncthreads();
 }

@@ -1,0 +1,678 @@
+// Online serving with continuous batching (see include/ddp_server.h).
+//
+// One persistent kernel: every workgroup is a slot that claims the next
+// request from a ring in pinned, host-coherent memory as soon as it is free,
+// so requests that arrive while others run are admitted as slots free up and
+// their searches overlap on the device.  Query slots reuse the batched
+// search (slot_search.h) on a versioned view of the graph.
+//
+// Graph versions.  Update batch k is applied by whichever slot claims it, in
+// version order: each delete claims (CAS INF -> k) the delete-version word of
+// the first instance of u->v still live, base row first and then inserted
+// edges oldest first (DA_star/GA_merge.cu:1019-1037 semantics); inserts are
+// appended to their source's list with links that carry k; then the applied
+// version is published with release ordering.  A query at version V waits
+// until V is applied, acquires it, and then reads only data written by
+// versions <= V -- newer list entries are never dereferenced, and a stale copy
+// of a newer delete version is as good as INF -- so its answer depends only on
+// (s, t, V).
+//
+// Every device wait is bounded; idle workgroups exit after idle_ms and the host
+// relaunches the kernel when new requests arrive.
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <tuple>
+#include <chrono>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/ddp_server.h"
+#include "slot_search.h"
+
+namespace {
+
+struct Req {       // pinned, written by the host
+    int32_t type;
+    int32_t s, t;  // query endpoints
+    int32_t version;
+    int32_t del_off, del_n;  // updates: (u,v) pairs in the arena
+    int32_t ins_off, ins_n;  // updates: (u,v,w,entry) quads, sorted by u
+};
+
+struct Res {       // pinned, written by the device
+    uint32_t seq;  // ticket + 1 once valid
+    int32_t cost;
+    int32_t status;
+    int32_t version;
+    uint64_t t0, t1;
+};
+
+enum : uint32_t { SERR_WAIT = 16, SERR_LOG = 32 };
+enum : int { CT_HEAD = 0, CT_APPLIED = 32, CT_LOG = 64, CT_WORDS = 96 };
+
+struct ServeArgs {
+    GraphView g;           // base rows + versioned view fields (ver set per query)
+    HeurView h;
+    int32_t *dver;         // E, writable
+    uint64_t *lhead;       // N, writable
+    uint64_t *lnext;       // log capacity, writable
+    int32_t *lv;
+    uint32_t *lw;
+    int32_t *ldel;
+    int32_t *ltail;        // N, last entry of each list (updaters only)
+    int32_t log_cap;
+    const Req *ring;       // pinned
+    Res *res;              // pinned
+    const int32_t *arena;  // pinned update payloads
+    const uint32_t *tail;  // pinned: requests published
+    const uint32_t *stop;  // pinned
+    uint32_t ring_mask;
+    uint32_t *ctrl;        // device: claimed count, applied version
+    uint64_t *gp;
+    int32_t *touched;
+    uint64_t *home;
+    unsigned long long *stats;
+    int32_t N, lcap, cap, maxit;
+    uint64_t idle_ticks, wait_ticks;
+};
+
+__device__ __forceinline__ uint32_t sys_ld(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ __forceinline__ int32_t sys_ld_i(const int32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int B>
+__device__ void apply_update(const ServeArgs &a, const Req &rq, uint32_t *s_err)
+{
+    const int32_t V = rq.version;
+    const int lane = threadIdx.x;
+    // deletes: first instance still live at V-1 (its delete word is INF)
+    for (int32_t i = lane; i < rq.del_n; i += B) {
+        const int32_t u = sys_ld_i(a.arena + rq.del_off + 2 * i), v = sys_ld_i(a.arena + rq.del_off + 2 * i + 1);
+        if (u < 0 || u >= a.N || v < 0 || v >= a.N) continue;
+        bool done = false;
+        for (int32_t j = a.g.off[u]; j < a.g.off[u + 1] && !done; ++j)
+            if (a.g.col[j] == v && atomicCAS(a.dver + j, INT32_MAX, V) == INT32_MAX) done = true;
+        uint64_t link = __hip_atomic_load(a.lhead + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (!done && (uint32_t)link < (uint32_t)V) {
+            const int32_t x = (int32_t)(link >> 32);
+            link = __hip_atomic_load(a.lnext + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (a.lv[x] == v && atomicCAS(a.ldel + x, INT32_MAX, V) == INT32_MAX) done = true;
+        }
+    }
+    block_sync_global();
+    // inserts: one lane per source (entries arrive sorted by source, entry ids
+    // assigned by the host in arrival order)
+    for (int32_t i = lane; i < rq.ins_n; i += B) {
+        const int32_t *q = a.arena + rq.ins_off + 4 * i;
+        const int32_t u = sys_ld_i(q);
+        if (i > 0 && sys_ld_i(q - 4) == u) continue;  // not the first of its source's group
+        for (int32_t k = i; k < rq.ins_n; ++k) {
+            const int32_t *e = a.arena + rq.ins_off + 4 * k;
+            if (sys_ld_i(e) != u) break;
+            const int32_t v = sys_ld_i(e + 1), x = sys_ld_i(e + 3);
+            const uint32_t w = (uint32_t)sys_ld_i(e + 2);
+            if (u < 0 || u >= a.N || v < 0 || v >= a.N || x < 0 || x >= a.log_cap) {
+                atomicOr(s_err, (uint32_t)SERR_LOG);
+                continue;
+            }
+            a.lv[x] = v;
+            a.lw[x] = w;
+            a.ldel[x] = INT32_MAX;
+            a.lnext[x] = ~0ull;
+            const uint64_t link = ((uint64_t)(uint32_t)x << 32) | (uint32_t)V;
+            const int32_t last = a.ltail[u];
+            if (last < 0) a.lhead[u] = link;
+            else a.lnext[last] = link;
+            a.ltail[u] = x;
+        }
+    }
+    block_sync_global();
+}
+
+template <int B, int HK>
+__global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
+{
+    extern __shared__ uint64_t s_heap[];
+    __shared__ SlotLds<B> S;
+    __shared__ int32_t s_ticket;
+    __shared__ uint32_t s_uerr;
+    const int lane = threadIdx.x;
+    const int64_t slot = blockIdx.x;
+    const int32_t N = a.N;
+    uint64_t *gp = a.gp + slot * N;
+    int32_t *touched = a.touched + slot * N;
+    const LaneHeap H{s_heap + lane, a.home + slot * a.cap * B + lane, B, a.lcap, (int64_t)B, a.cap};
+    ExpandCounters C;
+    uint64_t stale = 0;
+    for (;;) {
+        // ---- claim the next published request (bounded idle wait)
+        if (lane == 0) {
+            int32_t ticket = -1;
+            const uint64_t t0 = dev_clock_100mhz();
+            int backoff = 0;  // idle polling of host memory backs off to ~15 us
+            for (;;) {
+                const uint32_t tail = sys_ld(a.tail);
+                const uint32_t h = ld_u32(a.ctrl + CT_HEAD);
+                if (h < tail) {
+                    if (atomicCAS(a.ctrl + CT_HEAD, h, h + 1) == h) {
+                        ticket = (int32_t)h;
+                        break;
+                    }
+                    continue;
+                }
+                if (sys_ld(a.stop) || dev_clock_100mhz() - t0 > a.idle_ticks) break;
+                backoff = backoff < 8 ? backoff + 1 : 8;
+                for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(64);
+            }
+            s_ticket = ticket;
+            s_uerr = 0;
+        }
+        __syncthreads();
+        const int32_t ticket = s_ticket;
+        if (ticket < 0) break;
+        const Req *rp = a.ring + ((uint32_t)ticket & a.ring_mask);
+        Req rq;
+        rq.type = sys_ld_i(&rp->type);
+        rq.s = sys_ld_i(&rp->s);
+        rq.t = sys_ld_i(&rp->t);
+        rq.version = sys_ld_i(&rp->version);
+        rq.del_off = sys_ld_i(&rp->del_off);
+        rq.del_n = sys_ld_i(&rp->del_n);
+        rq.ins_off = sys_ld_i(&rp->ins_off);
+        rq.ins_n = sys_ld_i(&rp->ins_n);
+        const uint64_t tstart = dev_clock_100mhz();
+        // ---- wait for the graph version this request needs (bounded)
+        const uint32_t need = rq.type == DDP_REQ_UPDATE ? (uint32_t)rq.version - 1 : (uint32_t)rq.version;
+        if (lane == 0) {
+            while (__hip_atomic_load(a.ctrl + CT_APPLIED, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+                if (dev_clock_100mhz() - tstart > a.wait_ticks) {
+                    s_uerr = SERR_WAIT;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+        }
+        __syncthreads();
+        // every wave acquires the applied version before reading graph data
+        (void)__hip_atomic_load(a.ctrl + CT_APPLIED, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t cost = DDP_INF_COST;
+        uint32_t status = s_uerr;
+        if (rq.type == DDP_REQ_UPDATE) {
+            if (!status) apply_update<B>(a, rq, &s_uerr);
+            __syncthreads();
+            status = s_uerr;
+            if (lane == 0) {
+                __threadfence();
+                // publish even after a failed wait so later requests cannot stall behind it
+                __hip_atomic_store(a.ctrl + CT_APPLIED, (uint32_t)rq.version, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (!status) {
+            GraphView G = a.g;
+            G.ver = rq.version;
+            slot_search<B, HK>(G, a.h, gp, touched, H, s_heap, N, rq.s, rq.t, a.maxit, S, C, stale);
+            cost = (int32_t)S.F;
+            status = S.err;
+            slot_reset<B>(gp, touched, S);
+        }
+        if (lane == 0) {
+            Res *r = a.res + ((uint32_t)ticket & a.ring_mask);
+            __hip_atomic_store(&r->cost, cost, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->status, (int32_t)status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->version, rq.version, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->t0, tstart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->t1, dev_clock_100mhz(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->seq, (uint32_t)ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        __syncthreads();
+    }
+    slot_flush_stats<B>(a.stats, C, stale);
+}
+
+__global__ void serve_fill_u64(uint64_t *p, int64_t n, uint64_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void serve_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+using ServeFn = void (*)(ServeArgs);
+
+template <int B>
+ServeFn pick_s(int hk)
+{
+    switch (hk) {
+    case DDP_H_ZERO: return serve_kernel<B, DDP_H_ZERO>;
+    case DDP_H_ARRAY: return serve_kernel<B, DDP_H_ARRAY>;
+    case DDP_H_MANHATTAN: return serve_kernel<B, DDP_H_MANHATTAN>;
+    default: return serve_kernel<B, DDP_H_EUCLID>;
+    }
+}
+
+ServeFn pick_serve(int B, int hk)
+{
+    switch (B) {
+    case 64: return pick_s<64>(hk);
+    case 128: return pick_s<128>(hk);
+    case 256: return pick_s<256>(hk);
+    default: return pick_s<512>(hk);
+    }
+}
+
+using clk = std::chrono::steady_clock;
+
+}  // namespace
+
+struct ddp_server {
+    ddp_ctx *ctx = nullptr;
+    int device = 0;
+    const ddp_graph *g = nullptr;
+    HeurView hv{};
+    int32_t B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0;
+    uint32_t ring = 0;
+    int32_t log_cap = 0, log_used = 0;
+    int32_t idle_ms = 2000;
+    // host view of the graph (for nothing but sizes)
+    int32_t N = 0, E = 0;
+    // device
+    int32_t *dver = nullptr, *lv = nullptr, *ldel = nullptr, *ltail = nullptr, *touched = nullptr;
+    uint32_t *lw = nullptr, *ctrl = nullptr;
+    uint64_t *lhead = nullptr, *lnext = nullptr, *gp = nullptr, *home = nullptr;
+    unsigned long long *stats = nullptr;
+    // pinned
+    Req *ring_h = nullptr;
+    Res *res_h = nullptr;
+    int32_t *arena_h = nullptr;
+    uint32_t *flags_h = nullptr;  // [0] tail, [1] stop
+    Req *ring_d = nullptr;
+    Res *res_d = nullptr;
+    int32_t *arena_d = nullptr;
+    uint32_t *flags_d = nullptr;
+    size_t arena_cap = 0, arena_head = 0;       // int32 units, ring
+    std::vector<std::tuple<uint32_t, size_t, size_t>> arena_marks;  // (ticket, start, end) in use
+    // bookkeeping
+    uint32_t submitted = 0, collected = 0;
+    int32_t version = 0;
+    std::vector<uint64_t> tags;
+    std::vector<double> submit_ms;
+    std::vector<int32_t> types;
+    clk::time_point t_create;
+    hipEvent_t done_ev = nullptr;
+    bool running = false;
+};
+
+static double ms_now(const ddp_server *sv)
+{
+    return std::chrono::duration<double, std::milli>(clk::now() - sv->t_create).count();
+}
+
+static void server_free(ddp_server *sv)
+{
+    void *dev[] = {sv->dver, sv->lv, sv->ldel, sv->ltail, sv->touched, sv->lw, sv->ctrl, sv->lhead,
+                   sv->lnext, sv->gp, sv->home, sv->stats};
+    for (void *p : dev)
+        if (p) hipFree(p);
+    void *host[] = {sv->ring_h, sv->res_h, sv->arena_h, sv->flags_h};
+    for (void *p : host)
+        if (p) hipHostFree(p);
+    if (sv->done_ev) hipEventDestroy(sv->done_ev);
+}
+
+static int server_launch(ddp_server *sv)
+{
+    ddp_ctx *ctx = sv->ctx;
+    ServeArgs a{};
+    a.g = ddp_view(sv->g);
+    a.g.dver = sv->dver;
+    a.g.lhead = sv->lhead;
+    a.g.lnext = sv->lnext;
+    a.g.lv = sv->lv;
+    a.g.lw = sv->lw;
+    a.g.ldel = sv->ldel;
+    a.h = sv->hv;
+    a.dver = sv->dver;
+    a.lhead = sv->lhead;
+    a.lnext = sv->lnext;
+    a.lv = sv->lv;
+    a.lw = sv->lw;
+    a.ldel = sv->ldel;
+    a.ltail = sv->ltail;
+    a.log_cap = sv->log_cap;
+    a.ring = sv->ring_d;
+    a.res = sv->res_d;
+    a.arena = sv->arena_d;
+    a.tail = sv->flags_d;
+    a.stop = sv->flags_d + 1;
+    a.ring_mask = sv->ring - 1;
+    a.ctrl = sv->ctrl;
+    a.gp = sv->gp;
+    a.touched = sv->touched;
+    a.home = sv->home;
+    a.stats = sv->stats;
+    a.N = sv->N;
+    a.lcap = sv->lcap;
+    a.cap = sv->cap;
+    a.maxit = sv->maxit;
+    a.idle_ticks = (uint64_t)sv->idle_ms * 100000ull;
+    a.wait_ticks = 100000ull * 60000;  // a version wait longer than a minute is an error
+    void *args[] = {&a};
+    DDP_HIP(ctx, hipLaunchKernel((const void *)pick_serve(sv->B, sv->hv.kind), dim3(sv->slots), dim3(sv->B), args,
+                                 (size_t)sv->lcap * sv->B * 8, ctx->stream));
+    DDP_HIP(ctx, hipEventRecord(sv->done_ev, ctx->stream));
+    sv->running = true;
+    return DDP_OK;
+}
+
+// Relaunches the kernel if every workgroup has exited while requests wait.
+static int server_ensure(ddp_server *sv)
+{
+    if (sv->running) {
+        const hipError_t q = hipEventQuery(sv->done_ev);
+        if (q == hipErrorNotReady) return DDP_OK;
+        if (q != hipSuccess) return ddp_hip_err(sv->ctx, q, "serve kernel", __FILE__, __LINE__);
+        sv->running = false;
+    }
+    if (sv->collected < sv->submitted) {
+        // anything not yet reported was never claimed or is finished: both fine to relaunch over
+        return server_launch(sv);
+    }
+    return DDP_OK;
+}
+
+extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *o,
+                                 ddp_server **out)
+{
+    if (!ctx || !g || !out) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_server_create: null argument");
+    *out = nullptr;
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    HeurView hv;
+    int rc = ddp_make_heur(g, h, -1, &hv);
+    if (rc) return rc;
+    DDP_CHECK(hv.kind != DDP_H_ARRAY, ctx, DDP_ERR_ARG,
+              "serving needs a per-query heuristic: DDP_H_ZERO, DDP_H_MANHATTAN or DDP_H_EUCLID");
+    ddp_server *sv = new ddp_server();
+    sv->ctx = ctx;
+    sv->device = ctx->device;
+    sv->g = g;
+    sv->hv = hv;
+    sv->N = g->N;
+    sv->E = g->E;
+    sv->B = (o && o->block) ? o->block : 256;
+    sv->lcap = std::max(4, std::min(32, 4096 / std::max(64, sv->B)));
+    sv->cap = (o && o->heap_cap) ? o->heap_cap : 2048;
+    sv->ring = (o && o->ring) ? (uint32_t)o->ring : 65536u;
+    sv->log_cap = (o && o->max_inserts) ? o->max_inserts : (1 << 20);
+    sv->idle_ms = (o && o->idle_ms) ? o->idle_ms : 2000;
+    sv->maxit = (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
+    sv->t_create = clk::now();
+    auto fail = [&](int code) {
+        server_free(sv);
+        delete sv;
+        return code;
+    };
+    if (!(sv->B == 64 || sv->B == 128 || sv->B == 256 || sv->B == 512) || (sv->ring & (sv->ring - 1)) ||
+        sv->ring < 64)
+        return fail(ddp_set_err(ctx, DDP_ERR_ARG, "block must be 64..512 and ring a power of two >= 64"));
+    const void *fn = (const void *)pick_serve(sv->B, hv.kind);
+    int per_cu = 0;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sv->lcap * sv->B * 8));
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sv->B, (size_t)sv->lcap * sv->B * 8);
+    if (e != hipSuccess || per_cu < 1) return fail(ddp_set_err(ctx, DDP_ERR_HIP, "serve kernel occupancy query failed"));
+    int64_t slots = (o && o->slots) ? o->slots : (int64_t)per_cu * ctx->prop.multiProcessorCount;
+    slots = std::min<int64_t>(slots, (int64_t)per_cu * ctx->prop.multiProcessorCount);
+    // admission by free HBM: graph-version state first, then as many slots as fit (10% headroom)
+    size_t fr = 0, tot = 0;
+    e = hipMemGetInfo(&fr, &tot);
+    const size_t fixed = (size_t)g->E * 4 + (size_t)g->N * 12 + (size_t)sv->log_cap * 24 + (1 << 20);
+    const size_t per_slot = (size_t)g->N * 12 + (size_t)sv->cap * sv->B * 8;
+    const size_t budget = fr > fixed ? (size_t)((fr - fixed) * 0.9) : 0;
+    slots = std::min<int64_t>(slots, (int64_t)(budget / per_slot));
+    if (e != hipSuccess || slots < 1)
+        return fail(ddp_set_err(ctx, DDP_ERR_NOMEM, "not enough free HBM for one serving slot"));
+    sv->slots = (int32_t)slots;
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 64));
+    };
+    al((void **)&sv->dver, (size_t)g->E * 4);
+    al((void **)&sv->lhead, (size_t)g->N * 8);
+    al((void **)&sv->ltail, (size_t)g->N * 4);
+    al((void **)&sv->lnext, (size_t)sv->log_cap * 8);
+    al((void **)&sv->lv, (size_t)sv->log_cap * 4);
+    al((void **)&sv->lw, (size_t)sv->log_cap * 4);
+    al((void **)&sv->ldel, (size_t)sv->log_cap * 4);
+    al((void **)&sv->ctrl, CT_WORDS * 4);
+    al((void **)&sv->stats, 64);
+    al((void **)&sv->gp, (size_t)slots * g->N * 8);
+    al((void **)&sv->touched, (size_t)slots * g->N * 4);
+    al((void **)&sv->home, (size_t)slots * sv->cap * sv->B * 8);
+    sv->arena_cap = (size_t)1 << 22;
+    auto hal = [&](void **hp, void **dp, size_t bytes) {
+        if (e == hipSuccess) e = hipHostMalloc(hp, bytes, hipHostMallocCoherent | hipHostMallocMapped);
+        if (e == hipSuccess) std::memset(*hp, 0, bytes);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(dp, *hp, 0);
+    };
+    hal((void **)&sv->ring_h, (void **)&sv->ring_d, sizeof(Req) * sv->ring);
+    hal((void **)&sv->res_h, (void **)&sv->res_d, sizeof(Res) * sv->ring);
+    hal((void **)&sv->arena_h, (void **)&sv->arena_d, sv->arena_cap * 4);
+    hal((void **)&sv->flags_h, (void **)&sv->flags_d, 64);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->done_ev, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(ddp_hip_err(ctx, e, "serving allocation", __FILE__, __LINE__));
+    hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->dver, (int64_t)g->E, INT32_MAX);
+    hipLaunchKernelGGL(serve_fill_u64, dim3(1024), dim3(256), 0, ctx->stream, sv->lhead, (int64_t)g->N, ~0ull);
+    hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->ltail, (int64_t)g->N, -1);
+    hipLaunchKernelGGL(serve_fill_u64, dim3(4096), dim3(256), 0, ctx->stream, sv->gp, (int64_t)slots * g->N,
+                       (uint64_t)DDP_GP_INF);
+    if (e == hipSuccess) e = hipMemsetAsync(sv->ctrl, 0, CT_WORDS * 4, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(sv->stats, 0, 64, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ddp_hip_err(ctx, e, "serving setup", __FILE__, __LINE__));
+    *out = sv;
+    return DDP_OK;
+}
+
+extern "C" int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_t *version)
+{
+    if (!sv) return DDP_ERR_ARG;
+    if (slots) *slots = sv->slots;
+    if (block) *block = sv->B;
+    if (version) *version = sv->version;
+    return DDP_OK;
+}
+
+static void publish(ddp_server *sv, const Req &r, uint64_t tag, int32_t type)
+{
+    sv->ring_h[sv->submitted & (sv->ring - 1)] = r;
+    Res *res = sv->res_h + (sv->submitted & (sv->ring - 1));
+    __atomic_store_n(&res->seq, 0u, __ATOMIC_RELAXED);
+    sv->tags.push_back(tag);
+    sv->types.push_back(type);
+    sv->submit_ms.push_back(ms_now(sv));
+    ++sv->submitted;
+    __atomic_store_n(&sv->flags_h[0], sv->submitted, __ATOMIC_RELEASE);
+}
+
+extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uint64_t tag)
+{
+    if (!sv) return DDP_ERR_ARG;
+    ddp_ctx *ctx = sv->ctx;
+    DDP_CHECK(s >= 0 && s < sv->N && t >= 0 && t < sv->N, ctx, DDP_ERR_ARG, "query %d->%d outside [0,%d)", s, t, sv->N);
+    if (sv->submitted - sv->collected >= sv->ring) return ddp_set_err(ctx, DDP_ERR_NOMEM, "request ring full");
+    Req r{};
+    r.type = DDP_REQ_QUERY;
+    r.s = s;
+    r.t = t;
+    r.version = sv->version;
+    publish(sv, r, tag, DDP_REQ_QUERY);
+    return server_ensure(sv);
+}
+
+extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *lines, int32_t n, uint64_t tag)
+{
+    if (!sv || (n && !lines)) return DDP_ERR_ARG;
+    ddp_ctx *ctx = sv->ctx;
+    if (sv->submitted - sv->collected >= sv->ring) return ddp_set_err(ctx, DDP_ERR_NOMEM, "request ring full");
+    std::vector<int32_t> dels;
+    std::vector<std::array<int32_t, 3>> ins;
+    for (int32_t i = 0; i < n; ++i) {
+        const ddp_update_t &l = lines[i];
+        DDP_CHECK((l.flag == 0 || l.flag == 1) && l.u >= 0 && l.u < sv->N && l.v >= 0 && l.v < sv->N, ctx, DDP_ERR_ARG,
+                  "update line %d invalid", i);
+        if (l.flag == 0) {
+            dels.push_back(l.u);
+            dels.push_back(l.v);
+        } else {
+            ins.push_back({l.u, l.v, (int32_t)l.w});
+        }
+    }
+    DDP_CHECK(sv->log_used + (int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_NOMEM,
+              "inserted-edge log full (%d entries); raise max_inserts", sv->log_cap);
+    // entry ids in arrival order, then grouped by source (stable)
+    std::vector<std::array<int32_t, 4>> q(ins.size());
+    for (size_t k = 0; k < ins.size(); ++k) q[k] = {ins[k][0], ins[k][1], ins[k][2], sv->log_used + (int32_t)k};
+    std::stable_sort(q.begin(), q.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
+    const size_t need = dels.size() + 4 * q.size();
+    DDP_CHECK(need < sv->arena_cap / 2, ctx, DDP_ERR_ARG, "update batch too large for the payload arena");
+    // The arena is a ring: a payload is free again once its update has been
+    // collected (the device is done with it).  Refuse (caller polls and
+    // retries) if the new payload would overlap one still in use.
+    size_t pos = sv->arena_head;
+    if (pos + need > sv->arena_cap) pos = 0;
+    while (!sv->arena_marks.empty() && std::get<0>(sv->arena_marks.front()) < sv->collected)
+        sv->arena_marks.erase(sv->arena_marks.begin());
+    for (const auto &m : sv->arena_marks)
+        if (pos < std::get<2>(m) && std::get<1>(m) < pos + need)
+            return ddp_set_err(ctx, DDP_ERR_NOMEM, "update payload arena busy; poll and retry");
+    std::memcpy(sv->arena_h + pos, dels.data(), dels.size() * 4);
+    for (size_t k = 0; k < q.size(); ++k) std::memcpy(sv->arena_h + pos + dels.size() + 4 * k, q[k].data(), 16);
+    Req r{};
+    r.type = DDP_REQ_UPDATE;
+    r.version = sv->version + 1;
+    r.del_off = (int32_t)pos;
+    r.del_n = (int32_t)(dels.size() / 2);
+    r.ins_off = (int32_t)(pos + dels.size());
+    r.ins_n = (int32_t)q.size();
+    sv->arena_head = pos + need;
+    sv->arena_marks.emplace_back(sv->submitted, pos, pos + need);
+    sv->log_used += (int32_t)q.size();
+    sv->version += 1;
+    publish(sv, r, tag, DDP_REQ_UPDATE);
+    return server_ensure(sv);
+}
+
+extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t max, int32_t *n)
+{
+    if (!sv || !n || (max > 0 && !out)) return DDP_ERR_ARG;
+    *n = 0;
+    while (*n < max && sv->collected < sv->submitted) {
+        const uint32_t k = sv->collected;
+        Res *r = sv->res_h + (k & (sv->ring - 1));
+        if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != k + 1) break;
+        ddp_server_result_t &o = out[(*n)++];
+        o.tag = sv->tags[k];
+        o.type = sv->types[k];
+        o.cost = r->cost;
+        o.version = r->version;
+        o.status = r->status;
+        o.submit_ms = sv->submit_ms[k];
+        o.done_ms = ms_now(sv);
+        o.device_ms = (double)(r->t1 - r->t0) * 1e-5;
+        ++sv->collected;
+    }
+    return server_ensure(sv);
+}
+
+extern "C" int ddp_server_drain(ddp_server *sv, int32_t timeout_ms)
+{
+    if (!sv) return DDP_ERR_ARG;
+    const auto t0 = clk::now();
+    std::vector<ddp_server_result_t> sink(1024);
+    while (sv->collected < sv->submitted) {
+        // results are kept for the caller: only wait here, do not consume
+        const uint32_t k = sv->collected;
+        uint32_t done = k;
+        while (done < sv->submitted &&
+               __atomic_load_n(&sv->res_h[done & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) == done + 1)
+            ++done;
+        if (done == sv->submitted) break;
+        int rc = server_ensure(sv);
+        if (rc) return rc;
+        if (std::chrono::duration<double, std::milli>(clk::now() - t0).count() > timeout_ms)
+            return ddp_set_err(sv->ctx, DDP_ERR_TIMEOUT, "drain: %u requests still running", sv->submitted - done);
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    return DDP_OK;
+}
+
+extern "C" void ddp_server_destroy(ddp_server *sv)
+{
+    if (!sv) return;
+    hipSetDevice(sv->device);
+    if (sv->flags_h) __atomic_store_n(&sv->flags_h[1], 1u, __ATOMIC_RELEASE);
+    if (sv->running && sv->done_ev) hipEventSynchronize(sv->done_ev);
+    server_free(sv);
+    delete sv;
+}
+
+extern "C" int ddp_server_replay(ddp_server *sv, int32_t n, const double *arrival_ms, const int32_t *type,
+                                 const int32_t *s, const int32_t *t, const int32_t *upd_off, const ddp_update_t *upd,
+                                 ddp_server_result_t *out)
+{
+    if (!sv || n < 0 || (n && (!arrival_ms || !type || !out))) return DDP_ERR_ARG;
+    ddp_ctx *ctx = sv->ctx;
+    std::vector<ddp_server_result_t> got(4096);
+    std::vector<char> have(n, 0);
+    int32_t received = 0;
+    auto collect = [&]() -> int {
+        int32_t k = 0;
+        int rc = ddp_server_poll(sv, got.data(), (int32_t)got.size(), &k);
+        for (int32_t i = 0; i < k; ++i) {
+            const uint64_t idx = got[i].tag;
+            if (idx < (uint64_t)n && !have[idx]) {
+                out[idx] = got[i];
+                have[idx] = 1;
+                ++received;
+            }
+        }
+        return rc;
+    };
+    const double start = ms_now(sv);
+    for (int32_t i = 0; i < n; ++i) {
+        while (ms_now(sv) - start < arrival_ms[i]) {
+            int rc = collect();
+            if (rc) return rc;
+        }
+        for (;;) {
+            int rc = type[i] == DDP_REQ_UPDATE
+                         ? ddp_server_submit_update(sv, upd + upd_off[i], upd_off[i + 1] - upd_off[i], (uint64_t)i)
+                         : ddp_server_submit_query(sv, s[i], t[i], (uint64_t)i);
+            if (rc == DDP_OK) break;
+            if (rc != DDP_ERR_NOMEM) return rc;
+            if ((rc = collect())) return rc;  // ring full: make room
+        }
+    }
+    const auto t0 = clk::now();
+    while (received < n) {
+        int rc = collect();
+        if (rc) return rc;
+        if (std::chrono::duration<double>(clk::now() - t0).count() > 600)
+            return ddp_set_err(ctx, DDP_ERR_TIMEOUT, "replay: %d of %d requests unfinished", n - received, n);
+    }
+    // times relative to the replay start
+    for (int32_t i = 0; i < n; ++i) {
+        out[i].submit_ms -= start;
+        out[i].done_ms -= start;
+    }
+    return DDP_OK;
+}

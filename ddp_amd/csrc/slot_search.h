@@ -1,0 +1,182 @@
+// One query searched by one workgroup ("slot"): used by the batched kernel
+// and the serving kernel.  The slot owns an N-entry (f, parent) array, a log
+// of the nodes it touched (reset when the query ends, so the array never
+// needs a full clear) and B lane-owned heaps; an improved node is pushed into
+// the heap of the lane that relaxed it, so a search never leaves its
+// workgroup.
+#pragma once
+
+#include "search_expand.h"
+
+enum : uint32_t { QERR_HEAP = 1, QERR_ITERS = 2, QERR_BADQ = 4, QERR_PATH = 8 };
+
+template <int B>
+struct SlotLds {
+    ExpandLds<B> L;
+    uint32_t red[16];
+    uint32_t F;        // best f of the target so far
+    uint32_t err;
+    int32_t ntouch;
+    int32_t plen;
+};
+
+template <int B>
+struct SlotPush {
+    const LaneHeap *H;
+    int32_t *size;
+    uint32_t *s_F;
+    int32_t *touched;
+    int32_t *s_ntouch;
+    uint32_t err = 0;
+
+    __device__ void target(uint32_t f) { atomicMin(s_F, f); }
+    __device__ void touch(int32_t c, uint64_t prev)
+    {
+        if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(s_ntouch, 1)] = c;
+    }
+    __device__ void operator()(bool has, uint64_t key)
+    {
+        if (has && !heap_push(*H, *size, key)) err |= QERR_HEAP;
+    }
+};
+
+template <int HK>
+__device__ __forceinline__ void set_target(HeurView &hv, int32_t t)
+{
+    if constexpr (HK == DDP_H_MANHATTAN) {
+        hv.ty = t / hv.width;
+        hv.tx = t - hv.ty * hv.width;
+    } else if constexpr (HK == DDP_H_EUCLID) {
+        hv.tx = hv.xy[2 * t];
+        hv.ty = hv.xy[2 * t + 1];
+    }
+}
+
+// Runs one query to completion.  On return (block-uniform) S.F holds the
+// target's cost (DDP_INF_F if unreachable) and S.err the error flags; gp and
+// touched still describe the search so the caller can walk the path.
+template <int B, int HK>
+__device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched, const LaneHeap &H,
+                            uint64_t *s_heap, int32_t N, int32_t s, int32_t t, int32_t maxit, SlotLds<B> &S,
+                            ExpandCounters &C, uint64_t &stale)
+{
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+        S.F = DDP_INF_F;
+        S.ntouch = 0;
+        S.err = 0;
+    }
+    __syncthreads();
+    bool run = s >= 0 && s < N && t >= 0 && t < N;
+    int32_t size = 0;
+    if (run) {
+        set_target<HK>(hv, t);
+        const int32_t hs = heur_eval<HK>(hv, s);
+        if (HK == DDP_H_ARRAY && hs < 0) {
+            run = false;  // h(s) = -1: s cannot reach t
+        } else if (s == t) {
+            run = false;
+            if (lane == 0) S.F = (uint32_t)hs;
+        } else if (lane == 0) {
+            gp[s] = ((uint64_t)(uint32_t)hs << 32) | 0xFFFFFFFFull;
+            touched[0] = s;
+            S.ntouch = 1;
+            heap_push(H, size, ((uint64_t)(uint32_t)hs << 32) | (uint32_t)s);
+        }
+    } else if (lane == 0) {
+        S.err = QERR_BADQ;
+    }
+    block_sync_global();
+    SlotPush<B> push{&H, &size, &S.F, touched, &S.ntouch};
+    if (run) {
+        for (int32_t it = 0;; ++it) {
+            const uint32_t mytop = size > 0 ? (uint32_t)(s_heap[lane] >> 32) : 0xFFFFFFFFu;
+            const uint32_t topmin = block_min_u32<B>(mytop, S.red);
+            if constexpr (B == 64) __syncthreads();
+            if (S.F <= topmin) break;  // no queued node can improve t
+            if (it >= maxit) {
+                if (lane == 0) atomicOr(&S.err, (uint32_t)QERR_ITERS);
+                break;
+            }
+            int32_t u = -1;
+            uint32_t fu = 0;
+            while (size > 0) {
+                uint64_t key;
+                heap_pop(H, size, key);
+                const int32_t node = (int32_t)(uint32_t)key;
+                const uint32_t f = (uint32_t)(key >> 32);
+                if ((uint32_t)(ld_gp(gp + node) >> 32) == f) {
+                    u = node;
+                    fu = f;
+                    break;
+                }
+                ++stale;
+            }
+            expand_block<B, HK>(G, hv, gp, t, u, fu, S.L, C, push);
+            if (push.err) {
+                atomicOr(&S.err, push.err);
+                push.err = 0;
+            }
+        }
+    }
+    block_sync_global();
+}
+
+// Lane 0 writes the path s..t into out[0..cap) and returns its length in
+// S.plen (0 if unreachable, -len if longer than cap).
+template <int B>
+__device__ void slot_path(const uint64_t *gp, int32_t N, int32_t s, int32_t t, int32_t *out, int32_t cap,
+                          SlotLds<B> &S)
+{
+    if (threadIdx.x == 0) {
+        int32_t n = 0;
+        if (S.F != DDP_INF_F) {
+            int32_t v = t;
+            for (int32_t step = 0; step <= N; ++step) {
+                if (n < cap) out[n] = v;
+                ++n;
+                if (v == s) break;
+                const uint32_t p = (uint32_t)ld_gp(gp + v);
+                if (p >= (uint32_t)N) {
+                    n = 0;
+                    atomicOr(&S.err, (uint32_t)QERR_PATH);
+                    break;
+                }
+                v = (int32_t)p;
+            }
+            if (n > 0 && n <= cap)
+                for (int32_t i = 0; i < n / 2; ++i) {  // t..s -> s..t (this lane wrote every entry)
+                    const int32_t x = out[i];
+                    out[i] = out[n - 1 - i];
+                    out[n - 1 - i] = x;
+                }
+        }
+        S.plen = (n > cap) ? -n : n;
+    }
+}
+
+// Clears every (f, parent) word the query touched.  Starts with a barrier so
+// lane 0's path walk has finished reading parents; ends with one so the next
+// query's atomics see the cleared words.
+template <int B>
+__device__ void slot_reset(uint64_t *gp, const int32_t *touched, SlotLds<B> &S)
+{
+    __syncthreads();
+    const int32_t nt = S.ntouch;
+    for (int32_t i = (int32_t)threadIdx.x; i < nt; i += B)
+        gp[__hip_atomic_load(touched + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)] = DDP_GP_INF;
+    block_sync_global();
+}
+
+template <int B>
+__device__ void slot_flush_stats(unsigned long long *stats, const ExpandCounters &C, uint64_t stale)
+{
+    const uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
+                   v3 = wave_sum_u64(stale);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(stats + 0, (unsigned long long)v0);
+        atomicAdd(stats + 1, (unsigned long long)v1);
+        atomicAdd(stats + 2, (unsigned long long)v2);
+        atomicAdd(stats + 3, (unsigned long long)v3);
+    }
+}

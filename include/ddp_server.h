@@ -1,0 +1,84 @@
+/*
+ * ddp_server.h -- online A* serving on one GPU with continuous batching.
+ *
+ * No reference counterpart: the reference answers one query per process run
+ * (GA_star/GA_v2.cu:298-577) and applies update batches in a blocking host
+ * loop (DA_star/DA_dyn_insert_del_nocycle.cu:1490-2111).  A request loop that
+ * replaces those binds these entry points.
+ *
+ * A persistent kernel keeps the graph and every in-flight query resident in
+ * HBM.  Requests (point-to-point queries and edge-update batches) are
+ * appended to a ring in pinned host memory; each workgroup slot claims the
+ * next request as soon as it is free, so requests that arrive while others
+ * run are admitted as slots free up and overlap on the device.  Slot count is
+ * fixed at creation from free HBM (each slot owns an N-entry (f, parent) array
+ * and its heaps).
+ *
+ * Graph versions: update batch number k (in submission order) creates
+ * version k; a query submitted after k batches sees exactly version k.  Edges
+ * carry the version that deleted them and inserted edges the version that
+ * added them, so a query's answer depends only on (s, t, version), never on
+ * what else is running.  Update semantics per batch are the reference's
+ * (DA_star/GA_merge.cu:1019-1089): each delete removes the first instance of
+ * u->v still live at the previous version; inserts are appended.
+ */
+#ifndef DDP_SERVER_H
+#define DDP_SERVER_H
+
+#include "ddp_capi.h"
+#include "ddp_io.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ddp_server ddp_server;
+
+typedef struct {
+    int32_t block;          /* lanes (= queues) per query slot: 64..512; 0 = 256 */
+    int32_t slots;          /* 0 = as many as occupancy and free HBM allow */
+    int32_t heap_cap;       /* entries per queue; 0 = 2048 */
+    int32_t ring;           /* request ring entries (power of two); 0 = 65536 */
+    int32_t max_inserts;    /* capacity of the inserted-edge log; 0 = 1<<20 */
+    int32_t idle_ms;        /* the kernel exits after this long without work; 0 = 2000 */
+    int32_t reserved[2];
+} ddp_server_opts_t;
+
+#define DDP_REQ_QUERY 0
+#define DDP_REQ_UPDATE 1
+
+typedef struct {
+    uint64_t tag;           /* caller's id */
+    int32_t type;           /* DDP_REQ_QUERY / DDP_REQ_UPDATE */
+    int32_t cost;           /* queries: Cx of t, DDP_INF_COST if unreachable */
+    int32_t version;        /* graph version the request saw (queries) or made (updates) */
+    int32_t status;         /* 0 ok; 1 heap_cap exceeded; 2 iteration limit; 4 bad id */
+    double submit_ms;       /* host clock (ms since ddp_server_create) */
+    double done_ms;         /* host clock when the result was collected */
+    double device_ms;       /* device time from claim to completion */
+} ddp_server_result_t;
+
+int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *opts,
+                      ddp_server **out);
+int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_t *version);
+/* Non-blocking; DDP_ERR_NOMEM if the ring is full (try again after polling). */
+int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uint64_t tag);
+int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *lines, int32_t n, uint64_t tag);
+/* Collects up to max finished requests in completion order of submission. */
+int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t max, int32_t *n);
+/* Waits until every submitted request has finished (bounded by timeout_ms). */
+int ddp_server_drain(ddp_server *sv, int32_t timeout_ms);
+void ddp_server_destroy(ddp_server *sv);
+
+/* Open-loop replay: request i is submitted at arrival_ms[i] after the start
+ * (queries: s[i], t[i]; updates: type[i] = DDP_REQ_UPDATE with lines
+ * upd[upd_off[i] .. upd_off[i+1])), results are written to out[i]. */
+int ddp_server_replay(ddp_server *sv, int32_t n, const double *arrival_ms, const int32_t *type,
+                      const int32_t *s, const int32_t *t, const int32_t *upd_off,
+                      const ddp_update_t *upd, ddp_server_result_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DDP_SERVER_H */

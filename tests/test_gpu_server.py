@@ -1,0 +1,150 @@
+"""GPU parity of the continuous-batching server: every query's cost equals a
+from-scratch shortest path on the graph version it was bound to (its
+submission position among the update batches), whatever else was running."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import ddp_amd
+from oracle import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def versions(hg, batches):
+    out = [(hg.off, hg.col, hg.w)]
+    for b in batches:
+        out.append(pyoracle.apply_batch(*out[-1], b)[:3])
+    return out
+
+
+def make_trace(hg, n, upd_every, seed):
+    rng = np.random.default_rng(seed)
+    src, dst = hg.reachable_pairs(n, seed)
+    types, ups = [], []
+    ub = hg.gen_updates(n // upd_every + 1, 40, 40, 1, 100, seed + 1).batches
+    k = 0
+    for i in range(n):
+        if i % upd_every == upd_every - 1:
+            types.append(1)
+            ups.append(ub[k])
+            k += 1
+        else:
+            types.append(0)
+            ups.append(None)
+    arrival = np.cumsum(rng.exponential(0.05, n))  # ms
+    return arrival, np.array(types, np.int32), src, dst, ups
+
+
+def check(results, types, src, dst, ups, hg):
+    batches = [u for u in ups if u is not None]
+    vers = versions(hg, batches)
+    cache = {}
+    v = 0
+    for i, r in enumerate(results):
+        assert r["tag"] == i and r["status"] == 0, r
+        if types[i] == 1:
+            v += 1
+            assert r["version"] == v
+            continue
+        assert r["version"] == v
+        key = (v, int(src[i]))
+        if key not in cache:
+            cache[key], _ = pyoracle.dijkstra(*vers[v], int(src[i]))
+        d = cache[key][dst[i]]
+        assert r["cost"] == (int(d) if d >= 0 else ddp_amd.INF_COST), (i, r, d)
+
+
+def test_server_versions_and_costs(ctx):
+    hg = ddp_amd.HostGraph.random(6000, 5, 1, 100, 41)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    arrival, types, src, dst, ups = make_trace(hg, 400, 20, 42)
+    for block in (64, 256):
+        say("server block", block)
+        sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, block=block, slots=48)
+        res = sv.replay(arrival, types, src, dst, ups)
+        say("  info", sv.info(), "p50 device ms", np.median([r["device_ms"] for r in res]))
+        check(res, types, src, dst, ups, hg)
+        sv.close()
+
+
+def test_server_matches_batch_on_each_version(ctx):
+    """Same answer alone (batched kernel on a graph holding only that version)
+    and in the mix (server with updates interleaved)."""
+    hg = ddp_amd.HostGraph.geometric(20000, 10, 100000, 43)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    ub = hg.gen_updates(3, 200, 200, 1, 100, 44).batches
+    src, dst = hg.geo_pairs(60, 500, 100000, 45)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_EUCLID, slots=16)
+    tags = 0
+    plan = []  # (version, query index)
+    for v in range(4):
+        if v > 0:
+            assert sv.submit_update(ub[v - 1], 10000 + v) == 0
+        for i in range(v * 15, v * 15 + 15):
+            assert sv.submit_query(int(src[i]), int(dst[i]), i) == 0
+            plan.append((v, i))
+    sv.drain()
+    got = {r["tag"]: r for r in sv.poll(1000)}
+    sv.close()
+    vers = versions(hg, ub)
+    for v in range(4):
+        g = ddp_amd.HostGraph(*vers[v], xy=hg.xy)
+        dgv = ddp_amd.DeviceGraph(ctx, g)
+        idx = [i for (vv, i) in plan if vv == v]
+        cost, _, _ = ddp_amd.astar_batch(ctx, dgv, src[idx], dst[idx], heur=ddp_amd.H_EUCLID)
+        for k, i in enumerate(idx):
+            assert got[i]["version"] == v and got[i]["cost"] == cost[k], (v, i)
+        dgv.close()
+
+
+def test_server_idle_exit_and_relaunch(ctx):
+    hg = ddp_amd.HostGraph.random(3000, 4, 1, 50, 46)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.reachable_pairs(4, 47)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, slots=8, idle_ms=30)
+    for k in range(2):
+        assert sv.submit_query(int(src[2 * k]), int(dst[2 * k]), 2 * k) == 0
+        assert sv.submit_query(int(src[2 * k + 1]), int(dst[2 * k + 1]), 2 * k + 1) == 0
+        sv.drain(60000)
+        res = sv.poll()
+        assert len(res) == 2
+        for r in res:
+            d, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[r["tag"]]))
+            assert r["cost"] == d[dst[r["tag"]]]
+        time.sleep(0.2)  # every workgroup exits after 30 ms idle; the next submit relaunches
+    sv.close()
+
+
+def test_server_ring_backpressure(ctx):
+    hg = ddp_amd.HostGraph.random(3000, 4, 1, 50, 48)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.reachable_pairs(300, 49)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, slots=4, ring=64)
+    done = {}
+    i = 0
+    refused = 0
+    while i < 300:
+        rc = sv.submit_query(int(src[i]), int(dst[i]), i)
+        if rc == 0:
+            i += 1
+        else:
+            assert rc == 3  # DDP_ERR_NOMEM: ring full
+            refused += 1
+            for r in sv.poll():
+                done[r["tag"]] = r
+    sv.drain()
+    for r in sv.poll(1000):
+        done[r["tag"]] = r
+    assert len(done) == 300 and refused > 0
+    for k in range(0, 300, 30):
+        d, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[k]))
+        assert done[k]["cost"] == d[dst[k]]
+    sv.close()
