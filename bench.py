@@ -1,15 +1,18 @@
 """Benchmark: exact-cost A* queries/s, batched vs one-at-a-time GA*, on one MI355X.
 
-Workload (default): 4,096 source/target queries on a random geometric graph
-of 2,000,000 nodes (integer lattice [0,100000]^2, seed 4, mean degree ~10,
-edges both ways, w = ceil(Euclidean distance), h = floor(Euclidean distance
-to the target)), query separations log-uniform in [100, 100000] lattice
-units.  One step = one batched launch answering all 4,096 queries with their
-paths.  `value` = queries/s of the batched launches (graph and queries already
-resident in HBM); the same queries are also answered one at a time by the
-single-query GA* kernel and their rate and latencies are reported beside it.
+Default workload (--workload c4): 4,096 source/target queries on a random
+geometric graph of 2,000,000 nodes (integer lattice [0,100000]^2, seed 4, mean
+degree ~10, edges both ways, w = ceil(Euclidean distance), h = floor(Euclidean
+distance to the target)), query separations log-uniform in [100, 100000]
+lattice units.  One step = one batched launch answering all 4,096 queries with
+their paths.  `value` = queries/s of the batched launches (graph and queries
+already resident in HBM); the same queries are also answered one at a time by
+the single-query GA* kernel, and served open-loop with update batches
+interleaved, and those rates and latencies are reported beside it.
 
-Prints one JSON line.  Only --gpus 1 is supported (one GPU per box).
+Other workloads (one JSON line each): c1 single GA* query on a random graph,
+c2 grid maze with a K sweep, c3 DA* update batches vs full recompute, c5
+served R-MAT queries at offered loads.  Only --gpus 1 is supported.
 """
 import argparse
 import json
@@ -30,18 +33,38 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def geometric_workload(nq, seed_graph=4, seed_pairs=4):
+def pct(x, q):
+    return round(float(np.percentile(np.asarray(x), q)), 3)
+
+
+def base_record(args, value, unit, ms_per_step, workload, higher=True, dtype="int32"):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": higher,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": {"workload": workload}}
+
+
+def roofline(abytes, kernel_ms, kernel, copy_gbps=None):
+    achieved = abytes / (kernel_ms * 1e-3) / 1e9
+    r = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+         "kernel_ms": round(kernel_ms, 3), "algorithmic_bytes_per_launch": int(abytes)}
+    if copy_gbps:
+        r["copy_gbps"] = round(copy_gbps, 1)
+        r["frac_of_copy"] = round(achieved / copy_gbps, 5)
+    return r
+
+
+def copy_bandwidth(ctx):
+    """Measured device copy bandwidth (copy_kernel, 2 GiB; also the known-traffic
+    kernel the HBM counters in profiles/ are scaled by)."""
+    from ctypes import byref, c_float
+
     import ddp_amd
-    hg = ddp_amd.HostGraph.geometric(2000000, 10, 100000, seed_graph)
-    src, dst = hg.geo_pairs(nq, 100, 100000, seed_pairs)
-    return hg, src, dst
-
-
-def algorithmic_bytes(st):
-    # per expansion: 8 B row offsets + 4 B g(u) + 8 B coordinates of u;
-    # per scanned edge: 4 B column + 4 B weight + 4 B g(v) + 8 B coordinates of v;
-    # per improvement: 8 B (g, parent) write
-    return 20 * st["expansions"] + 20 * st["edges"] + 8 * st["improvements"]
+    ms, gb = c_float(), c_float()
+    if ddp_amd._capi.lib().ddp_copy_bandwidth(ctx.handle, 1 << 31, 3, byref(ms), byref(gb)) != 0:
+        return None
+    return gb.value
 
 
 def euclid_h(xy, t):
@@ -54,15 +77,23 @@ def euclid_h(xy, t):
     return r.astype(np.int32)
 
 
-def cpu_baseline(hg, src, dst, costs, budget_s):
+# --------------------------------------------------------------------- C4
+
+def algorithmic_bytes_geo(st):
+    # per expansion: 8 B row offsets + 4 B g(u) + 8 B coordinates of u;
+    # per scanned edge: 4 B column + 4 B weight + 4 B g(v) + 8 B coordinates of v;
+    # per improvement: 8 B (g, parent) write
+    return 20 * st["expansions"] + 20 * st["edges"] + 8 * st["improvements"]
+
+
+def cpu_baseline_geo(hg, src, dst, costs, budget_s):
     """The oracle's restatement of GA_star/A_star_cpu.cpp (binary heap,
     linear-scan decrease-key) on one host core, over uniformly chosen queries
     until the time budget is spent.  Every cost is checked against the GPU."""
     from oracle import pyoracle
     rng = np.random.default_rng(0)
-    order = rng.permutation(len(src))
     done, el = 0, 0.0
-    for i in order:
+    for i in rng.permutation(len(src)):
         h = euclid_h(hg.xy, int(dst[i]))
         t0 = time.perf_counter()
         found, cost, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, int(src[i]), int(dst[i]), quirk=False)
@@ -77,29 +108,78 @@ def cpu_baseline(hg, src, dst, costs, budget_s):
                       f"(linear-scan decrease-key), h = floor(Euclid) array per target, {el:.1f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--queries", type=int, default=4096)
-    ap.add_argument("--block", type=int, default=0, help="lanes per query workgroup (0 = autotune)")
-    ap.add_argument("--single-sample", type=int, default=256, help="queries answered one at a time")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box has a single MI355X "
-              f"and no multi-GPU path is built", file=sys.stderr)
-        sys.exit(2)
+def geometric_updates(hg, n, per, seed):
+    """Update batches that keep the Euclidean heuristic admissible: each insert
+    u->v weighs at least ceil(dist(u, v)); deletes take existing edges."""
+    rng = np.random.default_rng(seed)
+    p = hg.xy.reshape(-1, 2).astype(np.int64)
+    o = np.append(hg.off, hg.E)
+    out = []
+    for _ in range(n):
+        lines = []
+        for u in rng.integers(0, hg.N, per):
+            if o[u + 1] > o[u]:
+                lines.append((0, int(u), int(hg.col[rng.integers(o[u], o[u + 1])]), 0))
+        for u, v in zip(rng.integers(0, hg.N, per), rng.integers(0, hg.N, per)):
+            d2 = int(((p[u] - p[v]) ** 2).sum())
+            r = int(np.floor(np.sqrt(float(d2))))
+            while r * r > d2:
+                r -= 1
+            while (r + 1) * (r + 1) <= d2:
+                r += 1
+            w = r if r * r == d2 else r + 1
+            lines.append((1, int(u), int(v), max(1, w) + int(rng.integers(0, 1000))))
+        out.append(lines)
+    return out
 
+
+def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed):
+    """Open-loop replay: Poisson arrivals at `qps`, 95% point queries and 5%
+    update batches.  Returns (results, types, query index per request, secs)."""
+    import ddp_amd
+    rng = np.random.default_rng(seed)
+    types = (rng.random(nreq) < 0.05).astype(np.int32)
+    types[0] = 0
+    types[np.cumsum(types) > len(upd_batches)] = 0
+    qi = rng.integers(0, len(src), nreq)
+    ups, k = [], 0
+    for ty in types:
+        ups.append(upd_batches[k] if ty else None)
+        k += int(ty)
+    arrival = np.cumsum(rng.exponential(1000.0 / qps, nreq))
+    sv = ddp_amd.Server(ctx, dg, heur=heur)
+    t0 = time.perf_counter()
+    out = sv.replay(arrival, types, src[qi], dst[qi], ups)
+    el = time.perf_counter() - t0
+    info = sv.info()
+    sv.close()
+    return out, types, qi, el, info
+
+
+def served_summary(out, types, qi, el, info, qps, costs=None):
+    q = [r for r, ty in zip(out, types) if ty == 0]
+    bad = [r for r in out if r["status"] != 0]
+    if bad:
+        raise SystemExit(f"{len(bad)} served requests failed, first: {bad[0]}")
+    if costs is not None:  # requests before the first update saw the unmodified graph
+        first = int(np.argmax(types)) if types.any() else len(types)
+        for i in range(first):
+            if out[i]["cost"] != costs[qi[i]]:
+                raise SystemExit(f"served query {i} disagrees with the batch: {out[i]['cost']} vs {costs[qi[i]]}")
+    lat = [r["done_ms"] - r["submit_ms"] for r in q]
+    return {"offered_qps": qps, "requests": len(out), "updates": int(types.sum()), "slots": info["slots"],
+            "queries_per_s": round(len(q) / el, 1), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+            "device_p50_ms": pct([r["device_ms"] for r in q], 50)}
+
+
+def run_c4(args):
     import ddp_amd
     t0 = time.time()
-    hg, src, dst = geometric_workload(args.queries)
+    hg = ddp_amd.HostGraph.geometric(2000000, 10, 100000, 4)
+    src, dst = hg.geo_pairs(args.queries, 100, 100000, 4)
     log(f"workload ready in {time.time() - t0:.1f}s: N={hg.N} E={hg.E}")
     ctx = ddp_amd.Context(0)
     dg = ddp_amd.DeviceGraph(ctx, hg)
-
     block = args.block
     if block == 0:
         from ddp_amd import autotune
@@ -121,10 +201,10 @@ def main():
     out = b.download(paths=True)
     if stats["errors"]:
         raise SystemExit(f"{stats['errors']} batched queries reported errors")
-    costs = out["cost"]
-    lat = out["latency_ms"]
+    costs, lat = out["cost"], out["latency_ms"]
     info = b.info()
     b.close()
+    log(f"batched: {args.steps} steps in {elapsed:.3f}s")
 
     # the same queries, one at a time through the single-query GA* kernel
     rng = np.random.default_rng(1)
@@ -138,45 +218,274 @@ def main():
         if r.cost != costs[i]:
             raise SystemExit(f"single-query GA* disagrees with the batch on query {i}: {r.cost} vs {costs[i]}")
     single_qps = len(sample) / (time.perf_counter() - ts)
+    log(f"one at a time: {len(sample)} queries, {single_qps:.1f} q/s")
 
     value = len(src) * args.steps / elapsed
     kmean = float(np.mean(kms))
-    abytes = algorithmic_bytes(stats)
-    achieved = abytes / (kmean * 1e-3) / 1e9
-    res = {
-        "metric": METRIC,
-        "value": round(value, 1),
-        "unit": "queries/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed * 1e3 / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "int32",
-        "data": "synthetic",
-        "config": {"workload": "C4 batched A*: random geometric graph N=2,000,000 (seed 4, lattice 1e5, "
-                               "mean degree ~10), 4,096 queries with log-uniform separation, Euclidean "
-                               "heuristic, one persistent launch per step",
-                   "N": hg.N, "E": hg.E, "queries": len(src), "block": info["block"], "slots": info["slots"],
-                   "path_cap": 4096},
-        "p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+    res = base_record(args, round(value, 1), "queries/s", round(elapsed * 1e3 / args.steps, 3),
+                      "C4 batched A*: random geometric graph N=2,000,000 (seed 4, lattice 1e5, mean degree ~10), "
+                      "4,096 queries with log-uniform separation, Euclidean heuristic, one persistent launch "
+                      "per step")
+    res["config"].update({"N": hg.N, "E": hg.E, "queries": len(src), "block": info["block"],
+                          "slots": info["slots"], "path_cap": 4096})
+    res.update({
+        "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
         "one_at_a_time": {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
-                          "p50_ms": round(float(np.percentile(single_lat, 50)), 3),
-                          "p99_ms": round(float(np.percentile(single_lat, 99)), 3), "K": 4096},
+                          "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096},
         "speedup_vs_one_at_a_time": round(value / single_qps, 2),
         "expansions_per_s": round(stats["expansions"] / (kmean * 1e-3), 1),
         "edges_per_s": round(stats["edges"] / (kmean * 1e-3), 1),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "batch_kernel", "kernel_ms": round(kmean, 3),
-                     "algorithmic_bytes_per_launch": int(abytes)},
-    }
+        "roofline": roofline(algorithmic_bytes_geo(stats), kmean, "batch_kernel", copy_bandwidth(ctx)),
+    })
+    if not args.no_serve:
+        ups = geometric_updates(hg, args.serve_requests // 10, 100, 8)
+        o, ty, qi, el, sinfo = serve_trace(ctx, dg, ddp_amd.H_EUCLID, src, dst, args.serve_qps, args.serve_requests,
+                                           ups, 7)
+        res["served"] = served_summary(o, ty, qi, el, sinfo, args.serve_qps, costs)
+        log(f"served: {res['served']}")
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(hg, src, dst, costs, args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline_geo(hg, src, dst, costs, args.cpu_budget)
     ctx.close()
+    return res
+
+
+# ----------------------------------------------------------------- C1 / C2
+
+def time_single(ctx, dg, s, t, heur, K, steps, warmup, block=0):
+    import ddp_amd
+    for _ in range(warmup):
+        ddp_amd.astar(ctx, dg, s, t, heur=heur, K=K, block=block)
+    walls, kms, r = [], [], None
+    for _ in range(steps):
+        w0 = time.perf_counter()
+        r = ddp_amd.astar(ctx, dg, s, t, heur=heur, K=K, block=block)
+        walls.append((time.perf_counter() - w0) * 1e3)
+        kms.append(r.stats["kernel_ms"])
+    return r, float(np.mean(walls)), float(np.mean(kms))
+
+
+def run_c1(args):
+    import ddp_amd
+    from oracle import pyoracle
+    hg = ddp_amd.HostGraph.random(100000, 6, 1, 100, 1)
+    src, dst = hg.reachable_pairs(1, 1)
+    s, t = int(src[0]), int(dst[0])
+    ctx = ddp_amd.Context(0)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)
+    minw = int(hg.w.min())
+    h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
+    r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup)
+    # CPU: the oracle's restatement of A_star_cpu.cpp on the same inputs
+    reps, el = 0, 0.0
+    while el < min(args.cpu_budget, 10.0) or reps == 0:
+        c0 = time.perf_counter()
+        found, cost, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=False)
+        el += time.perf_counter() - c0
+        reps += 1
+    if not found or cost != r.cost:
+        raise SystemExit(f"C1 cost mismatch: GPU {r.cost} CPU {cost}")
+    st = r.stats
+    abytes = 16 * st["expansions"] + 16 * st["edges"] + 8 * st["improvements"]
+    res = base_record(args, round(1000.0 / wall, 2), "queries/s", round(wall, 3),
+                      "C1 single GA* query: random N=100,000, 6 out-edges, w in [1,100], seed 1, h = reverse-BFS "
+                      "hops x min weight, K=1024")
+    res["config"].update({"s": s, "t": t, "K": st["K"], "block": st["block"], "cost": r.cost})
+    res.update({"kernel_ms": round(kms, 3), "iterations": st["iterations"], "expansions": st["expansions"],
+                "expansions_per_s": round(st["expansions"] / (kms * 1e-3), 1),
+                "edges_per_s": round(st["edges"] / (kms * 1e-3), 1),
+                "roofline": roofline(abytes, kms, "ga_grid_kernel"),
+                "cpu_baseline": {"value": round(reps / el, 3), "unit": "queries/s", "cores": 1, "kind": "port",
+                                 "sample": f"the same query {reps} times, oracle restatement of A_star_cpu.cpp"}})
+    ctx.close()
+    return res
+
+
+def run_c2(args):
+    import ddp_amd
+    from oracle import pyoracle
+    W = 4096
+    hg = ddp_amd.HostGraph.grid_maze(W, W, 300, 2)
+    s, t = 0, W * W - 1
+    ctx = ddp_amd.Context(0)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg.set_grid(W)
+    sweep = {}
+    cost = None
+    for K in (512, 2048, 8192):
+        r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_MANHATTAN, K, args.steps, args.warmup)
+        st = r.stats
+        if cost is not None and r.cost != cost:
+            raise SystemExit(f"C2 cost differs across K: {r.cost} vs {cost}")
+        cost = r.cost
+        abytes = 8 * st["expansions"] + 12 * st["edges"] + 8 * st["improvements"]  # closed-form h reads nothing
+        sweep[K] = {"wall_ms": round(wall, 3), "kernel_ms": round(kms, 3), "iterations": st["iterations"],
+                    "expansions": st["expansions"], "edges": st["edges"], "block": st["block"],
+                    "roofline": roofline(abytes, kms, "ga_grid_kernel")}
+        log(f"C2 K={K}: {sweep[K]}")
+    best = min(sweep, key=lambda k: sweep[k]["wall_ms"])
+    xs = np.arange(W * W, dtype=np.int64)
+    h = ((W - 1 - xs // W) + (W - 1 - xs % W)).astype(np.int32)
+    c0 = time.perf_counter()
+    found, ccost, cexp = pyoracle.astar_indexed(hg.off, hg.col, hg.w, h, s, t)
+    cel = time.perf_counter() - c0
+    if not found or ccost != cost:
+        raise SystemExit(f"C2 cost mismatch: GPU {cost} CPU {ccost}")
+    res = base_record(args, round(1000.0 / sweep[best]["wall_ms"], 3), "queries/s", sweep[best]["wall_ms"],
+                      "C2 single GA* query: 4096x4096 4-connected grid maze, p_block=0.30, seed 2, carved corridor, "
+                      "unit weights, Manhattan heuristic, corner to corner, K swept")
+    res["config"].update({"N": hg.N, "E": hg.E, "cost": cost, "best_K": best})
+    res.update({"sweep": sweep, "roofline": sweep[best]["roofline"],
+                "cpu_baseline": {"value": round(1.0 / cel, 4), "unit": "queries/s", "cores": 1,
+                                 "kind": "port", "sample": f"the full query once ({cexp} expansions), indexed-heap "
+                                 f"variant of the A_star_cpu.cpp restatement (the linear-scan decrease-key does not "
+                                 f"finish on 16.7M nodes)"}})
+    ctx.close()
+    return res
+
+
+# --------------------------------------------------------------------- C3
+
+def run_c3(args):
+    import ddp_amd
+    from oracle import pyoracle
+    n_nodes, nb = args.c3_nodes, args.c3_batches
+    hg = ddp_amd.HostGraph.random(n_nodes, 8, 1, 1000, 3)
+    upd = hg.gen_updates(nb, args.c3_batch // 2, args.c3_batch // 2, 1, 1000, 33).batches
+    src, dst = hg.reachable_pairs(1, 3)
+    s, t = int(src[0]), int(dst[0])
+    # heuristic on the union of every version: admissible for all of them
+    o = np.append(hg.off, hg.E)
+    us = [np.repeat(np.arange(hg.N), np.diff(o))]
+    vs, ws = [hg.col], [hg.w]
+    for b in upd:
+        a = np.array([l for l in b if l[0] == 1], np.int64)
+        us.append(a[:, 1])
+        vs.append(a[:, 2])
+        ws.append(a[:, 3])
+    us, vs, ws = np.concatenate(us), np.concatenate(vs).astype(np.int32), np.concatenate(ws).astype(np.uint32)
+    order = np.argsort(us, kind="stable")
+    uoff = np.searchsorted(us[order], np.arange(hg.N)).astype(np.int32)
+    ctx = ddp_amd.Context(0)
+    ug = ddp_amd.DeviceGraph(ctx, ddp_amd.HostGraph(uoff, vs[order], ws[order]), reverse=True)
+    h, _ = ug.bfs_hops(t, reverse=True, scale=int(ws.min()))
+    ug.close()
+    rep = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K)
+    rec = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K)
+    rep_ms, rec_ms, rep_search, rec_search, per = [], [], [], [], []
+    cur = (hg.off, hg.col, hg.w)
+    checked = 0
+    for bi, lines in enumerate(upd):
+        c1, st1 = rep.apply(lines, mode=ddp_amd.DAStar.REPAIR)
+        c2, st2 = rec.apply(lines, mode=ddp_amd.DAStar.RECOMPUTE)
+        if c1 != c2:
+            raise SystemExit(f"C3 batch {bi}: repair {c1} != recompute {c2}")
+        rep_ms.append(st1["total_ms"])
+        rec_ms.append(st2["total_ms"])
+        rep_search.append(st1["repair_ms"] + st1["search_ms"])
+        rec_search.append(st2["search_ms"])
+        per.append({"batch": bi, "cost": c1, "invalidated": st1["invalidated"], "repair_ms": round(st1["total_ms"], 3),
+                    "recompute_ms": round(st2["total_ms"], 3)})
+        if bi % max(1, nb // 5) == 0 or bi == nb - 1:
+            cur_full = cur
+            for b2 in upd[checked:bi + 1]:
+                cur_full = pyoracle.apply_batch(*cur_full, b2)[:3]
+            cur, checked = cur_full, bi + 1
+            d, _ = pyoracle.dijkstra(*cur, s, t)
+            want = int(d[t]) if d[t] >= 0 else ddp_amd.INF_COST
+            if c1 != want:
+                raise SystemExit(f"C3 batch {bi}: GPU {c1} oracle {want}")
+        log(f"C3 batch {bi}: {per[-1]}")
+    rep.close()
+    rec.close()
+    mean_rep, mean_rec = float(np.mean(rep_ms)), float(np.mean(rec_ms))
+    res = base_record(args, round(1000.0 / mean_rep, 3), "batches/s", round(mean_rep, 3),
+                      f"C3 DA*: random N={n_nodes:,}, out-degree 8, w in [1,1000], seed 3, {nb} batches of "
+                      f"{args.c3_batch // 2:,} inserts + {args.c3_batch // 2:,} deletes, h on the union graph")
+    res["config"].update({"N": hg.N, "E": hg.E, "K": args.c3_K, "s": s, "t": t})
+    res.update({"repair_ms_mean": round(mean_rep, 3), "recompute_ms_mean": round(mean_rec, 3),
+                "repair_device_ms_mean": round(float(np.mean(rep_search)), 3),
+                "recompute_device_ms_mean": round(float(np.mean(rec_search)), 3),
+                "speedup_vs_recompute": round(mean_rec / mean_rep, 2), "batches": per})
+    ctx.close()
+    return res
+
+
+# --------------------------------------------------------------------- C5
+
+def run_c5(args):
+    import ddp_amd
+    scale = args.rmat_scale
+    t0 = time.time()
+    hg = ddp_amd.HostGraph.rmat(scale, 16 * (1 << scale), 0.57, 0.19, 0.19, 1, 1000, 5)
+    log(f"R-MAT scale {scale}: N={hg.N} E={hg.E} in {time.time() - t0:.1f}s")
+    src, dst = hg.reachable_pairs(args.queries, 5)
+    rng = np.random.default_rng(9)
+    o = np.append(hg.off, hg.E)
+    ups = []
+    for _ in range(max(1, args.serve_requests // 10)):
+        lines = []
+        for u in rng.integers(0, hg.N, 1000):
+            if o[u + 1] > o[u]:
+                lines.append((0, int(u), int(hg.col[rng.integers(o[u], o[u + 1])]), 0))
+        for u, v, w in zip(rng.integers(0, hg.N, 1000), rng.integers(0, hg.N, 1000), rng.integers(1, 1001, 1000)):
+            lines.append((1, int(u), int(v), int(w)))
+        ups.append(lines)
+    ctx = ddp_amd.Context(0)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    # one-at-a-time GA* on a few of the queries, and the batched kernel on all of them (unmodified graph)
+    cost_b, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO)
+    single, lat1 = [], []
+    for i in range(min(args.single_sample, 16)):
+        q0 = time.perf_counter()
+        r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_ZERO, K=8192)
+        lat1.append((time.perf_counter() - q0) * 1e3)
+        if r.cost != cost_b[i]:
+            raise SystemExit(f"C5 query {i}: single {r.cost} batch {cost_b[i]}")
+    sweep = []
+    for qps in args.c5_qps:
+        o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.serve_requests, ups, 11)
+        sweep.append(served_summary(o_, ty, qi, el, info, qps, cost_b))
+        log(f"C5 served {sweep[-1]}")
+    best = max(sweep, key=lambda x: x["queries_per_s"])
+    res = base_record(args, best["queries_per_s"], "queries/s", None,
+                      f"C5 online serving: R-MAT scale {scale} ({hg.N:,} nodes, {hg.E:,} edges, a/b/c/d = "
+                      f"0.57/0.19/0.19/0.05, seed 5), w in [1,1000], h = 0, open-loop Poisson, 95% queries, 5% "
+                      f"update batches of 1,000 inserts + 1,000 deletes")
+    res["config"].update({"N": hg.N, "E": hg.E})
+    res.update({"p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "sweep": sweep,
+                "batched_queries_per_s": round(len(src) / (bst["kernel_ms"] * 1e-3), 1),
+                "one_at_a_time": {"queries_per_s": round(1000.0 / float(np.mean(lat1)), 2),
+                                  "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99), "sample": len(lat1)}})
+    ctx.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--queries", type=int, default=4096)
+    ap.add_argument("--block", type=int, default=0, help="lanes per query workgroup (0 = autotune)")
+    ap.add_argument("--single-sample", type=int, default=256, help="queries answered one at a time")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=5000.0, help="offered load of the served leg")
+    ap.add_argument("--serve-requests", type=int, default=8192)
+    ap.add_argument("--c3-nodes", type=int, default=1000000)
+    ap.add_argument("--c3-batches", type=int, default=20)
+    ap.add_argument("--c3-batch", type=int, default=20000, help="lines per batch (half inserts, half deletes)")
+    ap.add_argument("--c3-K", type=int, default=4096)
+    ap.add_argument("--rmat-scale", type=int, default=23)
+    ap.add_argument("--c5-qps", type=float, nargs="+", default=[50.0, 200.0, 1000.0])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box has a single MI355X "
+              f"and no multi-GPU path is built", file=sys.stderr)
+        sys.exit(2)
+    res = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -98,6 +98,7 @@ SIGNATURES = {
     "ddp_create_error": (c_char_p, []),
     "ddp_device_info": (c_int, [c_void_p, POINTER(DeviceInfo)]),
     "ddp_ctx_synchronize": (c_int, [c_void_p]),
+    "ddp_copy_bandwidth": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float), POINTER(c_float)]),
     "ddp_graph_upload": (c_int, [c_void_p, c_int32, c_int32, P_I32, P_I32, P_U32, c_uint32, PP]),
     "ddp_graph_destroy": (None, [c_void_p]),
     "ddp_graph_info": (c_int, [c_void_p, P_I32, P_I32, P_U32, P_I32]),
@@ -135,6 +136,11 @@ SIGNATURES = {
     "ddp_server_destroy": (None, [c_void_p]),
     "ddp_server_replay": (c_int, [c_void_p, c_int32, P_F64, P_I32, P_I32, P_I32, P_I32, POINTER(Update),
                                   POINTER(ServerResult)]),
+    # ddp_tune.h
+    "ddp_tune_batch": (c_int, [c_void_p, c_void_p, Heur, c_int32, P_I32, P_I32, c_int32, POINTER(BatchOpts),
+                               POINTER(c_float)]),
+    "ddp_tune_astar": (c_int, [c_void_p, c_void_p, Heur, c_int32, P_I32, P_I32, c_int32, POINTER(AstarOpts),
+                               POINTER(c_float)]),
     # ddp_io.h
     "ddp_hgraph_from_arrays": (c_int, [c_int32, c_int32, P_I32, P_I32, P_U32, PP]),
     "ddp_hgraph_free": (None, [c_void_p]),

@@ -124,6 +124,41 @@ extern "C" int ddp_ctx_synchronize(ddp_ctx *ctx)
     return DDP_OK;
 }
 
+__global__ void copy_kernel(const int4 *__restrict__ a, int4 *__restrict__ b, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+extern "C" int ddp_copy_bandwidth(ddp_ctx *ctx, int64_t bytes, int32_t reps, float *ms, float *gbps)
+{
+    if (!ctx || bytes < 16 || reps < 1) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_copy_bandwidth: bad argument");
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    const int64_t n = bytes / 16;
+    int4 *a = nullptr, *b = nullptr;
+    DDP_HIP(ctx, hipMalloc((void **)&a, (size_t)n * 16));
+    hipError_t e = hipMalloc((void **)&b, (size_t)n * 16);
+    if (e == hipSuccess) e = hipMemsetAsync(a, 1, (size_t)n * 16, ctx->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b, 0, (size_t)n * 16, ctx->stream);
+    float best = 1e30f;
+    const int grid = ctx->prop.multiProcessorCount * 16;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        hipEventRecord(ctx->ev0, ctx->stream);
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, ctx->stream, a, b, n);
+        hipEventRecord(ctx->ev1, ctx->stream);
+        e = hipEventSynchronize(ctx->ev1);
+        float t = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+        best = std::min(best, t);
+    }
+    hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) return ddp_hip_err(ctx, e, "copy kernel", __FILE__, __LINE__);
+    if (ms) *ms = best;
+    if (gbps) *gbps = (float)(2.0 * (double)n * 16 / (best * 1e-3) / 1e9);
+    return DDP_OK;
+}
+
 // ------------------------------------------------------------------ graph
 
 extern "C" void ddp_graph_destroy(ddp_graph *g)

@@ -1,0 +1,179 @@
+// Launch-shape autotuner (see include/ddp_tune.h): times candidate shapes on
+// a sample of the caller's own queries and caches the winner per graph.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/ddp_tune.h"
+
+namespace {
+
+std::string cache_path()
+{
+    if (const char *p = std::getenv("DDP_TUNE_CACHE")) return p;
+    const char *home = std::getenv("HOME");
+    if (!home) return "";
+    std::string dir = std::string(home) + "/.cache";
+    mkdir(dir.c_str(), 0755);
+    dir += "/ddp_amd";
+    mkdir(dir.c_str(), 0755);
+    return dir + "/tune.txt";
+}
+
+std::string key(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const char *what, int32_t nq)
+{
+    int32_t N = 0, E = 0, md = 0;
+    uint32_t mw = 0;
+    ddp_graph_info(g, &N, &E, &mw, &md);
+    ddp_device_info_t di;
+    std::memset(&di, 0, sizeof di);
+    ddp_device_info(ctx, &di);
+    std::ostringstream k;
+    k << what << ":" << di.arch << ":" << N << ":" << E << ":" << md << ":" << mw << ":" << h.kind << ":" << nq;
+    return k.str();
+}
+
+bool cache_get(const std::string &k, std::vector<long> &vals)
+{
+    const std::string p = cache_path();
+    if (p.empty()) return false;
+    std::ifstream f(p);
+    std::string line;
+    bool found = false;
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        std::string kk;
+        ss >> kk;
+        if (kk != k) continue;
+        vals.clear();
+        long v;
+        while (ss >> v) vals.push_back(v);
+        found = true;  // last entry wins
+    }
+    return found;
+}
+
+void cache_put(const std::string &k, const std::vector<long> &vals)
+{
+    const std::string p = cache_path();
+    if (p.empty()) return;
+    std::ofstream f(p, std::ios::app);
+    if (!f) return;
+    f << k;
+    for (long v : vals) f << " " << v;
+    f << "\n";
+}
+
+// Every stride-th query: long and short searches stay represented.
+void sample_queries(int32_t nq, const int32_t *src, const int32_t *dst, int32_t sample, std::vector<int32_t> &s,
+                    std::vector<int32_t> &t)
+{
+    const int32_t n = std::max(1, std::min(nq, sample));
+    const double stride = (double)nq / n;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t k = std::min(nq - 1, (int32_t)(i * stride));
+        s.push_back(src[k]);
+        t.push_back(dst[k]);
+    }
+}
+
+}  // namespace
+
+extern "C" int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
+                              const int32_t *dst, int32_t sample, ddp_batch_opts_t *best, float *qps)
+{
+    if (!ctx || !g || !best || nq <= 0 || !src || !dst) return DDP_ERR_ARG;
+    const std::string k = key(ctx, g, h, "batch", std::min(nq, sample));
+    std::vector<long> cached;
+    if (cache_get(k, cached) && cached.size() >= 2) {
+        std::memset(best, 0, sizeof *best);
+        best->block = (int32_t)cached[0];
+        if (qps) *qps = (float)cached[1];
+        return DDP_OK;
+    }
+    std::vector<int32_t> s, t;
+    sample_queries(nq, src, dst, sample, s, t);
+    const int32_t n = (int32_t)s.size();
+    float best_qps = -1.f;
+    int32_t best_block = 256;
+    for (int32_t B : {128, 256, 512}) {
+        ddp_batch_opts_t o;
+        std::memset(&o, 0, sizeof o);
+        o.block = B;
+        ddp_batch *b = nullptr;
+        if (ddp_batch_create(ctx, g, h, n, &o, &b) != DDP_OK) continue;
+        float ms = 1e30f;
+        bool ok = ddp_batch_upload(b, n, s.data(), t.data()) == DDP_OK;
+        for (int rep = 0; ok && rep < 3; ++rep) {
+            ddp_batch_stats_t st;
+            ok = ddp_batch_launch(b) == DDP_OK && ddp_batch_wait(b, &st) == DDP_OK && st.errors == 0;
+            if (ok && rep > 0) ms = std::min(ms, st.kernel_ms);  // first launch warms up
+        }
+        ddp_batch_destroy(b);
+        if (!ok) continue;
+        const float q = n / (ms * 1e-3f);
+        if (q > best_qps) {
+            best_qps = q;
+            best_block = B;
+        }
+    }
+    if (best_qps < 0) return DDP_ERR_STATE;
+    std::memset(best, 0, sizeof *best);
+    best->block = best_block;
+    if (qps) *qps = best_qps;
+    cache_put(k, {best_block, (long)best_qps});
+    return DDP_OK;
+}
+
+extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
+                              const int32_t *dst, int32_t sample, ddp_astar_opts_t *best, float *mean_ms)
+{
+    if (!ctx || !g || !best || nq <= 0 || !src || !dst) return DDP_ERR_ARG;
+    const std::string k = key(ctx, g, h, "astar", std::min(nq, sample));
+    std::vector<long> cached;
+    if (cache_get(k, cached) && cached.size() >= 3) {
+        std::memset(best, 0, sizeof *best);
+        best->K = (int32_t)cached[0];
+        best->block = (int32_t)cached[1];
+        if (mean_ms) *mean_ms = cached[2] * 1e-3f;
+        return DDP_OK;
+    }
+    std::vector<int32_t> s, t;
+    sample_queries(nq, src, dst, sample, s, t);
+    double best_ms = 1e300;
+    ddp_astar_opts_t win;
+    std::memset(&win, 0, sizeof win);
+    for (int32_t K : {512, 1024, 2048, 4096, 8192})
+        for (int32_t B : {64, 256}) {
+            if (K < B) continue;
+            ddp_astar_opts_t o;
+            std::memset(&o, 0, sizeof o);
+            o.K = K;
+            o.block = B;
+            double total = 0;
+            bool ok = true;
+            for (size_t i = 0; i < s.size() && ok; ++i) {
+                int32_t cost = 0;
+                ddp_astar_stats_t st;
+                ok = ddp_astar(ctx, g, h, s[i], t[i], &o, &cost, nullptr, nullptr, 0, nullptr, &st) == DDP_OK;
+                total += st.total_ms;
+            }
+            if (ok && total < best_ms) {
+                best_ms = total;
+                win = o;
+            }
+        }
+    if (best_ms >= 1e300) return DDP_ERR_STATE;
+    *best = win;
+    const double mean = best_ms / s.size();
+    if (mean_ms) *mean_ms = (float)mean;
+    cache_put(k, {win.K, win.block, (long)(mean * 1e3)});
+    return DDP_OK;
+}

@@ -68,6 +68,12 @@ int ddp_device_info(ddp_ctx *ctx, ddp_device_info_t *info);
 /* Waits for all work on the context's device (the reference's
  * cudaDeviceSynchronize after each launch, GA_star/GA_v2.cu:495). */
 int ddp_ctx_synchronize(ddp_ctx *ctx);
+/* No reference counterpart: a plain device-to-device copy of `bytes` (a
+ * kernel named copy_kernel, 16-byte loads and stores), timed `reps` times
+ * with HIP events.  Gives the measured copy bandwidth the roofline is also
+ * reported against, and a kernel of known traffic to scale HBM counters by.
+ * *ms = best time, *gbps = 2*bytes / best time. */
+int ddp_copy_bandwidth(ddp_ctx *ctx, int64_t bytes, int32_t reps, float *ms, float *gbps);
 
 /* -------------------------------------------------------------------- graph */
 

@@ -253,6 +253,54 @@ ORC_API long long orc_dijkstra(int N, int E, const int *off, const int *col,
     return settled;
 }
 
+/* ---- the same A* with an indexed heap (labelled variant, CPU baseline) ----
+ * A_star_cpu.cpp's search with its O(heap) linear-scan decrease-key replaced
+ * by a position index (O(log heap)); same relax arithmetic, same stop rule
+ * (first pop of t).  Costs equal orc_astar_ref's; heap positions of equal
+ * keys may differ, so parent chains under ties may too.  Pinned by equality
+ * of costs with orc_astar_ref and orc_dijkstra (tests/test_oracle.py). */
+ORC_API int orc_astar_indexed(int N, int E, const int *off, const int *col, const uint32_t *w,
+                              const int *h, int s, int t, int *cost_out, long long *expansions_out)
+{
+    long long *key = (long long *)malloc(sizeof(long long) * (size_t)N);
+    int *cx = (int *)malloc(sizeof(int) * (size_t)N);
+    int *heap = (int *)malloc(sizeof(int) * (size_t)N);
+    int *pos = (int *)malloc(sizeof(int) * (size_t)N);
+    if (!key || !cx || !heap || !pos) { free(key); free(cx); free(heap); free(pos); return -1; }
+    for (int i = 0; i < N; i++) { cx[i] = INT_MAX; pos[i] = -1; }
+    ixheap H = {key, heap, pos, 0};
+    long long expanded = 0;
+    int found = 0;
+    cx[s] = h[s];
+    key[s] = cx[s];
+    heap[0] = s; pos[s] = 0; H.n = 1;
+    while (H.n > 0) {
+        int u = heap[0];
+        ix_swap(&H, 0, H.n - 1);
+        H.n--;
+        pos[u] = -1;
+        if (H.n > 0) ix_down(&H, 0);
+        if (u == t) { found = 1; break; }
+        expanded++;
+        int b = off[u], e = (u == N - 1) ? E : off[u + 1];
+        for (int j = b; j < e; j++) {
+            int c = col[j];
+            if (c < 0) continue;
+            uint32_t cand = (uint32_t)(cx[u] - h[u]) + w[j] + (uint32_t)h[c];
+            if ((uint32_t)cx[c] > cand) {
+                cx[c] = (int)cand;
+                key[c] = cx[c];
+                if (pos[c] < 0) { heap[H.n] = c; pos[c] = H.n; H.n++; }
+                ix_up(&H, pos[c]);
+            }
+        }
+    }
+    *cost_out = cx[t];
+    if (expansions_out) *expansions_out = expanded;
+    free(key); free(cx); free(heap); free(pos);
+    return found;
+}
+
 /*
  * Level-synchronous BFS from t over the graph given (the reference feeds the
  * REVERSED graph: GA_star/bfs_hx.cu:45,111-138), tombstones skipped.

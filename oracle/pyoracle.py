@@ -198,3 +198,24 @@ def chain_from_parent(parent, t, limit=None):
         if len(out) > limit:
             raise ValueError("parent cycle")
     return out
+
+
+def astar_indexed(off, col, w, h, s, t):
+    """Indexed-heap variant of the CPU A* (same costs, O(log) decrease-key).
+    Returns (found, cost, expansions)."""
+    L = lib()
+    if not hasattr(L, "_ix_bound"):
+        L.orc_astar_indexed.restype = c_int
+        L.orc_astar_indexed.argtypes = [c_int, c_int, _P_I, _P_I, _P_U, _P_I, c_int, c_int, _P_I, _P_LL]
+        L._ix_bound = True
+    N, E = len(off), len(col)
+    off, po = _i(off)
+    col, pc = _i(col)
+    w, pw = _u(w)
+    h, ph = _i(h)
+    cost = c_int()
+    ex = c_longlong()
+    found = L.orc_astar_indexed(N, E, po, pc, pw, ph, s, t, ctypes.byref(cost), ctypes.byref(ex))
+    if found < 0:
+        raise MemoryError("oracle allocation failed")
+    return bool(found), cost.value, ex.value

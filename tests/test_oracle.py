@@ -124,6 +124,20 @@ def test_astar_without_quirk_is_dijkstra_with_consistent_h():
             assert cost == dist[t] + h[t]
 
 
+def test_indexed_heap_variant_matches_faithful_astar():
+    rng = np.random.default_rng(17)
+    for _ in range(150):
+        N = int(rng.integers(2, 60))
+        off, col, w = random_graph(rng, N, max_deg=5, wmax=int(rng.choice([1, 3, 40])))
+        s, t = int(rng.integers(0, N)), int(rng.integers(0, N))
+        roff, rcol, rw = pyoracle.transpose(off, col, w)
+        hops, _ = pyoracle.bfs_hops(roff, rcol, t)
+        h = np.where(hops < 0, -1, hops * int(w.min(initial=1))).astype(np.int32)
+        f1, c1, _, _ = pyoracle.astar_ref(off, col, w, h, s, t, quirk=False)
+        f2, c2, _ = pyoracle.astar_indexed(off, col, w, h, s, t)
+        assert f1 == f2 and (not f1 or c1 == c2)
+
+
 def test_row_end_quirk_drops_only_the_last_edge():
     off = np.array([0, 1, 2], np.int32)
     col = np.array([2, 1, 0], np.int32)
