@@ -217,7 +217,7 @@ def run_c4(args):
         single_lat.append((time.perf_counter() - q0) * 1e3)
         if r.cost != costs[i]:
             raise SystemExit(f"single-query GA* disagrees with the batch on query {i}: {r.cost} vs {costs[i]}")
-    single_qps = len(sample) / (time.perf_counter() - ts)
+    single_qps = len(sample) / max(time.perf_counter() - ts, 1e-9)
     log(f"one at a time: {len(sample)} queries, {single_qps:.1f} q/s")
 
     value = len(src) * args.steps / elapsed
@@ -228,11 +228,12 @@ def run_c4(args):
                       "per step")
     res["config"].update({"N": hg.N, "E": hg.E, "queries": len(src), "block": info["block"],
                           "slots": info["slots"], "path_cap": 4096})
+    if len(sample):
+        res["one_at_a_time"] = {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
+                                "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096}
+        res["speedup_vs_one_at_a_time"] = round(value / single_qps, 2)
     res.update({
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
-        "one_at_a_time": {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
-                          "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096},
-        "speedup_vs_one_at_a_time": round(value / single_qps, 2),
         "expansions_per_s": round(stats["expansions"] / (kmean * 1e-3), 1),
         "edges_per_s": round(stats["edges"] / (kmean * 1e-3), 1),
         "roofline": roofline(algorithmic_bytes_geo(stats), kmean, "batch_kernel", copy_bandwidth(ctx)),
@@ -251,14 +252,14 @@ def run_c4(args):
 
 # ----------------------------------------------------------------- C1 / C2
 
-def time_single(ctx, dg, s, t, heur, K, steps, warmup, block=0):
+def time_single(ctx, dg, s, t, heur, K, steps, warmup, block=0, mode=0):
     import ddp_amd
     for _ in range(warmup):
-        ddp_amd.astar(ctx, dg, s, t, heur=heur, K=K, block=block)
+        ddp_amd.astar(ctx, dg, s, t, heur=heur, K=K, block=block, mode=mode)
     walls, kms, r = [], [], None
     for _ in range(steps):
         w0 = time.perf_counter()
-        r = ddp_amd.astar(ctx, dg, s, t, heur=heur, K=K, block=block)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=heur, K=K, block=block, mode=mode)
         walls.append((time.perf_counter() - w0) * 1e3)
         kms.append(r.stats["kernel_ms"])
     return r, float(np.mean(walls)), float(np.mean(kms))
@@ -275,6 +276,10 @@ def run_c1(args):
     minw = int(hg.w.min())
     h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
     r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup)
+    rg, wall_g, kms_g = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup,
+                                    mode=ddp_amd.MODE_WORKGROUP)
+    if rg.cost != r.cost:
+        raise SystemExit(f"C1: workgroup mode {rg.cost} != grid mode {r.cost}")
     # CPU: the oracle's restatement of A_star_cpu.cpp on the same inputs
     reps, el = 0, 0.0
     while el < min(args.cpu_budget, 10.0) or reps == 0:
@@ -288,9 +293,13 @@ def run_c1(args):
     abytes = 16 * st["expansions"] + 16 * st["edges"] + 8 * st["improvements"]
     res = base_record(args, round(1000.0 / wall, 2), "queries/s", round(wall, 3),
                       "C1 single GA* query: random N=100,000, 6 out-edges, w in [1,100], seed 1, h = reverse-BFS "
-                      "hops x min weight, K=1024")
+                      "hops x min weight, K=1024 (grid mode)")
     res["config"].update({"s": s, "t": t, "K": st["K"], "block": st["block"], "cost": r.cost})
+    sg = rg.stats
     res.update({"kernel_ms": round(kms, 3), "iterations": st["iterations"], "expansions": st["expansions"],
+                "workgroup_mode": {"wall_ms": round(wall_g, 3), "kernel_ms": round(kms_g, 3),
+                              "iterations": sg["iterations"], "expansions": sg["expansions"],
+                              "blocks": sg["blocks"], "block": sg["block"]},
                 "expansions_per_s": round(st["expansions"] / (kms * 1e-3), 1),
                 "edges_per_s": round(st["edges"] / (kms * 1e-3), 1),
                 "roofline": roofline(abytes, kms, "ga_grid_kernel"),
@@ -311,17 +320,20 @@ def run_c2(args):
     dg.set_grid(W)
     sweep = {}
     cost = None
-    for K in (512, 2048, 8192):
-        r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_MANHATTAN, K, args.steps, args.warmup)
+    for name, K, mode in (("512-workgroup", 512, ddp_amd.MODE_WORKGROUP), ("1024-workgroup", 1024,
+                          ddp_amd.MODE_WORKGROUP), ("512-grid", 512, ddp_amd.MODE_GRID),
+                          ("2048-grid", 2048, ddp_amd.MODE_GRID), ("8192-grid", 8192, ddp_amd.MODE_GRID)):
+        r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_MANHATTAN, K, args.steps, args.warmup, mode=mode)
         st = r.stats
         if cost is not None and r.cost != cost:
             raise SystemExit(f"C2 cost differs across K: {r.cost} vs {cost}")
         cost = r.cost
         abytes = 8 * st["expansions"] + 12 * st["edges"] + 8 * st["improvements"]  # closed-form h reads nothing
-        sweep[K] = {"wall_ms": round(wall, 3), "kernel_ms": round(kms, 3), "iterations": st["iterations"],
-                    "expansions": st["expansions"], "edges": st["edges"], "block": st["block"],
-                    "roofline": roofline(abytes, kms, "ga_grid_kernel")}
-        log(f"C2 K={K}: {sweep[K]}")
+        kern = "ga_grid_kernel" if mode == ddp_amd.MODE_GRID else "ga_wg_kernel"
+        sweep[name] = {"K": K, "wall_ms": round(wall, 3), "kernel_ms": round(kms, 3), "iterations": st["iterations"],
+                       "expansions": st["expansions"], "edges": st["edges"], "block": st["block"],
+                       "blocks": st["blocks"], "roofline": roofline(abytes, kms, kern)}
+        log(f"C2 {name}: {sweep[name]}")
     best = min(sweep, key=lambda k: sweep[k]["wall_ms"])
     xs = np.arange(W * W, dtype=np.int64)
     h = ((W - 1 - xs // W) + (W - 1 - xs % W)).astype(np.int32)
@@ -332,8 +344,8 @@ def run_c2(args):
         raise SystemExit(f"C2 cost mismatch: GPU {cost} CPU {ccost}")
     res = base_record(args, round(1000.0 / sweep[best]["wall_ms"], 3), "queries/s", sweep[best]["wall_ms"],
                       "C2 single GA* query: 4096x4096 4-connected grid maze, p_block=0.30, seed 2, carved corridor, "
-                      "unit weights, Manhattan heuristic, corner to corner, K swept")
-    res["config"].update({"N": hg.N, "E": hg.E, "cost": cost, "best_K": best})
+                      "unit weights, Manhattan heuristic, corner to corner, K and execution mode swept")
+    res["config"].update({"N": hg.N, "E": hg.E, "cost": cost, "best": best})
     res.update({"sweep": sweep, "roofline": sweep[best]["roofline"],
                 "cpu_baseline": {"value": round(1.0 / cel, 4), "unit": "queries/s", "cores": 1,
                                  "kind": "port", "sample": f"the full query once ({cexp} expansions), indexed-heap "

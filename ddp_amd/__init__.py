@@ -13,12 +13,14 @@ from ctypes import byref, c_int32, c_uint32, c_void_p
 import numpy as np
 
 from . import _capi
-from ._capi import (DDPError, GRAPH_REVERSE, H_ARRAY, H_EUCLID, H_MANHATTAN, H_ZERO, INF_COST, AstarOpts,
+from ._capi import (DDPError, GRAPH_REVERSE, H_ARRAY, H_EUCLID, H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
+                    MODE_GRID, MODE_WORKGROUP, AstarOpts,
                     AstarStats, BatchOpts, BatchStats, DeviceInfo, Heur, Update, as_i32, as_u32, ptr_f64, ptr_i32,
                     ptr_u32)
 
 __all__ = [
-    "DDPError", "INF_COST", "H_ZERO", "H_ARRAY", "H_MANHATTAN", "H_EUCLID", "Context", "HostGraph",
+    "DDPError", "INF_COST", "H_ZERO", "H_ARRAY", "H_MANHATTAN", "H_EUCLID", "MODE_AUTO", "MODE_GRID",
+    "MODE_WORKGROUP", "Context", "HostGraph",
     "DeviceGraph", "AStarResult", "astar", "Batch", "astar_batch", "Updates", "GPU_Graph", "GPU_A_Star",
 ]
 
@@ -356,9 +358,10 @@ def _heur(kind, scale):
 
 
 def astar(ctx, dg, s, t, heur=H_ARRAY, scale=1, K=0, block=0, heap_cap=0, lds_cap=0, max_iters=0,
-          want_parent=False, path_cap=None):
-    """One query through the single-query GA* kernel (ddp_astar)."""
-    o = AstarOpts(K, block, heap_cap, lds_cap, max_iters)
+          want_parent=False, path_cap=None, mode=MODE_AUTO):
+    """One query through the single-query GA* kernel (ddp_astar); mode picks
+    grid or workgroup execution (MODE_AUTO: grid)."""
+    o = AstarOpts(K, block, heap_cap, lds_cap, max_iters, mode)
     st = AstarStats()
     cost = c_int32()
     par = np.zeros(dg.N, np.int32) if want_parent else None

@@ -22,6 +22,7 @@ ERR_NAMES = {
 INF_COST = 2147483647
 
 H_ZERO, H_ARRAY, H_MANHATTAN, H_EUCLID = 0, 1, 2, 3
+MODE_AUTO, MODE_GRID, MODE_WORKGROUP = 0, 1, 2  # DDP_ASTAR_*
 GRAPH_REVERSE = 1
 
 
@@ -37,7 +38,7 @@ class Heur(ctypes.Structure):
 
 class AstarOpts(ctypes.Structure):
     _fields_ = [("K", c_int32), ("block", c_int32), ("heap_cap", c_int32), ("lds_cap", c_int32),
-                ("max_iters", c_int32), ("reserved", c_int32 * 3)]
+                ("max_iters", c_int32), ("mode", c_int32), ("reserved", c_int32 * 2)]
 
 
 class AstarStats(ctypes.Structure):

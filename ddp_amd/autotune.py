@@ -33,4 +33,4 @@ def tune_astar(ctx, dg, src, dst, heur, scale=1, sample=16):
     ms = c_float()
     _check(_capi.lib().ddp_tune_astar(ctx.handle, dg.handle, Heur(heur, scale), src.shape[0], ptr_i32(src),
                                       ptr_i32(dst), sample, byref(o), byref(ms)), ctx)
-    return {"K": o.K, "block": o.block, "mean_ms": ms.value}
+    return {"K": o.K, "block": o.block, "mode": o.mode, "mean_ms": ms.value}

@@ -67,6 +67,7 @@ enum {
     WS_ASTAR_HSIZE,
     WS_ASTAR_PBUF,
     WS_ASTAR_CTRL,
+    WS_ASTAR_TOUCH,
     WS_BFS_FRONT,
     WS_BFS_CTRL,
     WS_PATH,

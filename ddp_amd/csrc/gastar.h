@@ -18,6 +18,7 @@ enum : int {
 
 struct GaGridPlan {
     int32_t K, B, nblocks, lcap, cap, pcap, maxit;
+    bool wg = false;  // workgroup mode: one workgroup, one queue per lane (ddp_astar only)
 };
 
 // Device buffers of one GA* search state.
@@ -25,6 +26,7 @@ struct GaGridState {
     uint64_t *gp = nullptr;      // N packed (f << 32 | parent)
     uint64_t *home = nullptr;    // cap * K heap entries
     int32_t *hsize = nullptr;    // K
+    int32_t *touched = nullptr;  // N, workgroup mode only
     uint64_t *pbuf = nullptr;    // 2 * pcap push entries
     uint32_t *ctrl = nullptr;    // GA_CTRL_WORDS
     unsigned long long *stats = nullptr;  // 8 counters after ctrl
@@ -32,8 +34,9 @@ struct GaGridState {
 };
 
 // min_pcap: lower bound on the push buffer (DA* repairs push before a launch).
+// allow_wg: honour opts->mode and plan the workgroup mode when it applies.
 int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p,
-                int64_t min_pcap = 0);
+                int64_t min_pcap = 0, bool allow_wg = false);
 // State in the context's reusable scratch (valid until the next ddp_astar on ctx).
 int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridState *S);
 // State owned by the caller (DA*), freed by ddp_ga_free.
@@ -48,5 +51,9 @@ int ddp_ga_prepare_resume(ddp_ctx *ctx, const GaGridState &S);
 int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
                const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
                int32_t *rr_out, int32_t *iters_out, float *kernel_ms);
+// Workgroup mode: one search from s to completion (no seed needed).
+int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+                  const GaGridState &S, int32_t N, int32_t s, int32_t t, uint32_t *F_out, int32_t *iters_out,
+                  float *kernel_ms);
 // Reads the four search counters (expanded, edges, improved, stale).
 int ddp_ga_stats(ddp_ctx *ctx, const GaGridState &S, unsigned long long out[4]);

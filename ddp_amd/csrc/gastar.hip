@@ -1,4 +1,5 @@
-// Single-query GA*: K priority queues over one persistent cooperative grid.
+// Single-query GA*: K priority queues over one persistent grid (grid mode),
+// or all of them inside one workgroup (workgroup mode, further down).
 //
 // Reference loop (GA_star/GA_v2.cu:488-559): per iteration the host launches
 // extractMin, A_star_expand, keepHeapPQ, setNV (an O(N) flag scan), insertPQ,
@@ -26,7 +27,7 @@
 #include <cstring>
 
 #include "gastar.h"
-#include "search_expand.h"
+#include "slot_search.h"
 
 namespace {
 
@@ -251,13 +252,94 @@ KernelFn pick(int B, int hk)
     }
 }
 
+// Workgroup mode: all K <= 1024 queues live in one workgroup, one per lane,
+// and an improved node goes to the queue of the lane that relaxed it.  An
+// iteration then synchronises through LDS barriers only, where the grid
+// mode pays a device-wide barrier and its atomics; searches whose frontier
+// holds a few thousand nodes (or whose iteration count is set by a long
+// path) are bound by that per-iteration latency, not by bandwidth.
+struct WgArgs {
+    GraphView g;
+    HeurView h;
+    uint64_t *gp;
+    uint64_t *home;      // heap spill: entry i of queue q at home[i*B + q]
+    int32_t *touched;    // N: nodes whose word left INF
+    uint32_t *ctrl;
+    unsigned long long *stats;
+    int32_t N, s, t, lcap, cap, maxit;
+};
+
+template <int B, int HK>
+__global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
+{
+    extern __shared__ uint64_t s_heap[];  // lcap * B
+    __shared__ SlotLds<B> S;
+    const int lane = threadIdx.x;
+    const LaneHeap H{s_heap + lane, a.home + lane, B, a.lcap, (int64_t)B, a.cap};
+    ExpandCounters C;
+    uint64_t stale = 0;
+    slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
+    if (lane == 0) {
+        const uint32_t e = S.err;
+        a.ctrl[GA_RES_IT] = (uint32_t)S.iters;
+        a.ctrl[GA_RES_F] = S.F;
+        a.ctrl[GA_RES_ERR] = ((e & QERR_HEAP) ? ERR_HEAP : 0u) | ((e & QERR_ITERS) ? ERR_ITERS : 0u);
+    }
+    slot_flush_stats<B>(a.stats, C, stale);
+}
+
+using WgFn = void (*)(WgArgs);
+
+template <int B>
+WgFn pick_wg_b(int hk)
+{
+    switch (hk) {
+    case DDP_H_ZERO: return ga_wg_kernel<B, DDP_H_ZERO>;
+    case DDP_H_ARRAY: return ga_wg_kernel<B, DDP_H_ARRAY>;
+    case DDP_H_MANHATTAN: return ga_wg_kernel<B, DDP_H_MANHATTAN>;
+    default: return ga_wg_kernel<B, DDP_H_EUCLID>;
+    }
+}
+
+WgFn pick_wg(int B, int hk)
+{
+    switch (B) {
+    case 64: return pick_wg_b<64>(hk);
+    case 128: return pick_wg_b<128>(hk);
+    case 256: return pick_wg_b<256>(hk);
+    case 512: return pick_wg_b<512>(hk);
+    default: return pick_wg_b<1024>(hk);
+    }
+}
+
 }  // namespace
 
-int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p, int64_t min_pcap)
+int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaGridPlan *p, int64_t min_pcap,
+                bool allow_wg)
 {
     int32_t B = (o && o->block) ? o->block : 0;
     int32_t K = (o && o->K) ? o->K : 1024;
     DDP_CHECK(K >= 1, ctx, DDP_ERR_ARG, "K must be >= 1 (got %d)", K);
+    const int32_t mode = (o && allow_wg) ? o->mode : DDP_ASTAR_GRID;
+    DDP_CHECK(mode >= DDP_ASTAR_AUTO && mode <= DDP_ASTAR_WORKGROUP, ctx, DDP_ERR_ARG, "unknown mode %d", mode);
+    DDP_CHECK(!(mode == DDP_ASTAR_WORKGROUP && K > 1024), ctx, DDP_ERR_ARG,
+              "workgroup mode holds at most 1024 queues (K=%d)", K);
+    if (mode == DDP_ASTAR_WORKGROUP) {
+        int32_t WB = 64;
+        while (WB < K) WB *= 2;
+        DDP_CHECK(B == 0 || B == WB, ctx, DDP_ERR_ARG,
+                  "workgroup mode runs K=%d queues on a block of %d lanes (got block %d)", K, WB, B);
+        const int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(1, std::min(32, 8192 / WB));
+        DDP_CHECK((int64_t)lcap * WB * 8 <= 96 * 1024, ctx, DDP_ERR_ARG,
+                  "lds_cap %d x block %d exceeds the LDS budget", lcap, WB);
+        int64_t cap = (o && o->heap_cap) ? o->heap_cap : (2 * (int64_t)g->N + WB - 1) / WB + 256;
+        cap = std::max<int64_t>(cap, lcap);
+        DDP_CHECK(cap < (1 << 30), ctx, DDP_ERR_ARG, "heap_cap too large");
+        const int32_t maxit =
+            (o && o->max_iters) ? o->max_iters : (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
+        *p = GaGridPlan{WB, WB, 1, lcap, (int32_t)cap, 1024, maxit, true};
+        return DDP_OK;
+    }
     if (B == 0) B = K >= 4096 ? 256 : 64;
     DDP_CHECK(B == 64 || B == 256 || B == 1024, ctx, DDP_ERR_ARG,
               "block must be 64, 256 or 1024 (got %d)", B);
@@ -302,6 +384,11 @@ int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridSt
     if ((rc = ddp_ws(ctx, WS_ASTAR_CTRL, GA_CTRL_WORDS * 4 + 64, &p))) return rc;
     S->ctrl = (uint32_t *)p;
     S->stats = (unsigned long long *)((char *)p + GA_CTRL_WORDS * 4);
+    S->touched = nullptr;
+    if (P.wg) {
+        if ((rc = ddp_ws(ctx, WS_ASTAR_TOUCH, (size_t)g->N * 4, &p))) return rc;
+        S->touched = (int32_t *)p;
+    }
     S->owned = false;
     return DDP_OK;
 }
@@ -436,6 +523,47 @@ int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGr
     return DDP_OK;
 }
 
+int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+                  const GaGridState &S, int32_t N, int32_t s, int32_t t, uint32_t *F_out, int32_t *iters_out,
+                  float *kernel_ms)
+{
+    DDP_CHECK(P.wg && S.touched && P.B == P.K && P.nblocks == 1, ctx, DDP_ERR_STATE, "GA*: not a workgroup plan");
+    WgArgs a{};
+    a.g = gv;
+    a.h = hv;
+    a.gp = S.gp;
+    a.home = S.home;  // cap * K entries
+    a.touched = S.touched;
+    a.ctrl = S.ctrl;
+    a.stats = S.stats;
+    a.N = N;
+    a.s = s;
+    a.t = t;
+    a.lcap = P.lcap;
+    a.cap = P.cap;
+    a.maxit = P.maxit;
+    const size_t lds = (size_t)P.lcap * P.B * 8;
+    WgFn fn = pick_wg(P.B, hv.kind);
+    DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(fn, dim3(1), dim3(P.B), lds, ctx->stream, a);
+    DDP_HIP(ctx, hipGetLastError());
+    DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    uint32_t res[4];
+    DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + GA_RES_IT, 16, hipMemcpyDeviceToHost, ctx->stream));
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (kernel_ms) *kernel_ms = ms;
+    if (iters_out) *iters_out = (int32_t)res[0];
+    if (F_out) *F_out = res[2];
+    const uint32_t err = res[3];
+    DDP_CHECK(!(err & ERR_ITERS), ctx, DDP_ERR_TIMEOUT, "GA*: iteration limit %d reached", P.maxit);
+    DDP_CHECK(!(err & ERR_HEAP), ctx, DDP_ERR_OVERFLOW,
+              "GA*: a queue exceeded heap_cap=%d entries; raise heap_cap", P.cap);
+    return DDP_OK;
+}
+
 extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t s, int32_t t,
                          const ddp_astar_opts_t *opts, int32_t *cost, int32_t *parent_out,
                          int32_t *path_out, int32_t path_cap, int32_t *path_len,
@@ -451,7 +579,7 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
     int rc = ddp_make_heur(g, h, t, &hv);
     if (rc) return rc;
     GaGridPlan P;
-    if ((rc = ddp_ga_plan(ctx, g, opts, &P))) return rc;
+    if ((rc = ddp_ga_plan(ctx, g, opts, &P, 0, true))) return rc;
     GaGridState S;
     if ((rc = ddp_ga_alloc(ctx, g, P, &S))) return rc;
     int32_t hs = 0;
@@ -476,11 +604,15 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
             *path_len = -1;
         }
     } else {
-        if ((rc = ddp_ga_seed(ctx, P, S, s, (uint32_t)hs))) return rc;
         uint32_t F = 0;
         int32_t iters = 0;
         float kms = 0;
-        rc = ddp_ga_run(ctx, ddp_view(g), hv, P, S, t, DDP_INF_F, 0, &F, nullptr, &iters, &kms);
+        if (P.wg) {
+            rc = ddp_ga_run_wg(ctx, ddp_view(g), hv, P, S, g->N, s, t, &F, &iters, &kms);
+        } else {
+            if ((rc = ddp_ga_seed(ctx, P, S, s, (uint32_t)hs))) return rc;
+            rc = ddp_ga_run(ctx, ddp_view(g), hv, P, S, t, DDP_INF_F, 0, &F, nullptr, &iters, &kms);
+        }
         if (rc) return rc;
         *cost = (int32_t)F;
         if (stats) {

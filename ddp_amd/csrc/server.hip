@@ -50,7 +50,9 @@ struct Res {       // pinned, written by the device
 };
 
 enum : uint32_t { SERR_WAIT = 16, SERR_LOG = 32 };
-enum : int { CT_HEAD = 0, CT_APPLIED = 32, CT_LOG = 64, CT_WORDS = 96 };
+// ctrl words, one 64-byte line apart: claimed count, device copy of the host's
+// published count, poller lease, stop flag copy, applied version
+enum : int { CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_STOP = 80, CT_WORDS = 96 };
 
 struct ServeArgs {
     GraphView g;           // base rows + versioned view fields (ver set per query)
@@ -78,10 +80,6 @@ struct ServeArgs {
     uint64_t idle_ticks, wait_ticks;
 };
 
-__device__ __forceinline__ uint32_t sys_ld(const uint32_t *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-}
 __device__ __forceinline__ int32_t sys_ld_i(const int32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -152,14 +150,20 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
     ExpandCounters C;
     uint64_t stale = 0;
     for (;;) {
-        // ---- claim the next published request (bounded idle wait)
+        // ---- claim the next published request (bounded idle wait).  Idle
+        // slots watch device words only; one of them at a time holds the
+        // poller lease and copies the host's published count (and the stop
+        // flag) into them with plain relaxed loads of host memory, so idle
+        // slots neither flood the host link nor invalidate caches that the
+        // running searches depend on.
         if (lane == 0) {
             int32_t ticket = -1;
             const uint64_t t0 = dev_clock_100mhz();
-            int backoff = 0;  // idle polling of host memory backs off to ~15 us
+            bool poller = false;
+            int backoff = 0;  // non-pollers back off to ~15 us
             for (;;) {
-                const uint32_t tail = sys_ld(a.tail);
                 const uint32_t h = ld_u32(a.ctrl + CT_HEAD);
+                const uint32_t tail = ld_u32(a.ctrl + CT_TAIL);
                 if (h < tail) {
                     if (atomicCAS(a.ctrl + CT_HEAD, h, h + 1) == h) {
                         ticket = (int32_t)h;
@@ -167,10 +171,23 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                     }
                     continue;
                 }
-                if (sys_ld(a.stop) || dev_clock_100mhz() - t0 > a.idle_ticks) break;
-                backoff = backoff < 8 ? backoff + 1 : 8;
-                for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(64);
+                if (ld_u32(a.ctrl + CT_STOP) || dev_clock_100mhz() - t0 > a.idle_ticks) break;
+                if (!poller) poller = atomicCAS(a.ctrl + CT_POLL, 0u, 1u) == 0u;
+                if (poller) {
+                    const uint32_t ht = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (ht > tail) atomicMax(a.ctrl + CT_TAIL, ht);
+                    if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                        atomicExch(a.ctrl + CT_STOP, 1u);
+                    __builtin_amdgcn_s_sleep(16);
+                } else {
+                    backoff = backoff < 8 ? backoff + 1 : 8;
+                    for (int k = 0; k < backoff; ++k) __builtin_amdgcn_s_sleep(64);
+                }
             }
+            if (poller) atomicExch(a.ctrl + CT_POLL, 0u);
+            // the request's fields were written before the published count:
+            // one system-scope acquire per claim orders the reads below
+            if (ticket >= 0) __atomic_thread_fence(__ATOMIC_ACQUIRE);
             s_ticket = ticket;
             s_uerr = 0;
         }
@@ -308,6 +325,7 @@ struct ddp_server {
     std::vector<uint64_t> tags;
     std::vector<double> submit_ms;
     std::vector<int32_t> types;
+    std::vector<uint8_t> delivered;  // per ring entry: handed to the caller, not yet retired
     clk::time_point t_create;
     hipEvent_t done_ev = nullptr;
     bool running = false;
@@ -384,10 +402,9 @@ static int server_ensure(ddp_server *sv)
         if (q != hipSuccess) return ddp_hip_err(sv->ctx, q, "serve kernel", __FILE__, __LINE__);
         sv->running = false;
     }
-    if (sv->collected < sv->submitted) {
-        // anything not yet reported was never claimed or is finished: both fine to relaunch over
-        return server_launch(sv);
-    }
+    // with every workgroup gone, a request without a result was never claimed
+    for (uint32_t k = sv->collected; k < sv->submitted; ++k)
+        if (__atomic_load_n(&sv->res_h[k & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) != k + 1) return server_launch(sv);
     return DDP_OK;
 }
 
@@ -467,6 +484,7 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     hal((void **)&sv->res_h, (void **)&sv->res_d, sizeof(Res) * sv->ring);
     hal((void **)&sv->arena_h, (void **)&sv->arena_d, sv->arena_cap * 4);
     hal((void **)&sv->flags_h, (void **)&sv->flags_d, 64);
+    sv->delivered.assign(sv->ring, 0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->done_ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(ddp_hip_err(ctx, e, "serving allocation", __FILE__, __LINE__));
     hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->dver, (int64_t)g->E, INT32_MAX);
@@ -575,10 +593,13 @@ extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t
 {
     if (!sv || !n || (max > 0 && !out)) return DDP_ERR_ARG;
     *n = 0;
-    while (*n < max && sv->collected < sv->submitted) {
-        const uint32_t k = sv->collected;
-        Res *r = sv->res_h + (k & (sv->ring - 1));
-        if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != k + 1) break;
+    // Results are handed out as they finish, in any order; a ring entry is
+    // retired (reusable) once it and every earlier one have been handed out.
+    const uint32_t mask = sv->ring - 1;
+    for (uint32_t k = sv->collected; k < sv->submitted && *n < max; ++k) {
+        if (sv->delivered[k & mask]) continue;
+        Res *r = sv->res_h + (k & mask);
+        if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != k + 1) continue;
         ddp_server_result_t &o = out[(*n)++];
         o.tag = sv->tags[k];
         o.type = sv->types[k];
@@ -588,6 +609,10 @@ extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t
         o.submit_ms = sv->submit_ms[k];
         o.done_ms = ms_now(sv);
         o.device_ms = (double)(r->t1 - r->t0) * 1e-5;
+        sv->delivered[k & mask] = 1;
+    }
+    while (sv->collected < sv->submitted && sv->delivered[sv->collected & mask]) {
+        sv->delivered[sv->collected & mask] = 0;
         ++sv->collected;
     }
     return server_ensure(sv);

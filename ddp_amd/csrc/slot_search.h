@@ -18,6 +18,7 @@ struct SlotLds {
     uint32_t err;
     int32_t ntouch;
     int32_t plen;
+    int32_t iters;     // iterations of the last search (pop rounds)
 };
 
 template <int B>
@@ -65,6 +66,7 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
         S.F = DDP_INF_F;
         S.ntouch = 0;
         S.err = 0;
+        S.iters = 0;
     }
     __syncthreads();
     bool run = s >= 0 && s < N && t >= 0 && t < N;
@@ -93,9 +95,11 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
             const uint32_t mytop = size > 0 ? (uint32_t)(s_heap[lane] >> 32) : 0xFFFFFFFFu;
             const uint32_t topmin = block_min_u32<B>(mytop, S.red);
             if constexpr (B == 64) __syncthreads();
-            if (S.F <= topmin) break;  // no queued node can improve t
-            if (it >= maxit) {
-                if (lane == 0) atomicOr(&S.err, (uint32_t)QERR_ITERS);
+            if (S.F <= topmin || it >= maxit) {  // no queued node can improve t, or out of iterations
+                if (lane == 0) {
+                    S.iters = it;
+                    if (S.F > topmin) atomicOr(&S.err, (uint32_t)QERR_ITERS);
+                }
                 break;
             }
             int32_t u = -1;

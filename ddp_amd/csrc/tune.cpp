@@ -136,13 +136,14 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
                               const int32_t *dst, int32_t sample, ddp_astar_opts_t *best, float *mean_ms)
 {
     if (!ctx || !g || !best || nq <= 0 || !src || !dst) return DDP_ERR_ARG;
-    const std::string k = key(ctx, g, h, "astar", std::min(nq, sample));
+    const std::string k = key(ctx, g, h, "astar-mode", std::min(nq, sample));
     std::vector<long> cached;
-    if (cache_get(k, cached) && cached.size() >= 3) {
+    if (cache_get(k, cached) && cached.size() >= 4) {
         std::memset(best, 0, sizeof *best);
         best->K = (int32_t)cached[0];
         best->block = (int32_t)cached[1];
-        if (mean_ms) *mean_ms = cached[2] * 1e-3f;
+        best->mode = (int32_t)cached[2];
+        if (mean_ms) *mean_ms = cached[3] * 1e-3f;
         return DDP_OK;
     }
     std::vector<int32_t> s, t;
@@ -150,13 +151,20 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     double best_ms = 1e300;
     ddp_astar_opts_t win;
     std::memset(&win, 0, sizeof win);
-    for (int32_t K : {512, 1024, 2048, 4096, 8192})
-        for (int32_t B : {64, 256}) {
-            if (K < B) continue;
+    // workgroup mode with 256..1024 queues, grid mode with 1024..8192
+    struct Cand {
+        int32_t K, B, mode;
+    };
+    std::vector<Cand> cands;
+    for (int32_t K : {256, 512, 1024}) cands.push_back({K, 0, DDP_ASTAR_WORKGROUP});
+    for (int32_t K : {1024, 2048, 4096, 8192})
+        for (int32_t B : {64, 256}) cands.push_back({K, B, DDP_ASTAR_GRID});
+    for (const Cand &c : cands) {
             ddp_astar_opts_t o;
             std::memset(&o, 0, sizeof o);
-            o.K = K;
-            o.block = B;
+            o.K = c.K;
+            o.block = c.B;
+            o.mode = c.mode;
             double total = 0;
             bool ok = true;
             for (size_t i = 0; i < s.size() && ok; ++i) {
@@ -174,6 +182,6 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     *best = win;
     const double mean = best_ms / s.size();
     if (mean_ms) *mean_ms = (float)mean;
-    cache_put(k, {win.K, win.block, (long)(mean * 1e3)});
+    cache_put(k, {win.K, win.block, win.mode, (long)(mean * 1e3)});
     return DDP_OK;
 }

@@ -132,13 +132,24 @@ int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t reverse,
 /* ------------------------------------------------------ single-query GA* */
 
 typedef struct {
-    int32_t K;          /* number of priority queues (one per lane); 0 = autotune */
-    int32_t block;      /* lanes per workgroup (64,128,256,512,1024); 0 = auto */
+    int32_t K;          /* number of priority queues (one per lane); 0 = 1024 */
+    int32_t block;      /* lanes per workgroup; grid mode: 64, 256 or 1024; 0 = auto */
     int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = auto */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
     int32_t max_iters;  /* iteration limit; 0 = auto (bounded) */
-    int32_t reserved[3];
+    int32_t mode;       /* DDP_ASTAR_AUTO / _GRID / _WORKGROUP */
+    int32_t reserved[2];
 } ddp_astar_opts_t;
+
+/* Execution modes of the single-query search.  GRID spreads the K queues over
+ * K/block workgroups that meet at a device-wide barrier every iteration.
+ * WORKGROUP keeps all K <= 1024 queues in one workgroup (block = K rounded up
+ * to a power of two >= 64): no device-wide barrier, but one CU's worth of
+ * memory parallelism, which measures slower on the random and grid-maze
+ * workloads, so AUTO selects GRID; ddp_tune_astar times both. */
+#define DDP_ASTAR_AUTO      0
+#define DDP_ASTAR_GRID      1
+#define DDP_ASTAR_WORKGROUP 2
 
 typedef struct {
     int64_t expansions;     /* nodes popped and expanded */

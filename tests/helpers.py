@@ -57,3 +57,33 @@ def transpose_np(off, col, w):
     cnt = np.bincount(dst, minlength=N)
     roff[1:] = np.cumsum(cnt)[:-1]
     return roff, src[order].astype(np.int32), ww[order].astype(np.uint32)
+
+
+def euclid_ceil(p, q):
+    """ceil(sqrt(|p - q|^2)) in exact integer arithmetic."""
+    d2 = int(((np.asarray(p, np.int64) - np.asarray(q, np.int64)) ** 2).sum())
+    r = int(np.floor(np.sqrt(float(d2))))
+    while r * r > d2:
+        r -= 1
+    while (r + 1) * (r + 1) <= d2:
+        r += 1
+    return r if r * r == d2 else r + 1
+
+
+def geometric_updates(hg, nb, per, seed, extra=1000):
+    """Update batches for a graph with coordinates that keep the Euclidean
+    heuristic admissible: deletes pick existing edges, each insert u->v
+    weighs ceil(dist(u, v)) plus up to `extra`."""
+    rng = np.random.default_rng(seed)
+    p = hg.xy.reshape(-1, 2)
+    o = full_offsets(hg.off, hg.E)
+    out = []
+    for _ in range(nb):
+        lines = []
+        for u in rng.integers(0, hg.N, per):
+            if o[u + 1] > o[u]:
+                lines.append((0, int(u), int(hg.col[rng.integers(o[u], o[u + 1])]), 0))
+        for u, v in zip(rng.integers(0, hg.N, per), rng.integers(0, hg.N, per)):
+            lines.append((1, int(u), int(v), max(1, euclid_ceil(p[u], p[v])) + int(rng.integers(0, extra))))
+        out.append(lines)
+    return out

@@ -59,9 +59,14 @@ def test_bfs_hops_config1(ctx):
         assert np.array_equal(got, want) and glv == lv
 
 
-@pytest.mark.parametrize("K,block", [(1, 64), (64, 64), (1024, 64), (1024, 256), (4096, 1024)])
-def test_astar_small_graphs(ctx, K, block):
-    rng = np.random.default_rng(1000 + K + block)
+@pytest.mark.parametrize("K,block,mode", [(1, 64, ddp_amd.MODE_AUTO), (64, 64, ddp_amd.MODE_AUTO),
+                                          (200, 0, ddp_amd.MODE_WORKGROUP), (1024, 0, ddp_amd.MODE_WORKGROUP),
+                                          (1024, 64, ddp_amd.MODE_GRID), (1024, 256, ddp_amd.MODE_GRID),
+                                          (4096, 1024, ddp_amd.MODE_AUTO)])
+def test_astar_small_graphs(ctx, K, block, mode):
+    """Grid mode (K queues over K/block workgroups) and workgroup mode (all K
+    queues in one workgroup) against the oracle's CPU A* and Dijkstra."""
+    rng = np.random.default_rng(1000 + K + block + 7 * mode)
     for k in range(60):
         N = int(rng.integers(2, 300))
         off, col, w = random_graph(rng, N, max_deg=int(rng.integers(1, 7)), wmax=int(rng.choice([1, 3, 100])),
@@ -72,14 +77,14 @@ def test_astar_small_graphs(ctx, K, block):
         t = int(rng.choice(reach)) if rng.random() < 0.85 else int(rng.integers(0, N))
         hg = ddp_amd.HostGraph(off, col, w)
         dg = ddp_amd.DeviceGraph(ctx, hg)
-        mode = k % 3
-        if mode == 0:
+        hmode = k % 3
+        if hmode == 0:
             h = np.zeros(N, np.int32)
         else:
-            h, _ = bfs_h_oracle(off, col, w, t, int(w[col >= 0].min(initial=1)) if mode == 1 else 1)
+            h, _ = bfs_h_oracle(off, col, w, t, int(w[col >= 0].min(initial=1)) if hmode == 1 else 1)
         dg.set_h(h)
-        say("astar case", K, block, k, "N", N, "s", s, "t", t, "mode", mode)
-        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=K, block=block, want_parent=True)
+        say("astar case", K, block, mode, k, "N", N, "s", s, "t", t, "h", hmode)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=K, block=block, want_parent=True, mode=mode)
         found, cost, parent, _ = pyoracle.astar_ref(off, col, w, h, s, t, quirk=False)
         if not found or (h[s] < 0):
             assert r.cost == ddp_amd.INF_COST and len(r.path) == 0
@@ -102,6 +107,8 @@ def test_astar_config1(ctx):
         h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
         say("C1 s", s, "t", t)
         r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=1024)
+        rg = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=1024, mode=ddp_amd.MODE_WORKGROUP)
+        assert rg.cost == r.cost
         found_q, cost_q, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=True)
         found, cost, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=False)
         dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s, t)
@@ -119,9 +126,10 @@ def test_astar_grid_manhattan(ctx):
     s, t = 0, 128 * 96 - 1
     dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s)
     assert dist[t] > 0
-    for K in (512, 2048, 8192):
-        say("grid K", K)
-        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_MANHATTAN, K=K)
+    for K, mode in ((512, ddp_amd.MODE_AUTO), (512, ddp_amd.MODE_WORKGROUP), (2048, ddp_amd.MODE_AUTO),
+                    (8192, ddp_amd.MODE_AUTO)):
+        say("grid K", K, "mode", mode)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_MANHATTAN, K=K, mode=mode)
         assert r.cost == dist[t]
         check_path(hg.off, hg.col, hg.w, r.path, s, t, int(dist[t]))
 
@@ -187,8 +195,10 @@ def test_heap_spill_to_hbm(ctx):
         want.append(int(dist[t]))
     for i in range(0, 64, 16):
         say("spill single", i)
-        r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_EUCLID, K=256, block=256, lds_cap=1)
-        assert r.cost == want[i]
+        for mode in (ddp_amd.MODE_WORKGROUP, ddp_amd.MODE_GRID):
+            r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_EUCLID, K=256, block=256, lds_cap=1,
+                              mode=mode)
+            assert r.cost == want[i]
     for block, lcap in ((64, 4), (256, 4), (512, 4), (128, 1)):
         say("spill batch", block, lcap)
         cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=block, lds_cap=lcap,

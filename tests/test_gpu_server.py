@@ -10,6 +10,8 @@ import pytest
 import ddp_amd
 from oracle import pyoracle
 
+from helpers import geometric_updates
+
 pytestmark = pytest.mark.gpu
 
 
@@ -80,7 +82,7 @@ def test_server_matches_batch_on_each_version(ctx):
     and in the mix (server with updates interleaved)."""
     hg = ddp_amd.HostGraph.geometric(20000, 10, 100000, 43)
     dg = ddp_amd.DeviceGraph(ctx, hg)
-    ub = hg.gen_updates(3, 200, 200, 1, 100, 44).batches
+    ub = geometric_updates(hg, 3, 200, 44)  # inserts keep the Euclidean heuristic admissible
     src, dst = hg.geo_pairs(60, 500, 100000, 45)
     sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_EUCLID, slots=16)
     tags = 0
