@@ -179,7 +179,7 @@ def run_c4(args):
     src, dst = hg.geo_pairs(args.queries, 100, 100000, 4)
     log(f"workload ready in {time.time() - t0:.1f}s: N={hg.N} E={hg.E}")
     ctx = ddp_amd.Context(0)
-    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=not args.no_reorder)
     block = args.block
     if block == 0:
         from ddp_amd import autotune
@@ -227,7 +227,7 @@ def run_c4(args):
                       "4,096 queries with log-uniform separation, Euclidean heuristic, one persistent launch "
                       "per step")
     res["config"].update({"N": hg.N, "E": hg.E, "queries": len(src), "block": info["block"],
-                          "slots": info["slots"], "path_cap": 4096})
+                          "slots": info["slots"], "path_cap": 4096, "relabel": not args.no_reorder})
     if len(sample):
         res["one_at_a_time"] = {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
                                 "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096}
@@ -272,7 +272,7 @@ def run_c1(args):
     src, dst = hg.reachable_pairs(1, 1)
     s, t = int(src[0]), int(dst[0])
     ctx = ddp_amd.Context(0)
-    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True, reorder=not args.no_reorder)
     minw = int(hg.w.min())
     h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
     r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup)
@@ -443,7 +443,7 @@ def run_c5(args):
             lines.append((1, int(u), int(v), int(w)))
         ups.append(lines)
     ctx = ddp_amd.Context(0)
-    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=not args.no_reorder)
     # one-at-a-time GA* on a few of the queries, and the batched kernel on all of them (unmodified graph)
     cost_b, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO)
     single, lat1 = [], []
@@ -484,6 +484,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-reorder", action="store_true",
+                    help="keep the input node numbering on the device (default: Hilbert/BFS relabelling)")
     ap.add_argument("--serve-qps", type=float, default=5000.0, help="offered load of the served leg")
     ap.add_argument("--serve-requests", type=int, default=8192)
     ap.add_argument("--c3-nodes", type=int, default=1000000)

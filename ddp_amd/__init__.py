@@ -13,7 +13,7 @@ from ctypes import byref, c_int32, c_uint32, c_void_p
 import numpy as np
 
 from . import _capi
-from ._capi import (DDPError, GRAPH_REVERSE, H_ARRAY, H_EUCLID, H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
+from ._capi import (DDPError, GRAPH_REORDER, GRAPH_REVERSE, H_ARRAY, H_EUCLID, H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
                     MODE_GRID, MODE_WORKGROUP, AstarOpts,
                     AstarStats, BatchOpts, BatchStats, DeviceInfo, Heur, Update, as_i32, as_u32, ptr_f64, ptr_i32,
                     ptr_u32)
@@ -289,16 +289,20 @@ def write_hx(path, h):
 class DeviceGraph:
     """CSR resident in HBM (ddp_graph)."""
 
-    def __init__(self, ctx, hg, reverse=False):
+    def __init__(self, ctx, hg, reverse=False, reorder=False):
+        """reorder: relabel nodes on the device for cache locality (Hilbert
+        order of the coordinates, else breadth-first); every method still
+        takes and returns the host graph's node ids."""
         self.ctx = ctx
         self.N, self.E = hg.N, hg.E
+        self.reordered = bool(reorder)
         h = c_void_p()
-        _check(_capi.lib().ddp_graph_upload(ctx.handle, hg.N, hg.E, ptr_i32(hg.off), ptr_i32(hg.col),
-                                            ptr_u32(hg.w), GRAPH_REVERSE if reverse else 0, byref(h)), ctx.handle)
+        flags = (GRAPH_REVERSE if reverse else 0) | (GRAPH_REORDER if reorder else 0)
+        xy = None if hg.xy is None else as_i32(hg.xy)
+        _check(_capi.lib().ddp_graph_upload_ex(ctx.handle, hg.N, hg.E, ptr_i32(hg.off), ptr_i32(hg.col),
+                                               ptr_u32(hg.w), ptr_i32(xy), flags, byref(h)), ctx.handle)
         self._h = h
         ctx._adopt(self)
-        if hg.xy is not None:
-            self.set_coords(hg.xy)
 
     @property
     def handle(self):

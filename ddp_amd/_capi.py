@@ -24,6 +24,7 @@ INF_COST = 2147483647
 H_ZERO, H_ARRAY, H_MANHATTAN, H_EUCLID = 0, 1, 2, 3
 MODE_AUTO, MODE_GRID, MODE_WORKGROUP = 0, 1, 2  # DDP_ASTAR_*
 GRAPH_REVERSE = 1
+GRAPH_REORDER = 2
 
 
 class DDPError(RuntimeError):
@@ -101,6 +102,7 @@ SIGNATURES = {
     "ddp_ctx_synchronize": (c_int, [c_void_p]),
     "ddp_copy_bandwidth": (c_int, [c_void_p, c_int64, c_int32, POINTER(c_float), POINTER(c_float)]),
     "ddp_graph_upload": (c_int, [c_void_p, c_int32, c_int32, P_I32, P_I32, P_U32, c_uint32, PP]),
+    "ddp_graph_upload_ex": (c_int, [c_void_p, c_int32, c_int32, P_I32, P_I32, P_U32, P_I32, c_uint32, PP]),
     "ddp_graph_destroy": (None, [c_void_p]),
     "ddp_graph_info": (c_int, [c_void_p, P_I32, P_I32, P_U32, P_I32]),
     "ddp_graph_set_h": (c_int, [c_void_p, P_I32]),

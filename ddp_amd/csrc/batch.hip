@@ -257,6 +257,18 @@ extern "C" int ddp_batch_upload(ddp_batch *b, int32_t nq, const int32_t *src, co
     DDP_CHECK(nq >= 0 && nq <= b->maxq, ctx, DDP_ERR_ARG, "nq=%d outside [0,%d]", nq, b->maxq);
     DDP_CHECK(nq == 0 || (src && dst), ctx, DDP_ERR_ARG, "ddp_batch_upload: null query arrays");
     DDP_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> ms, md;
+    if (!b->g->perm.empty()) {  // caller ids -> device ids (out-of-range ids stay, the kernel flags them)
+        const int32_t N = b->g->N;
+        ms.assign(src, src + nq);
+        md.assign(dst, dst + nq);
+        for (int32_t i = 0; i < nq; ++i) {
+            if (ms[i] >= 0 && ms[i] < N) ms[i] = b->g->perm[ms[i]];
+            if (md[i] >= 0 && md[i] < N) md[i] = b->g->perm[md[i]];
+        }
+        src = ms.data();
+        dst = md.data();
+    }
     std::vector<int32_t> ord(nq);
     std::iota(ord.begin(), ord.end(), 0);
     if (b->schedule == 0 && b->hv.kind == DDP_H_EUCLID && !b->g->host_xy.empty()) {
@@ -389,6 +401,14 @@ extern "C" int ddp_batch_download(ddp_batch *b, int32_t *costs, int32_t *path_le
         DDP_HIP(ctx, hipMemcpyAsync(tt.data(), b->qtime, n * 16, hipMemcpyDeviceToHost, ctx->stream));
     }
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (paths && n && b->path_cap > 0 && !b->g->iperm.empty()) {  // device ids -> caller ids
+        std::vector<int32_t> pl(n);
+        DDP_HIP(ctx, hipMemcpyAsync(pl.data(), b->plen, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < n; ++i)
+            for (int32_t k = 0; k < std::min(pl[i], b->path_cap); ++k)
+                paths[i * b->path_cap + k] = ddp_to_user(b->g, paths[i * b->path_cap + k]);
+    }
     if (latency_ms && n) {
         uint64_t t0 = ~0ull;
         for (size_t i = 0; i < n; ++i) t0 = std::min(t0, tt[2 * i]);

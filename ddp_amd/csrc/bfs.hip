@@ -128,6 +128,7 @@ extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t rever
     DDP_CHECK(scale >= 1, ctx, DDP_ERR_ARG, "ddp_bfs_hops: scale must be >= 1");
     DDP_HIP(ctx, hipSetDevice(ctx->device));
     const int32_t N = g->N;
+    t = ddp_to_dev(g, t);
     void *p;
     int rc;
     if ((rc = ddp_ws(ctx, WS_BFS_FRONT, (size_t)N * 12 + 256, &p))) return rc;
@@ -169,7 +170,14 @@ extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t rever
         }
         hipLaunchKernelGGL(bfs_scale, dim3(1024), dim3(256), 0, ctx->stream, dist, scaled, N, scale);
     }
-    if (h_out) DDP_HIP(ctx, hipMemcpyAsync(h_out, scaled, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (h_out && g->iperm.empty()) {
+        DDP_HIP(ctx, hipMemcpyAsync(h_out, scaled, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+    } else if (h_out) {  // device order -> caller order
+        std::vector<int32_t> tmp(N);
+        DDP_HIP(ctx, hipMemcpyAsync(tmp.data(), scaled, (size_t)N * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        for (int32_t i = 0; i < N; ++i) h_out[g->iperm[i]] = tmp[i];
+    }
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return DDP_OK;
 }

@@ -179,15 +179,91 @@ static int upload(ddp_ctx *ctx, T **dst, const T *src, size_t n)
     return DDP_OK;
 }
 
+// Position of (x, y) along the Hilbert curve through a 2^bits x 2^bits grid
+// (the classic xy-to-distance walk: one quadrant digit per level, rotating
+// the coordinates so the curve stays continuous).
+static uint64_t hilbert_key(uint64_t x, uint64_t y, int bits)
+{
+    const uint64_t n = 1ull << bits;
+    uint64_t d = 0;
+    for (uint64_t sq = n >> 1; sq > 0; sq >>= 1) {
+        const uint64_t rx = (x & sq) ? 1 : 0, ry = (y & sq) ? 1 : 0;
+        d += sq * sq * ((3 * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+// New order of the nodes (iperm[new] = old): along a Hilbert curve through
+// the coordinates when there are any, else breadth-first over out-edges from
+// node 0, then from the lowest unvisited id, and so on.  Either way nodes that
+// are adjacent in the graph land on nearby ids, so a search's random reads of
+// per-node words (g, parent, coordinates, rows) share cache lines.
+static void relabel_order(int32_t N, const std::vector<int32_t> &o, const int32_t *col, const int32_t *xy,
+                          std::vector<int32_t> &iperm)
+{
+    iperm.resize(N);
+    if (xy) {
+        int64_t mnx = INT64_MAX, mny = INT64_MAX, mxx = INT64_MIN, mxy = INT64_MIN;
+        for (int32_t i = 0; i < N; ++i) {
+            mnx = std::min<int64_t>(mnx, xy[2 * i]);
+            mxx = std::max<int64_t>(mxx, xy[2 * i]);
+            mny = std::min<int64_t>(mny, xy[2 * i + 1]);
+            mxy = std::max<int64_t>(mxy, xy[2 * i + 1]);
+        }
+        const uint64_t span = (uint64_t)std::max(mxx - mnx, mxy - mny);
+        int bits = 1;
+        while (bits < 32 && (span >> bits) != 0) ++bits;
+        std::vector<std::pair<uint64_t, int32_t>> k(N);
+        for (int32_t i = 0; i < N; ++i)
+            k[i] = {hilbert_key((uint64_t)(xy[2 * i] - mnx), (uint64_t)(xy[2 * i + 1] - mny), bits), i};
+        std::sort(k.begin(), k.end());
+        for (int32_t i = 0; i < N; ++i) iperm[i] = k[i].second;
+        return;
+    }
+    std::vector<char> seen(N, 0);
+    int32_t head = 0, tail = 0;
+    for (int32_t r = 0; r < N; ++r) {
+        if (seen[r]) continue;
+        seen[r] = 1;
+        iperm[tail++] = r;
+        while (head < tail) {
+            const int32_t u = iperm[head++];
+            for (int32_t j = o[u]; j < o[u + 1]; ++j) {
+                const int32_t c = col[j];
+                if (c >= 0 && !seen[c]) {
+                    seen[c] = 1;
+                    iperm[tail++] = c;
+                }
+            }
+        }
+    }
+}
+
 extern "C" int ddp_graph_upload(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t *off,
                                 const int32_t *col, const uint32_t *w, uint32_t flags,
                                 ddp_graph **out)
+{
+    return ddp_graph_upload_ex(ctx, N, E, off, col, w, nullptr, flags, out);
+}
+
+extern "C" int ddp_graph_upload_ex(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t *off,
+                                   const int32_t *col, const uint32_t *w, const int32_t *xy,
+                                   uint32_t flags, ddp_graph **out)
 {
     if (!ctx || !out) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_graph_upload: null argument");
     *out = nullptr;
     DDP_CHECK(N > 0 && E >= 0, ctx, DDP_ERR_ARG, "ddp_graph_upload: bad sizes N=%d E=%d", N, E);
     DDP_CHECK(off && (E == 0 || (col && w)), ctx, DDP_ERR_ARG, "ddp_graph_upload: null array");
     DDP_CHECK(off[0] == 0, ctx, DDP_ERR_FORMAT, "ddp_graph_upload: off[0] = %d, expected 0", off[0]);
+    DDP_CHECK(!(flags & ~(DDP_GRAPH_REVERSE | DDP_GRAPH_REORDER)), ctx, DDP_ERR_ARG,
+              "ddp_graph_upload: unknown flags 0x%x", flags);
     // validate offsets and ids; compute min weight and max degree
     std::vector<int32_t> o(N + 1);
     int32_t maxdeg = 0;
@@ -206,6 +282,31 @@ extern "C" int ddp_graph_upload(ddp_ctx *ctx, int32_t N, int32_t E, const int32_
         if (col[j] >= 0) minw = std::min(minw, w[j]);
     }
     if (minw == 0xFFFFFFFFu) minw = 0;
+    // relabel: rows move to their new ids with their edges in the same order
+    // (so "first matching instance" rules are unaffected), columns are mapped
+    std::vector<int32_t> perm, iperm, o2, col2, xy2;
+    std::vector<uint32_t> w2;
+    if (flags & DDP_GRAPH_REORDER) {
+        relabel_order(N, o, col, xy, iperm);
+        perm.resize(N);
+        for (int32_t i = 0; i < N; ++i) perm[iperm[i]] = i;
+        o2.resize((size_t)N + 1);
+        col2.resize(E);
+        w2.resize(E);
+        int32_t pos = 0;
+        for (int32_t i = 0; i < N; ++i) {
+            o2[i] = pos;
+            const int32_t u = iperm[i];
+            for (int32_t j = o[u]; j < o[u + 1]; ++j, ++pos) {
+                col2[pos] = col[j] >= 0 ? perm[col[j]] : col[j];
+                w2[pos] = w[j];
+            }
+        }
+        o2[N] = E;
+        o.swap(o2);
+        col = col2.data();
+        w = w2.data();
+    }
     DDP_HIP(ctx, hipSetDevice(ctx->device));
     ddp_graph *g = new ddp_graph();
     g->ctx = ctx;
@@ -214,6 +315,8 @@ extern "C" int ddp_graph_upload(ddp_ctx *ctx, int32_t N, int32_t E, const int32_
     g->E = E;
     g->minw = minw;
     g->maxdeg = maxdeg;
+    g->perm.swap(perm);
+    g->iperm.swap(iperm);
     int rc;
     if ((rc = upload(ctx, &g->off, o.data(), (size_t)N + 1)) || (rc = upload(ctx, &g->col, col, (size_t)E)) ||
         (rc = upload(ctx, &g->w, w, (size_t)E))) {
@@ -254,6 +357,10 @@ extern "C" int ddp_graph_upload(ddp_ctx *ctx, int32_t N, int32_t E, const int32_
         ddp_graph_destroy(g);
         return ddp_hip_err(ctx, e, "graph upload", __FILE__, __LINE__);
     }
+    if (xy && (rc = ddp_graph_set_coords(g, xy))) {
+        ddp_graph_destroy(g);
+        return rc;
+    }
     *out = g;
     return DDP_OK;
 }
@@ -274,6 +381,12 @@ extern "C" int ddp_graph_set_h(ddp_graph *g, const int32_t *h)
     if (!g || !h) return ddp_set_err(g ? g->ctx : nullptr, DDP_ERR_ARG, "ddp_graph_set_h: null argument");
     ddp_ctx *ctx = g->ctx;
     DDP_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> hp;
+    if (!g->iperm.empty()) {  // caller order -> device order
+        hp.resize(g->N);
+        for (int32_t i = 0; i < g->N; ++i) hp[i] = h[g->iperm[i]];
+        h = hp.data();
+    }
     if (!g->h) DDP_HIP(ctx, hipMalloc((void **)&g->h, (size_t)g->N * 4));
     DDP_HIP(ctx, hipMemcpyAsync(g->h, h, (size_t)g->N * 4, hipMemcpyHostToDevice, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -285,6 +398,15 @@ extern "C" int ddp_graph_set_coords(ddp_graph *g, const int32_t *xy)
     if (!g || !xy) return ddp_set_err(g ? g->ctx : nullptr, DDP_ERR_ARG, "ddp_graph_set_coords: null argument");
     ddp_ctx *ctx = g->ctx;
     DDP_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<int32_t> xp;
+    if (!g->iperm.empty()) {  // caller order -> device order
+        xp.resize(2 * (size_t)g->N);
+        for (int32_t i = 0; i < g->N; ++i) {
+            xp[2 * (size_t)i] = xy[2 * (size_t)g->iperm[i]];
+            xp[2 * (size_t)i + 1] = xy[2 * (size_t)g->iperm[i] + 1];
+        }
+        xy = xp.data();
+    }
     if (!g->xy) DDP_HIP(ctx, hipMalloc((void **)&g->xy, (size_t)g->N * 8));
     DDP_HIP(ctx, hipMemcpyAsync(g->xy, xy, (size_t)g->N * 8, hipMemcpyHostToDevice, ctx->stream));
     g->host_xy.assign(xy, xy + 2 * (size_t)g->N);
@@ -297,6 +419,9 @@ extern "C" int ddp_graph_set_grid(ddp_graph *g, int32_t width)
     if (!g) return DDP_ERR_ARG;
     DDP_CHECK(width > 0 && g->N % width == 0, g->ctx, DDP_ERR_ARG,
               "ddp_graph_set_grid: width %d does not divide N = %d", width, g->N);
+    DDP_CHECK(g->perm.empty(), g->ctx, DDP_ERR_STATE,
+              "ddp_graph_set_grid: the grid heuristic reads row and column from the node id, which "
+              "DDP_GRAPH_REORDER has changed");
     g->grid_w = width;
     return DDP_OK;
 }

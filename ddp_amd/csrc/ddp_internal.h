@@ -39,7 +39,19 @@ struct ddp_graph {
     int32_t grid_w = 0;
     // host copy of the coordinates for work estimates (batched scheduling)
     std::vector<int32_t> host_xy;
+    // node relabelling of DDP_GRAPH_REORDER: perm[caller id] = device id,
+    // iperm[device id] = caller id; both empty when ids are kept
+    std::vector<int32_t> perm, iperm;
 };
+
+// Caller node id -> device node id and back (identity without relabelling).
+// Callers range-check ids before mapping them in; out-of-range ids (such as
+// the -1 of "no parent") map out unchanged.
+inline int32_t ddp_to_dev(const ddp_graph *g, int32_t v) { return g->perm.empty() ? v : g->perm[v]; }
+inline int32_t ddp_to_user(const ddp_graph *g, int32_t v)
+{
+    return (g->iperm.empty() || v < 0 || v >= g->N) ? v : g->iperm[v];
+}
 
 // Error helpers: record a message on the context and return the code.
 int ddp_set_err(ddp_ctx *ctx, int code, const char *fmt, ...);

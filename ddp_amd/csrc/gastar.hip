@@ -575,6 +575,8 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
               "ddp_astar: s=%d t=%d outside [0,%d)", s, t, g->N);
     DDP_CHECK(!path_out || path_len, ctx, DDP_ERR_ARG, "ddp_astar: path_out needs path_len");
     DDP_HIP(ctx, hipSetDevice(ctx->device));
+    s = ddp_to_dev(g, s);
+    t = ddp_to_dev(g, t);
     HeurView hv;
     int rc = ddp_make_heur(g, h, t, &hv);
     if (rc) return rc;
@@ -644,7 +646,13 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
         hipLaunchKernelGGL(unpack_parent_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.gp, dpar, g->N);
         DDP_HIP(ctx, hipMemcpyAsync(parent_out, dpar, (size_t)g->N * 4, hipMemcpyDeviceToHost, ctx->stream));
         DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (!g->iperm.empty()) {  // device ids and order -> caller ids and order
+            std::vector<int32_t> tmp(parent_out, parent_out + g->N);
+            for (int32_t i = 0; i < g->N; ++i) parent_out[g->iperm[i]] = ddp_to_user(g, tmp[i]);
+        }
     }
+    if (path_out && path_len && !g->iperm.empty())  // written only when the length is positive
+        for (int32_t i = 0; i < *path_len; ++i) path_out[i] = ddp_to_user(g, path_out[i]);
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (stats)
         stats->total_ms =

@@ -529,8 +529,8 @@ extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uin
     if (sv->submitted - sv->collected >= sv->ring) return ddp_set_err(ctx, DDP_ERR_NOMEM, "request ring full");
     Req r{};
     r.type = DDP_REQ_QUERY;
-    r.s = s;
-    r.t = t;
+    r.s = ddp_to_dev(sv->g, s);
+    r.t = ddp_to_dev(sv->g, t);
     r.version = sv->version;
     publish(sv, r, tag, DDP_REQ_QUERY);
     return server_ensure(sv);
@@ -547,11 +547,12 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
         const ddp_update_t &l = lines[i];
         DDP_CHECK((l.flag == 0 || l.flag == 1) && l.u >= 0 && l.u < sv->N && l.v >= 0 && l.v < sv->N, ctx, DDP_ERR_ARG,
                   "update line %d invalid", i);
+        const int32_t u = ddp_to_dev(sv->g, l.u), v = ddp_to_dev(sv->g, l.v);
         if (l.flag == 0) {
-            dels.push_back(l.u);
-            dels.push_back(l.v);
+            dels.push_back(u);
+            dels.push_back(v);
         } else {
-            ins.push_back({l.u, l.v, (int32_t)l.w});
+            ins.push_back({u, v, (int32_t)l.w});
         }
     }
     DDP_CHECK(sv->log_used + (int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_NOMEM,

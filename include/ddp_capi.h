@@ -78,6 +78,7 @@ int ddp_copy_bandwidth(ddp_ctx *ctx, int64_t bytes, int32_t reps, float *ms, flo
 /* -------------------------------------------------------------------- graph */
 
 #define DDP_GRAPH_REVERSE 1u  /* also keep the transposed CSR on the device */
+#define DDP_GRAPH_REORDER 2u  /* relabel nodes for locality (see ddp_graph_upload_ex) */
 
 /* Replaces GPU_Graph's constructor (LIB/include/graph.cuh:50-72) and the
  * cudaMalloc/cudaMemcpy block of every driver (GA_star/GA_v2.cu:414-447).
@@ -87,6 +88,17 @@ int ddp_copy_bandwidth(ddp_ctx *ctx, int64_t bytes, int32_t reps, float *ms, flo
 int ddp_graph_upload(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t *off,
                      const int32_t *col, const uint32_t *w, uint32_t flags,
                      ddp_graph **out);
+/* ddp_graph_upload plus, optionally, the coordinates of ddp_graph_set_coords
+ * (xy may be NULL).  With DDP_GRAPH_REORDER the device copy numbers the nodes
+ * along a Hilbert curve through xy (breadth-first order without xy) so that
+ * neighbours share cache lines; every entry point still takes and returns the
+ * caller's node ids (queries, paths, parent arrays, h arrays, updates), the
+ * translation happens at the boundary.  Not combinable with
+ * ddp_graph_set_grid, whose heuristic reads coordinates from the id.  No
+ * reference counterpart (its kernels index the file's ids directly). */
+int ddp_graph_upload_ex(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t *off,
+                        const int32_t *col, const uint32_t *w, const int32_t *xy,
+                        uint32_t flags, ddp_graph **out);
 void ddp_graph_destroy(ddp_graph *g);
 int ddp_graph_info(const ddp_graph *g, int32_t *N, int32_t *E,
                    uint32_t *min_weight, int32_t *max_out_degree);
