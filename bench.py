@@ -55,6 +55,22 @@ def roofline(abytes, kernel_ms, kernel, copy_gbps=None):
     return r
 
 
+def measured_traffic(roof, **key):
+    """HBM bytes per launch from the committed counter summary of this exact
+    configuration (profiles/<kernel>_<workload>_traffic.json, written from two
+    rocprofv3 --pmc runs of bench.py); left null for any other configuration."""
+    path = os.path.join(ROOT, "profiles", f"{roof['kernel']}_{key['workload']}_traffic.json")
+    if not os.path.exists(path):
+        return roof
+    with open(path) as f:
+        t = json.load(f)
+    if all(t.get(k) == v for k, v in key.items()):
+        roof["traffic"] = t["hbm_bytes_per_launch"]
+        roof["traffic_over_algorithmic"] = round(t["hbm_bytes_per_launch"] / roof["algorithmic_bytes_per_launch"], 3)
+        roof["traffic_source"] = os.path.relpath(path, ROOT)
+    return roof
+
+
 def copy_bandwidth(ctx):
     """Measured device copy bandwidth (copy_kernel, 2 GiB; also the known-traffic
     kernel the HBM counters in profiles/ are scaled by)."""
@@ -236,7 +252,9 @@ def run_c4(args):
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
         "expansions_per_s": round(stats["expansions"] / (kmean * 1e-3), 1),
         "edges_per_s": round(stats["edges"] / (kmean * 1e-3), 1),
-        "roofline": roofline(algorithmic_bytes_geo(stats), kmean, "batch_kernel", copy_bandwidth(ctx)),
+        "roofline": measured_traffic(roofline(algorithmic_bytes_geo(stats), kmean, "batch_kernel", copy_bandwidth(ctx)),
+                                     workload="c4", block=info["block"], relabel=not args.no_reorder,
+                                     queries=len(src)),
     })
     if not args.no_serve:
         ups = geometric_updates(hg, args.serve_requests // 10, 100, 8)

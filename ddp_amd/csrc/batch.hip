@@ -109,7 +109,8 @@ BatchFn pick_batch(int B, int hk)
     case 64: return pick_b<64>(hk);
     case 128: return pick_b<128>(hk);
     case 256: return pick_b<256>(hk);
-    default: return pick_b<512>(hk);
+    case 512: return pick_b<512>(hk);
+    default: return pick_b<1024>(hk);
     }
 }
 
@@ -164,8 +165,8 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     int rc = ddp_make_heur(g, h, -1, &hv);
     if (rc) return rc;
     int32_t B = (o && o->block) ? o->block : 256;
-    DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512, ctx, DDP_ERR_ARG,
-              "block must be 64, 128, 256 or 512 (got %d)", B);
+    DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512 || B == 1024, ctx, DDP_ERR_ARG,
+              "block must be 64, 128, 256, 512 or 1024 (got %d)", B);
     // deeper LDS heaps keep sift-downs out of HBM (about 10% on the C4 batch at
     // block 512, where occupancy is set by waves rather than LDS)
     int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(4, std::min(32, 8192 / B));

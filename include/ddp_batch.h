@@ -19,7 +19,7 @@ extern "C" {
 typedef struct ddp_batch ddp_batch;
 
 typedef struct {
-    int32_t block;      /* lanes (= queues) per query workgroup: 64,128,256,512; 0 = 256 */
+    int32_t block;      /* lanes (= queues) per query workgroup: 64..1024; 0 = 256 */
     int32_t slots;      /* concurrent queries; 0 = what fits (occupancy and free HBM) */
     int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = 2048 */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
