@@ -13,14 +13,15 @@ from ctypes import byref, c_int32, c_uint32, c_void_p
 import numpy as np
 
 from . import _capi
-from ._capi import (DDPError, GRAPH_REORDER, GRAPH_REVERSE, H_ARRAY, H_EUCLID, H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
+from ._capi import (DDPError, ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES, GRAPH_REORDER, GRAPH_REVERSE, H_ARRAY, H_EUCLID,
+                    H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
                     MODE_GRID, MODE_WORKGROUP, AstarOpts,
                     AstarStats, BatchOpts, BatchStats, DeviceInfo, Heur, Update, as_i32, as_u32, ptr_f64, ptr_i32,
                     ptr_u32)
 
 __all__ = [
     "DDPError", "INF_COST", "H_ZERO", "H_ARRAY", "H_MANHATTAN", "H_EUCLID", "MODE_AUTO", "MODE_GRID",
-    "MODE_WORKGROUP", "Context", "HostGraph",
+    "MODE_WORKGROUP", "ENGINE_AUTO", "ENGINE_HEAPS", "ENGINE_PILES", "Context", "HostGraph",
     "DeviceGraph", "AStarResult", "astar", "Batch", "astar_batch", "Updates", "GPU_Graph", "GPU_A_Star",
 ]
 
@@ -384,10 +385,10 @@ class Batch:
     """Batched queries with slots kept between launches (ddp_batch)."""
 
     def __init__(self, ctx, dg, max_queries, heur=H_EUCLID, scale=1, block=0, slots=0, heap_cap=0, lds_cap=0,
-                 max_iters=0, path_cap=0, schedule=0):
+                 max_iters=0, path_cap=0, schedule=0, engine=ENGINE_AUTO):
         self.ctx, self.dg = ctx, dg
         self.path_cap = path_cap
-        o = BatchOpts(block, slots, heap_cap, lds_cap, max_iters, path_cap, schedule, 0)
+        o = BatchOpts(block, slots, heap_cap, lds_cap, max_iters, path_cap, schedule, engine)
         h = c_void_p()
         _check(_capi.lib().ddp_batch_create(ctx.handle, dg.handle, _heur(heur, scale), max_queries, byref(o),
                                             byref(h)), ctx.handle)
@@ -441,10 +442,10 @@ class Batch:
 
 
 def astar_batch(ctx, dg, src, dst, heur=H_EUCLID, scale=1, block=0, slots=0, heap_cap=0, lds_cap=0,
-                path_cap=0, schedule=0):
+                path_cap=0, schedule=0, engine=ENGINE_AUTO):
     src, dst = as_i32(src), as_i32(dst)
     n = src.shape[0]
-    o = BatchOpts(block, slots, heap_cap, lds_cap, 0, path_cap, schedule, 0)
+    o = BatchOpts(block, slots, heap_cap, lds_cap, 0, path_cap, schedule, engine)
     cost = np.zeros(n, np.int32)
     plen = np.zeros(n, np.int32)
     pb = np.zeros(max(1, n * path_cap), np.int32) if path_cap else None

@@ -23,6 +23,7 @@ INF_COST = 2147483647
 
 H_ZERO, H_ARRAY, H_MANHATTAN, H_EUCLID = 0, 1, 2, 3
 MODE_AUTO, MODE_GRID, MODE_WORKGROUP = 0, 1, 2  # DDP_ASTAR_*
+ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES = 0, 1, 2  # DDP_BATCH_*
 GRAPH_REVERSE = 1
 GRAPH_REORDER = 2
 
@@ -56,7 +57,7 @@ class DeviceInfo(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("block", c_int32), ("slots", c_int32), ("heap_cap", c_int32), ("lds_cap", c_int32),
-                ("max_iters", c_int32), ("path_cap", c_int32), ("schedule", c_int32), ("reserved", c_int32)]
+                ("max_iters", c_int32), ("path_cap", c_int32), ("schedule", c_int32), ("engine", c_int32)]
 
 
 class BatchStats(ctypes.Structure):

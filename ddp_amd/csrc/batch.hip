@@ -36,12 +36,15 @@ struct BatchArgs {
     uint64_t *qtime;     // 2 per query: start/end, 100 MHz device clock
     unsigned long long *stats;
     int32_t N, lcap, cap, maxit;
+    uint32_t ncap, fcap, delta0;  // pile engine: near ring, far buffers, first window
 };
 
-template <int B, int HK>
+enum { ENG_HEAPS = 0, ENG_PILES = 1 };
+
+template <int B, int HK, int ENG>
 __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
 {
-    extern __shared__ uint64_t s_heap[];  // lcap * B
+    extern __shared__ uint64_t s_heap[];  // lcap * B (heap engine)
     __shared__ SlotLds<B> S;
     __shared__ int32_t s_q;
 
@@ -50,7 +53,9 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
     const int32_t N = a.N;
     uint64_t *gp = a.gp + slot * N;
     int32_t *touched = a.touched + slot * N;
-    const LaneHeap H{s_heap + lane, a.home + slot * a.cap * B + lane, B, a.lcap, (int64_t)B, a.cap};
+    uint64_t *home = a.home + slot * (int64_t)a.cap * B;  // this slot's queue memory
+    const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
+    const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap};
     ExpandCounters C;
     uint64_t stale = 0;
     for (;;) {
@@ -61,7 +66,10 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
         const int32_t q = a.order ? a.order[qi] : qi;
         const int32_t s = a.qs[q], t = a.qt[q];
         if (lane == 0) a.qtime[2 * (int64_t)q] = dev_clock_100mhz();
-        slot_search<B, HK>(a.g, a.h, gp, touched, H, s_heap, N, s, t, a.maxit, S, C, stale);
+        if constexpr (ENG == ENG_PILES)
+            slot_search_piles<B, HK>(a.g, a.h, gp, touched, P, N, s, t, a.maxit, a.delta0, S, C, stale);
+        else
+            slot_search<B, HK>(a.g, a.h, gp, touched, H, s_heap, N, s, t, a.maxit, S, C, stale);
         if (a.path_cap > 0) {
             slot_path<B>(gp, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
             if (lane == 0) a.plen[q] = S.plen;
@@ -92,26 +100,32 @@ __global__ void batch_reset(uint32_t *next, unsigned long long *stats)
 
 using BatchFn = void (*)(BatchArgs);
 
-template <int B>
+template <int B, int ENG>
 BatchFn pick_b(int hk)
 {
     switch (hk) {
-    case DDP_H_ZERO: return batch_kernel<B, DDP_H_ZERO>;
-    case DDP_H_ARRAY: return batch_kernel<B, DDP_H_ARRAY>;
-    case DDP_H_MANHATTAN: return batch_kernel<B, DDP_H_MANHATTAN>;
-    default: return batch_kernel<B, DDP_H_EUCLID>;
+    case DDP_H_ZERO: return batch_kernel<B, DDP_H_ZERO, ENG>;
+    case DDP_H_ARRAY: return batch_kernel<B, DDP_H_ARRAY, ENG>;
+    case DDP_H_MANHATTAN: return batch_kernel<B, DDP_H_MANHATTAN, ENG>;
+    default: return batch_kernel<B, DDP_H_EUCLID, ENG>;
     }
 }
 
-BatchFn pick_batch(int B, int hk)
+template <int ENG>
+BatchFn pick_be(int B, int hk)
 {
     switch (B) {
-    case 64: return pick_b<64>(hk);
-    case 128: return pick_b<128>(hk);
-    case 256: return pick_b<256>(hk);
-    case 512: return pick_b<512>(hk);
-    default: return pick_b<1024>(hk);
+    case 64: return pick_b<64, ENG>(hk);
+    case 128: return pick_b<128, ENG>(hk);
+    case 256: return pick_b<256, ENG>(hk);
+    case 512: return pick_b<512, ENG>(hk);
+    default: return pick_b<1024, ENG>(hk);
     }
+}
+
+BatchFn pick_batch(int B, int hk, int eng)
+{
+    return eng == ENG_PILES ? pick_be<ENG_PILES>(B, hk) : pick_be<ENG_HEAPS>(B, hk);
 }
 
 }  // namespace
@@ -123,6 +137,8 @@ struct ddp_batch {
     ddp_heur_t heur{};
     HeurView hv{};
     int32_t maxq = 0, B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0, path_cap = 0, schedule = 0;
+    int32_t eng = ENG_HEAPS;
+    uint32_t ncap = 0, fcap = 0, delta0 = 1;  // pile engine sizes
     int32_t nq = 0;
     // device
     int32_t *qs = nullptr, *qt = nullptr, *order = nullptr, *cost = nullptr, *plen = nullptr, *path = nullptr,
@@ -167,17 +183,23 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     int32_t B = (o && o->block) ? o->block : 256;
     DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512 || B == 1024, ctx, DDP_ERR_ARG,
               "block must be 64, 128, 256, 512 or 1024 (got %d)", B);
+    const int32_t engine = o ? o->engine : DDP_BATCH_AUTO;
+    DDP_CHECK(engine >= DDP_BATCH_AUTO && engine <= DDP_BATCH_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
+    const int eng = engine == DDP_BATCH_PILES ? ENG_PILES : ENG_HEAPS;
     // deeper LDS heaps keep sift-downs out of HBM (about 10% on the C4 batch at
-    // block 512, where occupancy is set by waves rather than LDS)
+    // block 512, where occupancy is set by waves rather than LDS); the pile
+    // engine keeps its queues in HBM and needs no dynamic LDS
     int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(4, std::min(32, 8192 / B));
+    if (eng == ENG_PILES) lcap = 0;
     DDP_CHECK((int64_t)lcap * B * 8 <= 64 * 1024, ctx, DDP_ERR_ARG, "lds_cap %d x block %d exceeds 64 KiB", lcap, B);
     int32_t cap = (o && o->heap_cap) ? o->heap_cap : std::max(lcap, 2048);
-    cap = std::max(cap, lcap);
+    cap = std::max(cap, std::max(lcap, 16));
+    const size_t lds = (size_t)lcap * B * 8;
     int per_cu = 0;
-    DDP_HIP(ctx, hipFuncSetAttribute((const void *)pick_batch(B, hv.kind), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)((size_t)lcap * B * 8)));
-    DDP_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pick_batch(B, hv.kind), B,
-                                                             (size_t)lcap * B * 8));
+    DDP_HIP(ctx, hipFuncSetAttribute((const void *)pick_batch(B, hv.kind, eng),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DDP_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)pick_batch(B, hv.kind, eng), B,
+                                                             lds));
     DDP_CHECK(per_cu > 0, ctx, DDP_ERR_ARG, "batch kernel does not fit on a CU with block %d", B);
     int64_t slots = (o && o->slots) ? o->slots : (int64_t)per_cu * ctx->prop.multiProcessorCount;
     slots = std::min<int64_t>(slots, max_queries);
@@ -204,6 +226,17 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     b->maxit = (o && o->max_iters) ? o->max_iters : (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
     b->path_cap = o ? std::max(0, o->path_cap) : 0;
     b->schedule = o ? o->schedule : 0;
+    b->eng = eng;
+    {
+        // pile engine: the slot's cap * B queue entries hold a near ring (a
+        // power of two, about a quarter) and two far buffers
+        const uint64_t total = (uint64_t)cap * B;
+        uint64_t nc = 1;
+        while (nc * 2 <= total / 4) nc *= 2;
+        b->ncap = (uint32_t)nc;
+        b->fcap = (uint32_t)std::min<uint64_t>((total - nc) / 2, 0xFFFFFFFFull);
+        b->delta0 = std::max<uint32_t>(1, 2 * g->meanw);
+    }
     const size_t M = (size_t)max_queries;
     hipError_t e = hipSuccess;
     auto al = [&](void **p, size_t bytes) {
@@ -329,10 +362,13 @@ static int batch_capture(ddp_batch *b)
     a.lcap = b->lcap;
     a.cap = b->cap;
     a.maxit = b->maxit;
+    a.ncap = b->ncap;
+    a.fcap = b->fcap;
+    a.delta0 = b->delta0;
     const int32_t grid = std::max(1, std::min(b->slots, std::max(1, b->nq)));
     DDP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, ctx->stream, b->next, b->stats);
-    hipLaunchKernelGGL(pick_batch(b->B, b->hv.kind), dim3(grid), dim3(b->B), (size_t)b->lcap * b->B * 8,
+    hipLaunchKernelGGL(pick_batch(b->B, b->hv.kind, b->eng), dim3(grid), dim3(b->B), (size_t)b->lcap * b->B * 8,
                        ctx->stream, a);
     hipError_t e = hipStreamEndCapture(ctx->stream, &b->graph);
     if (e != hipSuccess) return ddp_hip_err(ctx, e, "hipStreamEndCapture", __FILE__, __LINE__);

@@ -276,10 +276,15 @@ extern "C" int ddp_graph_upload_ex(ddp_ctx *ctx, int32_t N, int32_t E, const int
     }
     o[N] = E;
     uint32_t minw = 0xFFFFFFFFu;
+    uint64_t wsum = 0, nlive = 0;
     for (int32_t j = 0; j < E; ++j) {
         DDP_CHECK(col[j] < N, ctx, DDP_ERR_FORMAT, "ddp_graph_upload: col[%d] = %d >= N = %d", j,
                   col[j], N);
-        if (col[j] >= 0) minw = std::min(minw, w[j]);
+        if (col[j] >= 0) {
+            minw = std::min(minw, w[j]);
+            wsum += w[j];
+            ++nlive;
+        }
     }
     if (minw == 0xFFFFFFFFu) minw = 0;
     // relabel: rows move to their new ids with their edges in the same order
@@ -314,6 +319,7 @@ extern "C" int ddp_graph_upload_ex(ddp_ctx *ctx, int32_t N, int32_t E, const int
     g->N = N;
     g->E = E;
     g->minw = minw;
+    g->meanw = nlive ? (uint32_t)std::min<uint64_t>(wsum / nlive, 0x7FFFFFFFull) : 0;
     g->maxdeg = maxdeg;
     g->perm.swap(perm);
     g->iperm.swap(iperm);

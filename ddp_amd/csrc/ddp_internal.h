@@ -24,7 +24,7 @@ struct ddp_graph {
     ddp_ctx *ctx = nullptr;
     int device = 0;  // destroy uses this, never ctx (the context may be gone)
     int32_t N = 0, E = 0;
-    uint32_t minw = 0;
+    uint32_t minw = 0, meanw = 0;  // smallest and mean weight of live edges
     int32_t maxdeg = 0;
     // forward CSR, N+1 offsets (sentinel appended at upload)
     int32_t *off = nullptr, *col = nullptr;

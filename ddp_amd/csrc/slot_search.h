@@ -19,6 +19,11 @@ struct SlotLds {
     int32_t ntouch;
     int32_t plen;
     int32_t iters;     // iterations of the last search (pop rounds)
+    // pile engine (slot_search_piles)
+    uint32_t near_head, near_tail;  // ring positions (monotonic)
+    uint32_t far_n, far_n2;         // entries in the current / next far pile
+    uint32_t thr, delta;            // near window [.., thr) and its width
+    uint32_t phase_n;               // near entries taken since the last refill
 };
 
 template <int B>
@@ -123,6 +128,192 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
             }
         }
     }
+    block_sync_global();
+}
+
+// ---------------------------------------------------------------- piles
+//
+// A second queue discipline for one-slot searches: no heaps.  Open entries
+// (f << 32 | node) sit in two piles in HBM: a "near" ring holding the entries
+// with f below a threshold and a "far" pile with the rest.  Each round the
+// workgroup takes up to B near entries at once (one per lane, a coalesced
+// read), drops stale ones (f no longer the node's word) and those with
+// f >= F(t), and expands the rest cooperatively; an improvement goes to the
+// near ring when its f is below the threshold, else to the far pile.  When the
+// near ring runs dry, one pass over the far pile finds the smallest live f,
+// the threshold moves to it plus a window width, and a second pass moves the
+// entries under it to the near ring and compacts the rest into the other far
+// buffer.  The search ends when both piles are empty or every live far entry
+// has f >= F(t).  This is a label-correcting search (a node can be expanded
+// again after a later improvement, as GA*'s K queues also allow); it returns
+// the optimal cost for an admissible heuristic because an entry is only
+// dropped when it is stale or cannot lead to anything below F(t).  The window
+// width adapts: doubled when a phase fed the lanes fewer than 2B entries,
+// halved when it ran past 32B.
+
+struct PilePtrs {
+    uint64_t *near;  // ncap entries (power of two), ring
+    uint64_t *far0;  // fcap entries
+    uint64_t *far1;  // fcap entries
+    uint32_t ncap, fcap;
+};
+
+template <int B>
+struct PilePush {
+    uint64_t *near;
+    uint64_t *far;
+    uint32_t nmask, ncap, fcap;
+    SlotLds<B> *S;
+    int32_t *touched;
+    uint32_t err = 0;
+
+    __device__ void target(uint32_t f) { atomicMin(&S->F, f); }
+    __device__ void touch(int32_t c, uint64_t prev)
+    {
+        if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(&S->ntouch, 1)] = c;
+    }
+    __device__ void operator()(bool has, uint64_t key)
+    {
+        if (!has) return;
+        const uint32_t f = (uint32_t)(key >> 32);
+        if (f >= __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return;
+        if (f < S->thr) {
+            const uint32_t idx = atomicAdd(&S->near_tail, 1u);
+            if (idx - S->near_head < ncap) near[idx & nmask] = key;
+            else err |= QERR_HEAP;
+        } else {
+            const uint32_t idx = atomicAdd(&S->far_n, 1u);
+            if (idx < fcap) far[idx] = key;
+            else err |= QERR_HEAP;
+        }
+    }
+};
+
+__device__ __forceinline__ uint32_t sat_add_u32(uint32_t a, uint32_t b)
+{
+    return a > 0xFFFFFFFFu - b ? 0xFFFFFFFFu : a + b;
+}
+
+// Same contract as slot_search (S.F, S.err, gp and touched describe the search
+// on return).  delta0 is the first window width.
+template <int B, int HK>
+__device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched,
+                                  const PilePtrs &P, int32_t N, int32_t s, int32_t t, int32_t maxit,
+                                  uint32_t delta0, SlotLds<B> &S, ExpandCounters &C, uint64_t &stale)
+{
+    const int lane = threadIdx.x;
+    const uint32_t nmask = P.ncap - 1;
+    if (lane == 0) {
+        S.F = DDP_INF_F;
+        S.ntouch = 0;
+        S.err = 0;
+        S.iters = 0;
+        S.near_head = S.near_tail = 0;
+        S.far_n = S.far_n2 = 0;
+        S.delta = delta0 > 0 ? delta0 : 1;
+        S.thr = 0;
+        S.phase_n = 0;
+    }
+    __syncthreads();
+    bool run = s >= 0 && s < N && t >= 0 && t < N;
+    if (run) {
+        set_target<HK>(hv, t);
+        const int32_t hs = heur_eval<HK>(hv, s);
+        if (HK == DDP_H_ARRAY && hs < 0) {
+            run = false;  // h(s) = -1: s cannot reach t
+        } else if (s == t) {
+            run = false;
+            if (lane == 0) S.F = (uint32_t)hs;
+        } else if (lane == 0) {
+            const uint64_t key = ((uint64_t)(uint32_t)hs << 32) | (uint32_t)s;
+            gp[s] = ((uint64_t)(uint32_t)hs << 32) | 0xFFFFFFFFull;
+            touched[0] = s;
+            S.ntouch = 1;
+            P.near[0] = key;
+            S.near_tail = 1;
+            S.thr = sat_add_u32((uint32_t)hs, S.delta);
+        }
+    } else if (lane == 0) {
+        S.err = QERR_BADQ;
+    }
+    block_sync_global();
+    uint64_t *far = P.far0, *far2 = P.far1;
+    int32_t it = 0;
+    while (run) {
+        const uint32_t head = S.near_head, tail = S.near_tail, F = S.F;
+        if (head == tail) {
+            // ---- refill the near ring from the far pile
+            const uint32_t nf = S.far_n < P.fcap ? S.far_n : P.fcap;
+            uint32_t mymin = 0xFFFFFFFFu;
+            for (uint32_t i = lane; i < nf; i += B) {
+                const uint64_t key = far[i];
+                const uint32_t f = (uint32_t)(key >> 32);
+                if (f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f) mymin = f < mymin ? f : mymin;
+            }
+            const uint32_t fmin = block_min_u32<B>(mymin, S.red);
+            if (fmin >= F) break;  // nothing open can improve t (also: far pile empty)
+            if (lane == 0) {
+                if (S.phase_n < 2 * B) S.delta = sat_add_u32(S.delta, S.delta);
+                else if (S.phase_n > 32 * B && S.delta > 1) S.delta >>= 1;
+                S.thr = sat_add_u32(fmin, S.delta);
+                S.phase_n = 0;
+                S.far_n2 = 0;
+            }
+            __syncthreads();
+            const uint32_t thr = S.thr;
+            for (uint32_t i = lane; i < nf; i += B) {
+                const uint64_t key = far[i];
+                const uint32_t f = (uint32_t)(key >> 32);
+                if (f >= F || (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) != f) continue;  // hopeless or stale
+                if (f < thr) {
+                    const uint32_t idx = atomicAdd(&S.near_tail, 1u);
+                    if (idx - head < P.ncap) P.near[idx & nmask] = key;
+                    else atomicOr(&S.err, (uint32_t)QERR_HEAP);
+                } else {
+                    far2[atomicAdd(&S.far_n2, 1u)] = key;  // fewer than nf <= fcap entries
+                }
+            }
+            block_sync_global();
+            if (lane == 0) S.far_n = S.far_n2;
+            uint64_t *tmp = far;
+            far = far2;
+            far2 = tmp;
+            __syncthreads();
+            if (S.err) break;
+            continue;
+        }
+        if (it >= maxit) {
+            if (lane == 0) atomicOr(&S.err, (uint32_t)QERR_ITERS);
+            break;
+        }
+        ++it;
+        // ---- take up to B near entries, one per lane
+        const uint32_t n = tail - head < (uint32_t)B ? tail - head : (uint32_t)B;
+        int32_t u = -1;
+        uint32_t fu = 0;
+        if ((uint32_t)lane < n) {
+            const uint64_t key = P.near[(head + lane) & nmask];
+            const uint32_t f = (uint32_t)(key >> 32);
+            const int32_t node = (int32_t)(uint32_t)key;
+            if (f < F && (uint32_t)(ld_gp(gp + node) >> 32) == f) {
+                u = node;
+                fu = f;
+            } else {
+                ++stale;
+            }
+        }
+        __syncthreads();  // every lane has read head/tail and its entry
+        if (lane == 0) {
+            S.near_head = head + n;
+            S.phase_n += n;
+        }
+        PilePush<B> push{P.near, far, nmask, P.ncap, P.fcap, &S, touched};
+        expand_block<B, HK>(G, hv, gp, t, u, fu, S.L, C, push);
+        if (push.err) atomicOr(&S.err, push.err);
+        block_sync_global();
+        if (S.err) break;
+    }
+    if (lane == 0) S.iters = it;
     block_sync_global();
 }
 

@@ -21,13 +21,23 @@ typedef struct ddp_batch ddp_batch;
 typedef struct {
     int32_t block;      /* lanes (= queues) per query workgroup: 64..1024; 0 = 256 */
     int32_t slots;      /* concurrent queries; 0 = what fits (occupancy and free HBM) */
-    int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = 2048 */
+    int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = 2048.  A slot's
+                           queue memory is heap_cap * block entries in either engine */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
     int32_t max_iters;  /* per-query iteration limit; 0 = auto */
     int32_t path_cap;   /* path entries kept per query (0 = costs only) */
     int32_t schedule;   /* 0 = longest straight-line estimate first, 1 = given order */
-    int32_t reserved;
+    int32_t engine;     /* DDP_BATCH_AUTO / _HEAPS / _PILES */
 } ddp_batch_opts_t;
+
+/* Queue discipline inside a query slot.  HEAPS: one binary heap per lane (the
+ * GA* structure; an iteration pops one node per lane).  PILES: a near ring of
+ * entries under a moving f threshold and a far pile for the rest, both in HBM;
+ * a round expands up to one near entry per lane and no heap is ever sifted.
+ * Both return exact costs. */
+#define DDP_BATCH_AUTO  0
+#define DDP_BATCH_HEAPS 1
+#define DDP_BATCH_PILES 2
 
 typedef struct {
     int64_t expansions, edges, improvements, stale_pops;

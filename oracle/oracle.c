@@ -26,6 +26,9 @@
  *                   the reference's genOpCSR.cpp compiled into oracle/_ref.
  *   orc_csr_from_edges  GA_star/GenerateCSR.cpp:36-132. Pinned byte-for-byte
  *                   against the reference's GenerateCSR.cpp in oracle/_ref.
+ *   orc_astar_batch_euclid  orc_astar_ref's search over many queries on a
+ *                   thread pool, h = floor(Euclid) from coordinates. Pinned by
+ *                   cost equality with orc_astar_ref on per-target h arrays.
  *   orc_apply_batch DA_star/GA_merge.cu:1019-1089 (removeDelEdges + mergeDiff):
  *                   G_{b+1} = G_b minus the first live instance of each deleted
  *                   (u,v) + inserts appended per source in arrival order.
@@ -37,6 +40,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <limits.h>
+#include <pthread.h>
 
 #define ORC_API __attribute__((visibility("default")))
 
@@ -119,29 +123,32 @@ static void ref_update(int *pq, const int *size, int node, const int *cx)
     }
 }
 
-/*
- * A_star_cpu.cpp:100-204.  Returns 1 if the target was popped (found), else 0.
- * cost_out gets Cx[t] (f = g + h of the target); parent_out (N ints) gets the
- * parent array, so the reference's printed path t, parent[t], ... can be
- * rebuilt.  Relaxation arithmetic is unsigned 32-bit exactly as in
- * A_star_cpu.cpp:176 (int - int + unsigned + int); heap comparisons are signed
- * int as in A_star_cpu.cpp:21,45,55,61.
- */
-ORC_API int orc_astar_ref(int N, int E, const int *off, const int *col,
-                          const uint32_t *w, const int *h, int s, int t,
-                          int quirk, int *cost_out, int *parent_out,
-                          long long *expansions_out)
+/* h(v): the given array, or floor(sqrt(dx^2 + dy^2)) to (tx, ty) computed
+ * on demand from integer coordinates (exact: the float root is corrected). */
+static inline int hval(const int *h, const int *xy, int tx, int ty, int v)
 {
-    int *cx = (int *)malloc(sizeof(int) * (size_t)N);
-    int *open = (int *)malloc(sizeof(int) * (size_t)N);
-    int *pq = (int *)malloc(sizeof(int) * ((size_t)N + 1));
-    int *parent = parent_out;
+    if (h) return h[v];
+    const int64_t dx = (int64_t)xy[2 * (size_t)v] - tx, dy = (int64_t)xy[2 * (size_t)v + 1] - ty;
+    const int64_t d2 = dx * dx + dy * dy;
+    int64_t r = (int64_t)__builtin_sqrt((double)d2);
+    while (r * r > d2) r--;
+    while ((r + 1) * (r + 1) <= d2) r++;
+    return (int)r;
+}
+
+/* The search of A_star_cpu.cpp:100-204 on caller-provided buffers (cx, open,
+ * pq of N+1 ints, parent), with the heuristic from an array or coordinates. */
+static int astar_ref_core(int N, int E, const int *off, const int *col, const uint32_t *w,
+                          const int *h, const int *xy, int s, int t, int quirk, int *cx, int *open,
+                          int *pq, int *parent, int *cost_out, long long *expansions_out)
+{
+    int tx = 0, ty = 0;
+    if (!h) { tx = xy[2 * (size_t)t]; ty = xy[2 * (size_t)t + 1]; }
     int size = 0;
     long long expanded = 0;
-    if (!cx || !open || !pq) { free(cx); free(open); free(pq); return -1; }
     for (int i = 0; i < N; i++) { cx[i] = INT_MAX; parent[i] = -1; open[i] = -1; }
 
-    cx[s] = h[s];
+    cx[s] = hval(h, xy, tx, ty, s);
     ref_insert(pq, &size, s, cx);
     open[s] = 1;
     int found = 0;
@@ -150,10 +157,11 @@ ORC_API int orc_astar_ref(int N, int E, const int *off, const int *col,
         open[u] = -1;
         if (u == t) { found = 1; break; }
         expanded++;
+        const int hu = hval(h, xy, tx, ty, u);
         int b = off[u], e = row_end(off, N, E, u, quirk);
         for (int j = b; j < e; j++) {
             int c = col[j];
-            uint32_t cand = (uint32_t)(cx[u] - h[u]) + w[j] + (uint32_t)h[c];
+            uint32_t cand = (uint32_t)(cx[u] - hu) + w[j] + (uint32_t)hval(h, xy, tx, ty, c);
             if ((uint32_t)cx[c] > cand) {
                 cx[c] = (int)cand;
                 if (open[c] == 1) {
@@ -170,8 +178,98 @@ ORC_API int orc_astar_ref(int N, int E, const int *off, const int *col,
     }
     *cost_out = cx[t];
     if (expansions_out) *expansions_out = expanded;
+    return found;
+}
+
+/*
+ * A_star_cpu.cpp:100-204.  Returns 1 if the target was popped (found), else 0.
+ * cost_out gets Cx[t] (f = g + h of the target); parent_out (N ints) gets the
+ * parent array, so the reference's printed path t, parent[t], ... can be
+ * rebuilt.  Relaxation arithmetic is unsigned 32-bit exactly as in
+ * A_star_cpu.cpp:176 (int - int + unsigned + int); heap comparisons are signed
+ * int as in A_star_cpu.cpp:21,45,55,61.
+ */
+ORC_API int orc_astar_ref(int N, int E, const int *off, const int *col,
+                          const uint32_t *w, const int *h, int s, int t,
+                          int quirk, int *cost_out, int *parent_out,
+                          long long *expansions_out)
+{
+    int *cx = (int *)malloc(sizeof(int) * (size_t)N);
+    int *open = (int *)malloc(sizeof(int) * (size_t)N);
+    int *pq = (int *)malloc(sizeof(int) * ((size_t)N + 1));
+    if (!cx || !open || !pq) { free(cx); free(open); free(pq); return -1; }
+    int found = astar_ref_core(N, E, off, col, w, h, NULL, s, t, quirk, cx, open, pq, parent_out, cost_out,
+                               expansions_out);
     free(cx); free(open); free(pq);
     return found;
+}
+
+/* ---- many queries of A_star_cpu.cpp's search on a thread pool ----
+ * The CPU baseline of the batched configurations: nthreads workers take
+ * queries from a shared counter, one query per worker at a time, each with
+ * its own N-sized buffers (re-initialised per query, as the reference's one
+ * process per query does).  h is floor(Euclid) to each query's target,
+ * computed on demand from the coordinates.  Pinned by equality of every cost
+ * with orc_astar_ref on a per-target h array (tests/test_oracle.py). */
+typedef struct {
+    int N, E;
+    const int *off, *col, *xy;
+    const uint32_t *w;
+    int nq;
+    const int *src, *dst;
+    int *costs;
+    long long expanded;
+    int next;
+    int failed;
+    pthread_mutex_t mu;
+} batch_job;
+
+static void *batch_worker(void *arg)
+{
+    batch_job *J = (batch_job *)arg;
+    int *cx = (int *)malloc(sizeof(int) * (size_t)J->N);
+    int *open = (int *)malloc(sizeof(int) * (size_t)J->N);
+    int *pq = (int *)malloc(sizeof(int) * ((size_t)J->N + 1));
+    int *parent = (int *)malloc(sizeof(int) * (size_t)J->N);
+    long long mine = 0;
+    if (!cx || !open || !pq || !parent) {
+        pthread_mutex_lock(&J->mu);
+        J->failed = 1;
+        pthread_mutex_unlock(&J->mu);
+    } else {
+        for (;;) {
+            int q = __atomic_fetch_add(&J->next, 1, __ATOMIC_RELAXED);
+            if (q >= J->nq) break;
+            long long ex = 0;
+            int cost = 0;
+            int found = astar_ref_core(J->N, J->E, J->off, J->col, J->w, NULL, J->xy, J->src[q], J->dst[q], 0, cx,
+                                       open, pq, parent, &cost, &ex);
+            J->costs[q] = found ? cost : INT_MAX;
+            mine += ex;
+        }
+    }
+    free(cx); free(open); free(pq); free(parent);
+    pthread_mutex_lock(&J->mu);
+    J->expanded += mine;
+    pthread_mutex_unlock(&J->mu);
+    return NULL;
+}
+
+ORC_API int orc_astar_batch_euclid(int N, int E, const int *off, const int *col, const uint32_t *w,
+                                   const int *xy, int nq, const int *src, const int *dst, int nthreads,
+                                   int *costs_out, long long *expansions_out)
+{
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    batch_job J = {N, E, off, col, xy, w, nq, src, dst, costs_out, 0, 0, 0, PTHREAD_MUTEX_INITIALIZER};
+    pthread_t th[256];
+    int started = 0;
+    for (int i = 0; i < nthreads; i++)
+        if (pthread_create(&th[started], NULL, batch_worker, &J) == 0) started++;
+    if (started == 0) batch_worker(&J);
+    for (int i = 0; i < started; i++) pthread_join(th[i], NULL);
+    if (expansions_out) *expansions_out = J.expanded;
+    return J.failed ? -1 : started ? started : 1;
 }
 
 /* ---- textbook Dijkstra with an indexed binary heap (64-bit distances) ---- */

@@ -219,3 +219,29 @@ def astar_indexed(off, col, w, h, s, t):
     if found < 0:
         raise MemoryError("oracle allocation failed")
     return bool(found), cost.value, ex.value
+
+
+def astar_batch_euclid(off, col, w, xy, src, dst, nthreads):
+    """orc_astar_batch_euclid: A_star_cpu.cpp's search for every (src[i],
+    dst[i]) on a pool of nthreads workers, h = floor(Euclid) to the target.
+    Returns (costs, total expansions, threads started); unreachable = INT_MAX."""
+    L = lib()
+    if not hasattr(L, "_bx_bound"):
+        L.orc_astar_batch_euclid.restype = c_int
+        L.orc_astar_batch_euclid.argtypes = [c_int, c_int, _P_I, _P_I, _P_U, _P_I, c_int, _P_I, _P_I, c_int, _P_I,
+                                             _P_LL]
+        L._bx_bound = True
+    N, E = len(off), len(col)
+    off, po = _i(off)
+    col, pc = _i(col)
+    w, pw = _u(w)
+    xy, pxy = _i(xy)
+    src, ps = _i(src)
+    dst, pd = _i(dst)
+    costs = np.zeros(len(src), np.int32)
+    ex = c_longlong()
+    started = L.orc_astar_batch_euclid(N, E, po, pc, pw, pxy, len(src), ps, pd, int(nthreads),
+                                       costs.ctypes.data_as(_P_I), ctypes.byref(ex))
+    if started < 0:
+        raise MemoryError("oracle allocation failed")
+    return costs, ex.value, started

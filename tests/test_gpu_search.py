@@ -240,3 +240,86 @@ def test_batch_plan_replay(ctx):
                 assert first[i] == dist[dst[i]]
         assert np.array_equal(out["cost"], first)
     b.close()
+
+
+@pytest.mark.parametrize("block", [64, 256, 512])
+def test_batch_piles_parity(ctx, block):
+    """The pile engine (near ring + far pile, no heaps) returns the oracle's
+    costs and real paths on geometric (Euclid), random (zero and BFS
+    heuristics) and grid-maze (Manhattan) inputs."""
+    hg = ddp_amd.HostGraph.geometric(30000, 10, 100000, 31)
+    src, dst = hg.geo_pairs(96, 100, 100000, 32)
+    for reorder in (False, True):
+        dg = ddp_amd.DeviceGraph(ctx, hg, reorder=reorder)
+        say("piles geo", block, reorder)
+        cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=block,
+                                              path_cap=4096, engine=ddp_amd.ENGINE_PILES)
+        assert st["errors"] == 0
+        for i in range(0, 96, 5):
+            dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
+            assert cost[i] == dist[dst[i]], (i, cost[i], dist[dst[i]])
+            check_path(hg.off, hg.col, hg.w, paths[i], int(src[i]), int(dst[i]), int(dist[dst[i]]))
+        heaps, _, _ = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=block,
+                                          engine=ddp_amd.ENGINE_HEAPS)
+        assert np.array_equal(cost, heaps)
+        dg.close()
+    rng = np.random.default_rng(33 + block)
+    off, col, w = random_graph(rng, 4000, max_deg=6, wmax=100, tombstones=0.03)
+    rg = ddp_amd.HostGraph(off, col, w)
+    dg = ddp_amd.DeviceGraph(ctx, rg, reverse=True)
+    rs, rt = rg.reachable_pairs(40, 34)
+    rs = np.append(rs, [0, 5]).astype(np.int32)  # plus two arbitrary pairs (maybe unreachable)
+    rt = np.append(rt, [3999, 5]).astype(np.int32)
+    say("piles random zero", block)
+    cost, paths, st = ddp_amd.astar_batch(ctx, dg, rs, rt, heur=ddp_amd.H_ZERO, block=block, path_cap=4000,
+                                          engine=ddp_amd.ENGINE_PILES)
+    assert st["errors"] == 0
+    for i in range(len(rs)):
+        dist, _ = pyoracle.dijkstra(off, col, w, int(rs[i]))
+        want = int(dist[rt[i]]) if dist[rt[i]] >= 0 else ddp_amd.INF_COST
+        assert cost[i] == want, (i, cost[i], want)
+        if want != ddp_amd.INF_COST:
+            check_path(off, col, w, paths[i], int(rs[i]), int(rt[i]), want)
+    # one target, its reverse-BFS heuristic (h = -1 prunes)
+    t = int(rt[0])
+    h, _ = dg.bfs_hops(t, reverse=True, scale=int(w[col >= 0].min()), store_as_h=True)
+    ss = np.array(rs[:20], np.int32)
+    say("piles random bfs-h", block)
+    cost, _, st = ddp_amd.astar_batch(ctx, dg, ss, np.full(20, t, np.int32), heur=ddp_amd.H_ARRAY, block=block,
+                                      engine=ddp_amd.ENGINE_PILES)
+    for i in range(20):
+        dist, _ = pyoracle.dijkstra(off, col, w, int(ss[i]))
+        want = int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST
+        assert cost[i] == want
+    dg.close()
+    mz = ddp_amd.HostGraph.grid_maze(256, 200, 300, 35)
+    dg = ddp_amd.DeviceGraph(ctx, mz)
+    dg.set_grid(256)
+    ms, mt = mz.reachable_pairs(24, 36)
+    say("piles maze", block)
+    cost, _, st = ddp_amd.astar_batch(ctx, dg, ms, mt, heur=ddp_amd.H_MANHATTAN, block=block,
+                                      engine=ddp_amd.ENGINE_PILES)
+    for i in range(24):
+        dist, _ = pyoracle.dijkstra(mz.off, mz.col, mz.w, int(ms[i]), int(mt[i]))
+        assert cost[i] == dist[mt[i]]
+    dg.close()
+
+
+def test_batch_piles_overflow_is_reported(ctx):
+    """Queue memory too small for a query: the query is flagged, never
+    answered wrongly; the others stay exact."""
+    hg = ddp_amd.HostGraph.geometric(30000, 10, 100000, 37)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.geo_pairs(32, 100, 100000, 38)
+    b = ddp_amd.Batch(ctx, dg, 32, heur=ddp_amd.H_EUCLID, block=64, heap_cap=16, engine=ddp_amd.ENGINE_PILES)
+    b.upload(src, dst)
+    b.launch()
+    st = b.wait()
+    out = b.download()
+    b.close()
+    say("piles overflow", st)
+    assert st["errors"] > 0
+    for i in range(32):
+        if out["qerr"][i] == 0:
+            dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
+            assert out["cost"][i] == dist[dst[i]]

@@ -246,3 +246,37 @@ def test_apply_batch_matches_reference_genCSRwholegraph():
             assert newoff.tolist() == cur[0].tolist()
             assert rcol[keep].tolist() == cur[1].tolist()
             assert rw[keep].tolist() == cur[2].tolist()
+
+
+def test_threaded_euclid_batch_matches_faithful_astar():
+    """orc_astar_batch_euclid (thread pool, h computed on demand) gives the
+    same costs as orc_astar_ref run with a per-target floor(Euclid) array
+    computed here independently, and as Dijkstra (the heuristic is consistent:
+    edge weights are ceil(Euclid))."""
+    rng = np.random.default_rng(41)
+    n = 600
+    xy = rng.integers(0, 5000, size=(n, 2)).astype(np.int64)
+    us, vs, ws = [], [], []
+    for u in range(n):
+        d2 = ((xy - xy[u]) ** 2).sum(1)
+        for v in np.argsort(d2)[1:5]:
+            r = int(np.ceil(np.sqrt(float(d2[v]))))
+            while (r - 1) * (r - 1) >= d2[v]:
+                r -= 1
+            us.append(u); vs.append(int(v)); ws.append(max(1, r))
+    off, col, w = pyoracle.csr_from_edges(np.array(us), np.array(vs), np.array(ws))
+    src = rng.integers(0, n, 40).astype(np.int32)
+    dst = rng.integers(0, n, 40).astype(np.int32)
+    costs, ex, started = pyoracle.astar_batch_euclid(off, col, w, xy.reshape(-1).astype(np.int32), src, dst, 4)
+    assert started == 4 and ex > 0
+    for i in range(40):
+        d2 = ((xy - xy[dst[i]]) ** 2).sum(1)
+        h = np.floor(np.sqrt(d2.astype(np.float64))).astype(np.int64)
+        h -= (h * h > d2)
+        h += ((h + 1) * (h + 1) <= d2)
+        found, cost, _, _ = pyoracle.astar_ref(off, col, w, h.astype(np.int32), int(src[i]), int(dst[i]), quirk=False)
+        dist, _ = pyoracle.dijkstra(off, col, w, int(src[i]))
+        if found:
+            assert costs[i] == cost == dist[dst[i]]
+        else:
+            assert costs[i] == 2 ** 31 - 1 and dist[dst[i]] < 0
