@@ -192,11 +192,13 @@ def run_c4(args):
     log(f"workload ready in {time.time() - t0:.1f}s: N={hg.N} E={hg.E}")
     ctx = ddp_amd.Context(0)
     dg = ddp_amd.DeviceGraph(ctx, hg, reorder=not args.no_reorder)
-    block = args.block
+    block, engine = args.block, args.engine
     if block == 0:
         from ddp_amd import autotune
-        block = autotune.pick_batch_block(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID)
-    b = ddp_amd.Batch(ctx, dg, len(src), heur=ddp_amd.H_EUCLID, block=block, path_cap=4096)
+        tuned = autotune.tune_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, sample=len(src))
+        block, engine = tuned["block"], engine or tuned["engine"]
+        log(f"tuned: {tuned}")
+    b = ddp_amd.Batch(ctx, dg, len(src), heur=ddp_amd.H_EUCLID, block=block, path_cap=4096, engine=engine)
     b.upload(src, dst)
     for _ in range(args.warmup):
         b.launch()
@@ -239,7 +241,8 @@ def run_c4(args):
                       "4,096 queries with log-uniform separation, Euclidean heuristic, one persistent launch "
                       "per step")
     res["config"].update({"N": hg.N, "E": hg.E, "queries": len(src), "block": info["block"],
-                          "slots": info["slots"], "path_cap": 4096, "relabel": not args.no_reorder})
+                          "slots": info["slots"], "path_cap": 4096, "relabel": not args.no_reorder,
+                          "engine": "heaps" if engine == ddp_amd.ENGINE_HEAPS else "piles"})
     if len(sample):
         res["one_at_a_time"] = {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
                                 "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096}
@@ -250,7 +253,7 @@ def run_c4(args):
         "edges_per_s": round(stats["edges"] / (kmean * 1e-3), 1),
         "roofline": measured_traffic(roofline(algorithmic_bytes_geo(stats), kmean, "batch_kernel", copy_bandwidth(ctx)),
                                      workload="c4", block=info["block"], relabel=not args.no_reorder,
-                                     queries=len(src)),
+                                     queries=len(src), engine=res["config"]["engine"]),
     })
     if not args.no_serve:
         ups = geometric_updates(hg, args.serve_requests // 10, 100, 8)
@@ -461,7 +464,7 @@ def run_c5(args):
     t0 = time.time()
     hg = ddp_amd.HostGraph.rmat(scale, 16 * (1 << scale), 0.57, 0.19, 0.19, 1, 1000, 5)
     log(f"R-MAT scale {scale}: N={hg.N} E={hg.E} in {time.time() - t0:.1f}s")
-    src, dst = hg.reachable_pairs(args.c5_queries, 5)
+    src, dst = hg.hub_pairs(args.c5_queries, 5)
     ups = rmat_updates(hg, max(1, args.c5_requests // 15), 1000, 9)
     ctx = ddp_amd.Context(0)
     t0 = time.time()
@@ -513,6 +516,7 @@ def main():
     ap.add_argument("--workload", default="c4", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--queries", type=int, default=4096)
     ap.add_argument("--block", type=int, default=0, help="lanes per query workgroup (0 = autotune)")
+    ap.add_argument("--engine", type=int, default=0, help="batched queue engine: 0 auto, 1 heaps, 2 piles")
     ap.add_argument("--single-sample", type=int, default=256, help="queries answered one at a time")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu", action="store_true")

@@ -208,6 +208,18 @@ class HostGraph:
             L.ddp_hgraph_free(h)
         return src, dst
 
+    def hub_pairs(self, nq, seed):
+        """Connected pairs through the largest out-degree node (ddp_gen_hub_pairs)."""
+        L = _capi.lib()
+        src = np.zeros(nq, np.int32)
+        dst = np.zeros(nq, np.int32)
+        h = self._handle()
+        try:
+            _check(L.ddp_gen_hub_pairs(h, nq, seed, ptr_i32(src), ptr_i32(dst)))
+        finally:
+            L.ddp_hgraph_free(h)
+        return src, dst
+
     def _with_coords_handle(self):
         if self.xy is None:
             raise ValueError("graph has no coordinates")
@@ -522,9 +534,9 @@ class Server:
     QUERY, UPDATE = 0, 1
 
     def __init__(self, ctx, dg, heur=H_ZERO, scale=1, block=0, slots=0, heap_cap=0, ring=0, max_inserts=0,
-                 idle_ms=0):
+                 idle_ms=0, engine=ENGINE_AUTO):
         self.ctx, self.dg = ctx, dg
-        o = _capi.ServerOpts(block, slots, heap_cap, ring, max_inserts, idle_ms)
+        o = _capi.ServerOpts(block, slots, heap_cap, ring, max_inserts, idle_ms, engine)
         h = c_void_p()
         _check(_capi.lib().ddp_server_create(ctx.handle, dg.handle, _heur(heur, scale), byref(o), byref(h)),
                ctx.handle)

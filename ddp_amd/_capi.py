@@ -79,7 +79,7 @@ class DAStats(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("block", c_int32), ("slots", c_int32), ("heap_cap", c_int32), ("ring", c_int32),
-                ("max_inserts", c_int32), ("idle_ms", c_int32), ("reserved", c_int32 * 2)]
+                ("max_inserts", c_int32), ("idle_ms", c_int32), ("engine", c_int32), ("reserved", c_int32)]
 
 
 class ServerResult(ctypes.Structure):
@@ -169,6 +169,7 @@ SIGNATURES = {
     "ddp_gen_updates": (c_int, [c_void_p, c_int32, c_int32, c_int32, c_uint32, c_uint32, c_uint64, PP]),
     "ddp_gen_geo_pairs": (c_int, [c_void_p, c_int32, c_double, c_double, c_uint64, P_I32, P_I32]),
     "ddp_gen_reachable_pairs": (c_int, [c_void_p, c_int32, c_uint64, P_I32, P_I32]),
+    "ddp_gen_hub_pairs": (c_int, [c_void_p, c_int32, c_uint64, P_I32, P_I32]),
 }
 
 _lib = None

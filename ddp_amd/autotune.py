@@ -20,7 +20,7 @@ def tune_batch(ctx, dg, src, dst, heur, scale=1, sample=1024):
     q = c_float()
     _check(_capi.lib().ddp_tune_batch(ctx.handle, dg.handle, Heur(heur, scale), src.shape[0], ptr_i32(src),
                                       ptr_i32(dst), sample, byref(o), byref(q)), ctx)
-    return {"block": o.block, "queries_per_s": q.value}
+    return {"block": o.block, "engine": o.engine, "queries_per_s": q.value}
 
 
 def pick_batch_block(ctx, dg, src, dst, heur, scale=1, sample=1024):

@@ -185,7 +185,7 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
               "block must be 64, 128, 256, 512 or 1024 (got %d)", B);
     const int32_t engine = o ? o->engine : DDP_BATCH_AUTO;
     DDP_CHECK(engine >= DDP_BATCH_AUTO && engine <= DDP_BATCH_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
-    const int eng = engine == DDP_BATCH_PILES ? ENG_PILES : ENG_HEAPS;
+    const int eng = engine == DDP_BATCH_HEAPS ? ENG_HEAPS : ENG_PILES;  // AUTO: piles (2-3x on the C4 batch)
     // deeper LDS heaps keep sift-downs out of HBM (about 10% on the C4 batch at
     // block 512, where occupancy is set by waves rather than LDS); the pile
     // engine keeps its queues in HBM and needs no dynamic LDS
@@ -193,6 +193,8 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     if (eng == ENG_PILES) lcap = 0;
     DDP_CHECK((int64_t)lcap * B * 8 <= 64 * 1024, ctx, DDP_ERR_ARG, "lds_cap %d x block %d exceeds 64 KiB", lcap, B);
     int32_t cap = (o && o->heap_cap) ? o->heap_cap : std::max(lcap, 2048);
+    if (eng == ENG_PILES && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot
+        cap = (int32_t)std::min<int64_t>(std::max<int64_t>(cap, (3LL * g->N + B - 1) / B), INT32_MAX / 2);
     cap = std::max(cap, std::max(lcap, 16));
     const size_t lds = (size_t)lcap * B * 8;
     int per_cu = 0;

@@ -433,3 +433,78 @@ extern "C" int ddp_gen_reachable_pairs(const ddp_hgraph *g, int32_t nq, uint64_t
     }
     return DDP_OK;
 }
+
+// Pairs through a hub: h = the node of largest out-degree; one forward search
+// from h and one over reversed edges give In (nodes that reach h) and Out
+// (nodes h reaches); s uniform in In and t uniform in Out (s != t) are
+// connected through h.  Two graph traversals in total, where drawing uniform
+// pairs and testing each one costs a traversal per rejected pair (ruinous on
+// R-MAT graphs, whose many isolated nodes make most uniform pairs
+// unreachable).
+extern "C" int ddp_gen_hub_pairs(const ddp_hgraph *g, int32_t nq, uint64_t seed, int32_t *src, int32_t *dst)
+{
+    if (!g || nq < 0 || !src || !dst || g->N < 2) return DDP_ERR_ARG;
+    const int32_t N = g->N;
+    auto row_end = [&](int32_t u) { return (u + 1 < N) ? g->off[u + 1] : g->E; };
+    int32_t hub = 0, best = -1;
+    for (int32_t u = 0; u < N; ++u)
+        if (row_end(u) - g->off[u] > best) {
+            best = row_end(u) - g->off[u];
+            hub = u;
+        }
+    // forward reach of the hub
+    std::vector<char> fw(N, 0), bw(N, 0);
+    std::vector<int32_t> q;
+    q.reserve(N);
+    q.push_back(hub);
+    fw[hub] = 1;
+    for (size_t i = 0; i < q.size(); ++i)
+        for (int32_t j = g->off[q[i]]; j < row_end(q[i]); ++j) {
+            const int32_t c = g->col[j];
+            if (c >= 0 && !fw[c]) {
+                fw[c] = 1;
+                q.push_back(c);
+            }
+        }
+    // reversed edges, then the nodes that reach the hub
+    std::vector<int32_t> roff(N + 1, 0), rsrc;
+    for (int32_t j = 0; j < g->E; ++j)
+        if (g->col[j] >= 0) roff[g->col[j] + 1]++;
+    for (int32_t i = 0; i < N; ++i) roff[i + 1] += roff[i];
+    rsrc.resize(roff[N]);
+    {
+        std::vector<int32_t> fill(roff.begin(), roff.end() - 1);
+        for (int32_t u = 0; u < N; ++u)
+            for (int32_t j = g->off[u]; j < row_end(u); ++j)
+                if (g->col[j] >= 0) rsrc[fill[g->col[j]]++] = u;
+    }
+    q.clear();
+    q.push_back(hub);
+    bw[hub] = 1;
+    for (size_t i = 0; i < q.size(); ++i)
+        for (int32_t j = roff[q[i]]; j < roff[q[i] + 1]; ++j) {
+            const int32_t p = rsrc[j];
+            if (!bw[p]) {
+                bw[p] = 1;
+                q.push_back(p);
+            }
+        }
+    std::vector<int32_t> ins, outs;
+    for (int32_t v = 0; v < N; ++v) {
+        if (bw[v]) ins.push_back(v);
+        if (fw[v]) outs.push_back(v);
+    }
+    if (ins.size() + outs.size() < 3 || (ins.size() == 1 && outs.size() == 1)) return DDP_ERR_STATE;
+    Rng r(seed);
+    for (int32_t k = 0; k < nq; ++k) {
+        for (int tries = 0;; ++tries) {
+            if (tries > 1000) return DDP_ERR_STATE;
+            const int32_t s = ins[r.below(ins.size())], t = outs[r.below(outs.size())];
+            if (s == t) continue;
+            src[k] = s;
+            dst[k] = t;
+            break;
+        }
+    }
+    return DDP_OK;
+}

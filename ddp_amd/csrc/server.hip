@@ -77,8 +77,11 @@ struct ServeArgs {
     uint64_t *home;
     unsigned long long *stats;
     int32_t N, lcap, cap, maxit;
+    uint32_t ncap, fcap, delta0;  // pile engine: near ring, far buffers, first window
     uint64_t idle_ticks, wait_ticks;
 };
+
+enum { SENG_HEAPS = 0, SENG_PILES = 1 };
 
 __device__ __forceinline__ int32_t sys_ld_i(const int32_t *p)
 {
@@ -134,7 +137,7 @@ __device__ void apply_update(const ServeArgs &a, const Req &rq, uint32_t *s_err)
     block_sync_global();
 }
 
-template <int B, int HK>
+template <int B, int HK, int ENG>
 __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
 {
     extern __shared__ uint64_t s_heap[];
@@ -146,7 +149,9 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
     const int32_t N = a.N;
     uint64_t *gp = a.gp + slot * N;
     int32_t *touched = a.touched + slot * N;
-    const LaneHeap H{s_heap + lane, a.home + slot * a.cap * B + lane, B, a.lcap, (int64_t)B, a.cap};
+    uint64_t *home = a.home + slot * (int64_t)a.cap * B;  // this slot's queue memory
+    const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
+    const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap};
     ExpandCounters C;
     uint64_t stale = 0;
     for (;;) {
@@ -234,7 +239,10 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         } else if (!status) {
             GraphView G = a.g;
             G.ver = rq.version;
-            slot_search<B, HK>(G, a.h, gp, touched, H, s_heap, N, rq.s, rq.t, a.maxit, S, C, stale);
+            if constexpr (ENG == SENG_PILES)
+                slot_search_piles<B, HK>(G, a.h, gp, touched, P, N, rq.s, rq.t, a.maxit, a.delta0, S, C, stale);
+            else
+                slot_search<B, HK>(G, a.h, gp, touched, H, s_heap, N, rq.s, rq.t, a.maxit, S, C, stale);
             cost = (int32_t)S.F;
             status = S.err;
             slot_reset<B>(gp, touched, S);
@@ -267,25 +275,32 @@ __global__ void serve_fill_i32(int32_t *p, int64_t n, int32_t v)
 
 using ServeFn = void (*)(ServeArgs);
 
-template <int B>
+template <int B, int ENG>
 ServeFn pick_s(int hk)
 {
     switch (hk) {
-    case DDP_H_ZERO: return serve_kernel<B, DDP_H_ZERO>;
-    case DDP_H_ARRAY: return serve_kernel<B, DDP_H_ARRAY>;
-    case DDP_H_MANHATTAN: return serve_kernel<B, DDP_H_MANHATTAN>;
-    default: return serve_kernel<B, DDP_H_EUCLID>;
+    case DDP_H_ZERO: return serve_kernel<B, DDP_H_ZERO, ENG>;
+    case DDP_H_ARRAY: return serve_kernel<B, DDP_H_ARRAY, ENG>;
+    case DDP_H_MANHATTAN: return serve_kernel<B, DDP_H_MANHATTAN, ENG>;
+    default: return serve_kernel<B, DDP_H_EUCLID, ENG>;
     }
 }
 
-ServeFn pick_serve(int B, int hk)
+template <int ENG>
+ServeFn pick_se(int B, int hk)
 {
     switch (B) {
-    case 64: return pick_s<64>(hk);
-    case 128: return pick_s<128>(hk);
-    case 256: return pick_s<256>(hk);
-    default: return pick_s<512>(hk);
+    case 64: return pick_s<64, ENG>(hk);
+    case 128: return pick_s<128, ENG>(hk);
+    case 256: return pick_s<256, ENG>(hk);
+    case 512: return pick_s<512, ENG>(hk);
+    default: return pick_s<1024, ENG>(hk);
     }
+}
+
+ServeFn pick_serve(int B, int hk, int eng)
+{
+    return eng == SENG_PILES ? pick_se<SENG_PILES>(B, hk) : pick_se<SENG_HEAPS>(B, hk);
 }
 
 using clk = std::chrono::steady_clock;
@@ -298,6 +313,8 @@ struct ddp_server {
     const ddp_graph *g = nullptr;
     HeurView hv{};
     int32_t B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0;
+    int32_t eng = SENG_PILES;
+    uint32_t ncap = 0, fcap = 0, delta0 = 1;
     uint32_t ring = 0;
     int32_t log_cap = 0, log_used = 0;
     int32_t idle_ms = 2000;
@@ -383,11 +400,14 @@ static int server_launch(ddp_server *sv)
     a.lcap = sv->lcap;
     a.cap = sv->cap;
     a.maxit = sv->maxit;
+    a.ncap = sv->ncap;
+    a.fcap = sv->fcap;
+    a.delta0 = sv->delta0;
     a.idle_ticks = (uint64_t)sv->idle_ms * 100000ull;
     a.wait_ticks = 100000ull * 60000;  // a version wait longer than a minute is an error
     void *args[] = {&a};
-    DDP_HIP(ctx, hipLaunchKernel((const void *)pick_serve(sv->B, sv->hv.kind), dim3(sv->slots), dim3(sv->B), args,
-                                 (size_t)sv->lcap * sv->B * 8, ctx->stream));
+    DDP_HIP(ctx, hipLaunchKernel((const void *)pick_serve(sv->B, sv->hv.kind, sv->eng), dim3(sv->slots), dim3(sv->B),
+                                 args, (size_t)sv->lcap * sv->B * 8, ctx->stream));
     DDP_HIP(ctx, hipEventRecord(sv->done_ev, ctx->stream));
     sv->running = true;
     return DDP_OK;
@@ -427,8 +447,23 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     sv->N = g->N;
     sv->E = g->E;
     sv->B = (o && o->block) ? o->block : 256;
-    sv->lcap = std::max(4, std::min(32, 4096 / std::max(64, sv->B)));
+    const int32_t engine = o ? o->engine : DDP_SERVER_AUTO;
+    DDP_CHECK(engine >= DDP_SERVER_AUTO && engine <= DDP_SERVER_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
+    sv->eng = engine == DDP_SERVER_HEAPS ? SENG_HEAPS : SENG_PILES;
+    sv->lcap = sv->eng == SENG_PILES ? 0 : std::max(4, std::min(32, 4096 / std::max(64, sv->B)));
     sv->cap = (o && o->heap_cap) ? o->heap_cap : 2048;
+    if (sv->eng == SENG_PILES && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot
+        sv->cap = (int32_t)std::min<int64_t>(std::max<int64_t>(sv->cap, (3LL * g->N + sv->B - 1) / sv->B),
+                                             INT32_MAX / 2);
+    sv->cap = std::max(sv->cap, 16);
+    {
+        const uint64_t total = (uint64_t)sv->cap * sv->B;
+        uint64_t nc = 1;
+        while (nc * 2 <= total / 4) nc *= 2;
+        sv->ncap = (uint32_t)nc;
+        sv->fcap = (uint32_t)std::min<uint64_t>((total - nc) / 2, 0xFFFFFFFFull);
+        sv->delta0 = std::max<uint32_t>(1, 2 * g->meanw);
+    }
     sv->ring = (o && o->ring) ? (uint32_t)o->ring : 65536u;
     sv->log_cap = (o && o->max_inserts) ? o->max_inserts : (1 << 20);
     sv->idle_ms = (o && o->idle_ms) ? o->idle_ms : 2000;
@@ -439,10 +474,10 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
         delete sv;
         return code;
     };
-    if (!(sv->B == 64 || sv->B == 128 || sv->B == 256 || sv->B == 512) || (sv->ring & (sv->ring - 1)) ||
-        sv->ring < 64)
-        return fail(ddp_set_err(ctx, DDP_ERR_ARG, "block must be 64..512 and ring a power of two >= 64"));
-    const void *fn = (const void *)pick_serve(sv->B, hv.kind);
+    if (!(sv->B == 64 || sv->B == 128 || sv->B == 256 || sv->B == 512 || sv->B == 1024) ||
+        (sv->ring & (sv->ring - 1)) || sv->ring < 64)
+        return fail(ddp_set_err(ctx, DDP_ERR_ARG, "block must be 64..1024 and ring a power of two >= 64"));
+    const void *fn = (const void *)pick_serve(sv->B, hv.kind, sv->eng);
     int per_cu = 0;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sv->lcap * sv->B * 8));
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sv->B, (size_t)sv->lcap * sv->B * 8);

@@ -90,23 +90,28 @@ extern "C" int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
                               const int32_t *dst, int32_t sample, ddp_batch_opts_t *best, float *qps)
 {
     if (!ctx || !g || !best || nq <= 0 || !src || !dst) return DDP_ERR_ARG;
-    const std::string k = key(ctx, g, h, "batch", std::min(nq, sample));
+    const std::string k = key(ctx, g, h, "batch-engine", std::min(nq, sample));
     std::vector<long> cached;
-    if (cache_get(k, cached) && cached.size() >= 2) {
+    if (cache_get(k, cached) && cached.size() >= 3) {
         std::memset(best, 0, sizeof *best);
         best->block = (int32_t)cached[0];
-        if (qps) *qps = (float)cached[1];
+        best->engine = (int32_t)cached[1];
+        if (qps) *qps = (float)cached[2];
         return DDP_OK;
     }
     std::vector<int32_t> s, t;
     sample_queries(nq, src, dst, sample, s, t);
     const int32_t n = (int32_t)s.size();
     float best_qps = -1.f;
-    int32_t best_block = 256;
-    for (int32_t B : {128, 256, 512}) {
+    int32_t best_block = 256, best_engine = DDP_BATCH_PILES;
+    const int32_t cand[][2] = {{256, DDP_BATCH_PILES}, {512, DDP_BATCH_PILES}, {1024, DDP_BATCH_PILES},
+                               {512, DDP_BATCH_HEAPS}};
+    for (const auto &c : cand) {
+        const int32_t B = c[0];
         ddp_batch_opts_t o;
         std::memset(&o, 0, sizeof o);
         o.block = B;
+        o.engine = c[1];
         ddp_batch *b = nullptr;
         if (ddp_batch_create(ctx, g, h, n, &o, &b) != DDP_OK) continue;
         float ms = 1e30f;
@@ -122,13 +127,15 @@ extern "C" int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
         if (q > best_qps) {
             best_qps = q;
             best_block = B;
+            best_engine = c[1];
         }
     }
     if (best_qps < 0) return DDP_ERR_STATE;
     std::memset(best, 0, sizeof *best);
     best->block = best_block;
+    best->engine = best_engine;
     if (qps) *qps = best_qps;
-    cache_put(k, {best_block, (long)best_qps});
+    cache_put(k, {best_block, best_engine, (long)best_qps});
     return DDP_OK;
 }
 

@@ -21,8 +21,9 @@ typedef struct ddp_batch ddp_batch;
 typedef struct {
     int32_t block;      /* lanes (= queues) per query workgroup: 64..1024; 0 = 256 */
     int32_t slots;      /* concurrent queries; 0 = what fits (occupancy and free HBM) */
-    int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = 2048.  A slot's
-                           queue memory is heap_cap * block entries in either engine */
+    int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = 2048 (heaps) or
+                           max(2048, 3N / block) (piles).  A slot's queue memory is
+                           heap_cap * block entries in either engine */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
     int32_t max_iters;  /* per-query iteration limit; 0 = auto */
     int32_t path_cap;   /* path entries kept per query (0 = costs only) */
@@ -34,7 +35,7 @@ typedef struct {
  * GA* structure; an iteration pops one node per lane).  PILES: a near ring of
  * entries under a moving f threshold and a far pile for the rest, both in HBM;
  * a round expands up to one near entry per lane and no heap is ever sifted.
- * Both return exact costs. */
+ * Both return exact costs.  AUTO = PILES. */
 #define DDP_BATCH_AUTO  0
 #define DDP_BATCH_HEAPS 1
 #define DDP_BATCH_PILES 2

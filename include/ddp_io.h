@@ -100,6 +100,11 @@ int ddp_gen_geo_pairs(const ddp_hgraph *g, int32_t nq, double dmin, double dmax,
 /* Uniform pairs (s != t) such that t is reachable from s (checked on the host). */
 int ddp_gen_reachable_pairs(const ddp_hgraph *g, int32_t nq, uint64_t seed, int32_t *src,
                             int32_t *dst);
+/* Connected pairs for large graphs: s uniform among the nodes that reach the
+ * hub (largest out-degree), t uniform among the nodes the hub reaches (s != t);
+ * two traversals in total.  DDP_ERR_STATE if the hub's reach is too small. */
+int ddp_gen_hub_pairs(const ddp_hgraph *g, int32_t nq, uint64_t seed, int32_t *src,
+                      int32_t *dst);
 
 #ifdef __cplusplus
 }

@@ -35,14 +35,22 @@ extern "C" {
 typedef struct ddp_server ddp_server;
 
 typedef struct {
-    int32_t block;          /* lanes (= queues) per query slot: 64..512; 0 = 256 */
+    int32_t block;          /* lanes per query slot: 64..1024; 0 = 256 */
     int32_t slots;          /* 0 = as many as occupancy and free HBM allow */
-    int32_t heap_cap;       /* entries per queue; 0 = 2048 */
+    int32_t heap_cap;       /* queue entries per lane (a slot holds heap_cap * block);
+                               0 = 2048 (heaps) or max(2048, 3N / block) (piles) */
     int32_t ring;           /* request ring entries (power of two); 0 = 65536 */
     int32_t max_inserts;    /* capacity of the inserted-edge log; 0 = 1<<20 */
     int32_t idle_ms;        /* the kernel exits after this long without work; 0 = 2000 */
-    int32_t reserved[2];
+    int32_t engine;         /* DDP_SERVER_AUTO / _HEAPS / _PILES (as DDP_BATCH_*) */
+    int32_t reserved;
 } ddp_server_opts_t;
+
+/* Queue discipline of a query slot, as for batches (include/ddp_batch.h):
+ * per-lane heaps, or near/far piles.  AUTO = PILES. */
+#define DDP_SERVER_AUTO  0
+#define DDP_SERVER_HEAPS 1
+#define DDP_SERVER_PILES 2
 
 #define DDP_REQ_QUERY 0
 #define DDP_REQ_UPDATE 1

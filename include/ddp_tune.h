@@ -16,9 +16,10 @@
 extern "C" {
 #endif
 
-/* Times the batched kernel for block sizes 128/256/512 on up to `sample`
- * of the given queries (spread over the schedule order) and writes the
- * fastest options to *best; *qps gets its measured queries/s. */
+/* Times the batched kernel (pile engine at blocks 256/512/1024, heap engine
+ * at 512) on up to `sample` of the given queries (spread over the schedule
+ * order) and writes the fastest options (block, engine) to *best; *qps gets
+ * its measured queries/s. */
 int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
                    const int32_t *dst, int32_t sample, ddp_batch_opts_t *best, float *qps);
 

@@ -174,3 +174,13 @@ def test_generators_are_seeded_and_well_formed():
     u = a.gen_updates(3, 50, 40, 1, 1000, 3)
     assert len(u.batches) == 3 and all(len(bt) == 90 for bt in u.batches)
     assert all(l[0] == 0 for l in u.batches[0][:40]) and all(l[0] == 1 for l in u.batches[0][40:])
+
+
+def test_hub_pairs_are_connected():
+    hg = ddp_amd.HostGraph.rmat(12, 8 << 12, 0.57, 0.19, 0.19, 1, 100, 7)
+    src, dst = hg.hub_pairs(40, 8)
+    assert len(set(zip(src.tolist(), dst.tolist()))) > 30
+    for s, t in zip(src, dst):
+        assert s != t
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(s))
+        assert dist[t] >= 0

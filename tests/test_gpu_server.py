@@ -68,9 +68,10 @@ def test_server_versions_and_costs(ctx):
     hg = ddp_amd.HostGraph.random(6000, 5, 1, 100, 41)
     dg = ddp_amd.DeviceGraph(ctx, hg)
     arrival, types, src, dst, ups = make_trace(hg, 400, 20, 42)
-    for block in (64, 256):
-        say("server block", block)
-        sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, block=block, slots=48)
+    for block, engine in ((64, ddp_amd.ENGINE_HEAPS), (256, ddp_amd.ENGINE_HEAPS), (64, ddp_amd.ENGINE_PILES),
+                          (256, ddp_amd.ENGINE_PILES), (1024, ddp_amd.ENGINE_PILES)):
+        say("server block", block, "engine", engine)
+        sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, block=block, slots=48, engine=engine)
         res = sv.replay(arrival, types, src, dst, ups)
         say("  info", sv.info(), "p50 device ms", np.median([r["device_ms"] for r in res]))
         check(res, types, src, dst, ups, hg)
