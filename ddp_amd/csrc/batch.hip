@@ -180,12 +180,12 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     HeurView hv;
     int rc = ddp_make_heur(g, h, -1, &hv);
     if (rc) return rc;
-    int32_t B = (o && o->block) ? o->block : 256;
-    DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512 || B == 1024, ctx, DDP_ERR_ARG,
-              "block must be 64, 128, 256, 512 or 1024 (got %d)", B);
     const int32_t engine = o ? o->engine : DDP_BATCH_AUTO;
     DDP_CHECK(engine >= DDP_BATCH_AUTO && engine <= DDP_BATCH_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
-    const int eng = engine == DDP_BATCH_HEAPS ? ENG_HEAPS : ENG_PILES;  // AUTO: piles (2-3x on the C4 batch)
+    const int eng = engine == DDP_BATCH_HEAPS ? ENG_HEAPS : ENG_PILES;  // AUTO: piles (3x on the C4 batch)
+    int32_t B = (o && o->block) ? o->block : (eng == ENG_PILES ? 1024 : 256);
+    DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512 || B == 1024, ctx, DDP_ERR_ARG,
+              "block must be 64, 128, 256, 512 or 1024 (got %d)", B);
     // deeper LDS heaps keep sift-downs out of HBM (about 10% on the C4 batch at
     // block 512, where occupancy is set by waves rather than LDS); the pile
     // engine keeps its queues in HBM and needs no dynamic LDS

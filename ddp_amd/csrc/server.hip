@@ -446,10 +446,11 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     sv->hv = hv;
     sv->N = g->N;
     sv->E = g->E;
-    sv->B = (o && o->block) ? o->block : 256;
     const int32_t engine = o ? o->engine : DDP_SERVER_AUTO;
     DDP_CHECK(engine >= DDP_SERVER_AUTO && engine <= DDP_SERVER_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
     sv->eng = engine == DDP_SERVER_HEAPS ? SENG_HEAPS : SENG_PILES;
+    // 1024-lane pile slots served the C4 mix with the best capacity and tail latency
+    sv->B = (o && o->block) ? o->block : (sv->eng == SENG_PILES ? 1024 : 256);
     sv->lcap = sv->eng == SENG_PILES ? 0 : std::max(4, std::min(32, 4096 / std::max(64, sv->B)));
     sv->cap = (o && o->heap_cap) ? o->heap_cap : 2048;
     if (sv->eng == SENG_PILES && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot

@@ -19,7 +19,7 @@ extern "C" {
 typedef struct ddp_batch ddp_batch;
 
 typedef struct {
-    int32_t block;      /* lanes (= queues) per query workgroup: 64..1024; 0 = 256 */
+    int32_t block;      /* lanes per query workgroup: 64..1024; 0 = 1024 (piles), 256 (heaps) */
     int32_t slots;      /* concurrent queries; 0 = what fits (occupancy and free HBM) */
     int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = 2048 (heaps) or
                            max(2048, 3N / block) (piles).  A slot's queue memory is

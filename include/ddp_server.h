@@ -35,7 +35,7 @@ extern "C" {
 typedef struct ddp_server ddp_server;
 
 typedef struct {
-    int32_t block;          /* lanes per query slot: 64..1024; 0 = 256 */
+    int32_t block;          /* lanes per query slot: 64..1024; 0 = 1024 (piles), 256 (heaps) */
     int32_t slots;          /* 0 = as many as occupancy and free HBM allow */
     int32_t heap_cap;       /* queue entries per lane (a slot holds heap_cap * block);
                                0 = 2048 (heaps) or max(2048, 3N / block) (piles) */
