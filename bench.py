@@ -471,8 +471,7 @@ def run_c5(args):
     dg = ddp_amd.DeviceGraph(ctx, hg, reorder=not args.no_reorder)
     log(f"uploaded in {time.time() - t0:.1f}s")
     # batched kernel on all queries (unmodified graph): reference costs for the served leg
-    cost_b, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, block=256,
-                                         heap_cap=args.c5_heap_cap)
+    cost_b, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, heap_cap=args.c5_heap_cap)
     if bst["errors"]:
         raise SystemExit(f"C5 batch: {bst['errors']} queries failed")
     log(f"C5 batch: {bst}")
@@ -480,7 +479,7 @@ def run_c5(args):
     for i in range(min(args.single_sample, 8)):
         q0 = time.perf_counter()
         r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_ZERO, K=8192,
-                          heap_cap=args.c5_heap_cap // 2)
+                          heap_cap=8192)
         lat1.append((time.perf_counter() - q0) * 1e3)
         if r.cost != cost_b[i]:
             raise SystemExit(f"C5 query {i}: single {r.cost} batch {cost_b[i]}")
@@ -499,7 +498,8 @@ def run_c5(args):
                       f"0.57/0.19/0.19/0.05, seed 5), w in [1,1000], h = 0, {args.c5_queries} reachable pairs, "
                       f"95% queries, 5% update batches of 1,000 deletes + 1,000 inserts; value = closed-loop "
                       f"served capacity")
-    res["config"].update({"N": hg.N, "E": hg.E, "relabel": not args.no_reorder, "heap_cap": args.c5_heap_cap})
+    res["config"].update({"N": hg.N, "E": hg.E, "relabel": not args.no_reorder, "block": bst["block"],
+                          "slots": bst["slots"]})
     res.update({"p50_ms": lat_at["p50_ms"], "p99_ms": lat_at["p99_ms"], "latency_at_qps": lat_at["offered_qps"],
                 "sweep": sweep, "batched_queries_per_s": round(len(src) / (bst["kernel_ms"] * 1e-3), 2),
                 "one_at_a_time": {"queries_per_s": round(1000.0 / float(np.mean(lat1)), 3),
@@ -534,8 +534,8 @@ def main():
     ap.add_argument("--rmat-scale", type=int, default=23)
     ap.add_argument("--c5-queries", type=int, default=256)
     ap.add_argument("--c5-requests", type=int, default=1024)
-    ap.add_argument("--c5-heap-cap", type=int, default=16384)
-    ap.add_argument("--c5-qps", type=float, nargs="+", default=[0.0, 50.0, 200.0],
+    ap.add_argument("--c5-heap-cap", type=int, default=0, help="queue entries per lane (0 = library default)")
+    ap.add_argument("--c5-qps", type=float, nargs="+", default=[0.0, 40.0],
                     help="offered loads of the served leg (0 = closed loop, all requests at once)")
     args = ap.parse_args()
     if args.gpus != 1:

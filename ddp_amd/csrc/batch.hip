@@ -95,7 +95,7 @@ __global__ void batch_fill_u64(uint64_t *p, int64_t n, uint64_t v)
 __global__ void batch_reset(uint32_t *next, unsigned long long *stats)
 {
     if (threadIdx.x == 0) *next = 0;
-    if (threadIdx.x < 4) stats[threadIdx.x] = 0;
+    if (threadIdx.x < 8) stats[threadIdx.x] = 0;
 }
 
 using BatchFn = void (*)(BatchArgs);
@@ -403,12 +403,13 @@ extern "C" int ddp_batch_wait(ddp_batch *b, ddp_batch_stats_t *st)
     b->last_ms = ms;
     if (st) {
         std::memset(st, 0, sizeof *st);
-        unsigned long long v[4];
-        DDP_HIP(ctx, hipMemcpy(v, b->stats, 32, hipMemcpyDeviceToHost));
+        unsigned long long v[5];
+        DDP_HIP(ctx, hipMemcpy(v, b->stats, 40, hipMemcpyDeviceToHost));
         st->expansions = (int64_t)v[0];
         st->edges = (int64_t)v[1];
         st->improvements = (int64_t)v[2];
         st->stale_pops = (int64_t)v[3];
+        st->pile_rebuilds = (int64_t)v[4];
         st->slots = b->slots;
         st->block = b->B;
         st->kernel_ms = ms;

@@ -26,6 +26,7 @@ struct ExpandLds {
 
 struct ExpandCounters {
     uint64_t expanded = 0, edges = 0, improved = 0;
+    uint64_t rebuilds = 0;  // pile engine: far piles rebuilt after an overflow (lane 0 counts)
 };
 
 // Edges handled per lane and round: more overlap for small blocks, fewer

@@ -24,6 +24,7 @@ struct SlotLds {
     uint32_t far_n, far_n2;         // entries in the current / next far pile
     uint32_t thr, delta;            // near window [.., thr) and its width
     uint32_t phase_n;               // near entries taken since the last refill
+    uint32_t ovf;                   // a push found both piles full and was dropped
 };
 
 template <int B>
@@ -150,6 +151,54 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
 // dropped when it is stale or cannot lead to anything below F(t).  The window
 // width adapts: doubled when a phase fed the lanes fewer than 2B entries,
 // halved when it ran past 32B.
+// Overflow: a push that finds the near ring full goes to the far pile; one
+// that finds the far pile full too is dropped and flagged, and the next refill
+// rebuilds the far pile from the slot's touched log (every node whose word
+// left INF, at most N) with its current f.  That re-queues some nodes that were
+// already expanded at their final f, which is harmless in a label-correcting
+// search, so results stay exact; only a far pile smaller than the touched set
+// is an error (QERR_HEAP).
+
+// Wave-aggregated reservations (one LDS atomic per wave).  near_reserve
+// takes ring slots only while the ring has room (the tail never runs more
+// than ncap ahead of head) and may grant fewer than asked; far_reserve always
+// grants.  Both return the lane's index, or UINT32_MAX when it got none.
+// Every lane of the wave must call them.
+__device__ __forceinline__ uint32_t near_reserve(bool want, uint32_t *tail, uint32_t head, uint32_t ncap)
+{
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0xFFFFFFFFu;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0, got = 0;
+    if ((int)(threadIdx.x & 63) == leader) {
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        uint32_t cur = __hip_atomic_load(tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (;;) {
+            const uint32_t used = cur - head;
+            got = used < ncap ? (ncap - used < cnt ? ncap - used : cnt) : 0u;
+            if (got == 0) break;
+            if (__hip_atomic_compare_exchange_weak(tail, &cur, cur + got, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP))
+                break;
+        }
+        base = cur;
+    }
+    base = __shfl(base, leader, 64);
+    got = __shfl(got, leader, 64);
+    const uint32_t r = lane_rank(m);
+    return (want && r < got) ? base + r : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ uint32_t far_reserve(bool want, uint32_t *count)
+{
+    const uint64_t m = __ballot(want);
+    if (m == 0) return 0xFFFFFFFFu;
+    const int leader = __builtin_ctzll(m);
+    uint32_t base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return want ? base + lane_rank(m) : 0xFFFFFFFFu;
+}
 
 struct PilePtrs {
     uint64_t *near;  // ncap entries (power of two), ring
@@ -174,18 +223,14 @@ struct PilePush {
     }
     __device__ void operator()(bool has, uint64_t key)
     {
-        if (!has) return;
         const uint32_t f = (uint32_t)(key >> 32);
-        if (f >= __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) return;
-        if (f < S->thr) {
-            const uint32_t idx = atomicAdd(&S->near_tail, 1u);
-            if (idx - S->near_head < ncap) near[idx & nmask] = key;
-            else err |= QERR_HEAP;
-        } else {
-            const uint32_t idx = atomicAdd(&S->far_n, 1u);
-            if (idx < fcap) far[idx] = key;
-            else err |= QERR_HEAP;
-        }
+        has = has && f < __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t ni = near_reserve(has && f < S->thr, &S->near_tail, S->near_head, ncap);
+        if (ni != 0xFFFFFFFFu) near[ni & nmask] = key;
+        const uint32_t fi = far_reserve(has && ni == 0xFFFFFFFFu, &S->far_n);
+        if (fi == 0xFFFFFFFFu) return;
+        if (fi < fcap) far[fi] = key;
+        else S->ovf = 1;  // dropped: the next refill rebuilds the far pile
     }
 };
 
@@ -213,6 +258,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
         S.delta = delta0 > 0 ? delta0 : 1;
         S.thr = 0;
         S.phase_n = 0;
+        S.ovf = 0;
     }
     __syncthreads();
     bool run = s >= 0 && s < N && t >= 0 && t < N;
@@ -242,6 +288,36 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
     while (run) {
         const uint32_t head = S.near_head, tail = S.near_tail, F = S.F;
         if (head == tail) {
+            if (S.ovf) {
+                // ---- rebuild the far pile from the touched log (entries were dropped)
+                const int32_t nt = S.ntouch;
+                if (lane == 0) S.far_n2 = 0;
+                __syncthreads();
+                for (int32_t i0 = 0; i0 < nt; i0 += B) {  // uniform trip count: the reservations are wave-wide
+                    const int32_t i = i0 + lane;
+                    int32_t v = -1;
+                    uint32_t f = 0xFFFFFFFFu;
+                    if (i < nt) {
+                        v = __hip_atomic_load(touched + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        f = (uint32_t)(ld_gp(gp + v) >> 32);
+                    }
+                    const uint32_t idx = far_reserve(f < F, &S.far_n2);  // f < F also excludes t
+                    if (idx == 0xFFFFFFFFu) continue;
+                    if (idx < P.fcap) far2[idx] = ((uint64_t)f << 32) | (uint32_t)v;
+                    else atomicOr(&S.err, (uint32_t)QERR_HEAP);
+                }
+                block_sync_global();
+                if (lane == 0) {
+                    S.far_n = S.far_n2;
+                    S.ovf = 0;
+                    C.rebuilds++;
+                }
+                uint64_t *tmp = far;
+                far = far2;
+                far2 = tmp;
+                __syncthreads();
+                if (S.err) break;
+            }
             // ---- refill the near ring from the far pile
             const uint32_t nf = S.far_n < P.fcap ? S.far_n : P.fcap;
             uint32_t mymin = 0xFFFFFFFFu;
@@ -261,17 +337,19 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             }
             __syncthreads();
             const uint32_t thr = S.thr;
-            for (uint32_t i = lane; i < nf; i += B) {
-                const uint64_t key = far[i];
-                const uint32_t f = (uint32_t)(key >> 32);
-                if (f >= F || (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) != f) continue;  // hopeless or stale
-                if (f < thr) {
-                    const uint32_t idx = atomicAdd(&S.near_tail, 1u);
-                    if (idx - head < P.ncap) P.near[idx & nmask] = key;
-                    else atomicOr(&S.err, (uint32_t)QERR_HEAP);
-                } else {
-                    far2[atomicAdd(&S.far_n2, 1u)] = key;  // fewer than nf <= fcap entries
+            for (uint32_t i0 = 0; i0 < nf; i0 += B) {  // uniform trip count: the reservations are wave-wide
+                const uint32_t i = i0 + lane;
+                uint64_t key = 0;
+                bool live = false;
+                if (i < nf) {
+                    key = far[i];
+                    const uint32_t f = (uint32_t)(key >> 32);
+                    live = f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f;  // else hopeless or stale
                 }
+                const uint32_t ni = near_reserve(live && (uint32_t)(key >> 32) < thr, &S.near_tail, head, P.ncap);
+                if (ni != 0xFFFFFFFFu) P.near[ni & nmask] = key;
+                const uint32_t fi = far_reserve(live && ni == 0xFFFFFFFFu, &S.far_n2);
+                if (fi != 0xFFFFFFFFu) far2[fi] = key;  // fewer than nf <= fcap entries
             }
             block_sync_global();
             if (lane == 0) S.far_n = S.far_n2;
@@ -367,11 +445,12 @@ template <int B>
 __device__ void slot_flush_stats(unsigned long long *stats, const ExpandCounters &C, uint64_t stale)
 {
     const uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
-                   v3 = wave_sum_u64(stale);
+                   v3 = wave_sum_u64(stale), v4 = wave_sum_u64(C.rebuilds);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(stats + 0, (unsigned long long)v0);
         atomicAdd(stats + 1, (unsigned long long)v1);
         atomicAdd(stats + 2, (unsigned long long)v2);
         atomicAdd(stats + 3, (unsigned long long)v3);
+        if (v4) atomicAdd(stats + 4, (unsigned long long)v4);
     }
 }

@@ -46,6 +46,7 @@ typedef struct {
     int32_t errors;     /* queries whose flags are non-zero */
     float kernel_ms;    /* launch-to-finish device time (HIP events on the ctx stream) */
     float total_ms;
+    int64_t pile_rebuilds;  /* pile engine: far piles rebuilt after an overflow */
 } ddp_batch_stats_t;
 
 /* Plan: allocates the query slots ((f,parent) arrays, touched logs, heaps) once

@@ -323,3 +323,31 @@ def test_batch_piles_overflow_is_reported(ctx):
         if out["qerr"][i] == 0:
             dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
             assert out["cost"][i] == dist[dst[i]]
+
+
+def test_batch_piles_rebuild_after_overflow(ctx):
+    """One expansion overflows both piles: hub 1 has 5,000 leaf edges of
+    weight 10^6 after its first edge 1 -> 2 -> 3 (the target).  The dropped
+    pushes trigger a rebuild of the far pile from the touched log, which keeps
+    only nodes with f below F(t) = 3; the answer stays exact."""
+    nleaf = 5000
+    N = 4 + nleaf
+    off = np.zeros(N, np.int32)
+    col = [1, 2] + list(range(4, N)) + [3]
+    w = [1, 1] + [1_000_000] * nleaf + [1]
+    off[1], off[2] = 1, 2 + nleaf
+    off[3:] = 3 + nleaf
+    hg = ddp_amd.HostGraph(off, np.array(col, np.int32), np.array(w, np.uint32))
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    # block 64, heap_cap 64: 4,096 entries = near ring 1,024 + two far piles of 1,536
+    b = ddp_amd.Batch(ctx, dg, 2, heur=ddp_amd.H_ZERO, block=64, heap_cap=64, engine=ddp_amd.ENGINE_PILES,
+                      path_cap=8)
+    b.upload(np.array([0, 0], np.int32), np.array([3, 2], np.int32))
+    b.launch()
+    st = b.wait()
+    out = b.download(paths=True)
+    b.close()
+    say("piles rebuild", st)
+    assert st["errors"] == 0 and st["pile_rebuilds"] >= 1
+    assert out["cost"].tolist() == [3, 2]
+    assert out["paths"][0].tolist() == [0, 1, 2, 3]
