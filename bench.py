@@ -144,7 +144,7 @@ def geometric_updates(hg, n, per, seed):
     return out
 
 
-def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=0):
+def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=0, block=0):
     """Open-loop replay: Poisson arrivals at `qps` (all at once when qps <= 0),
     95% point queries and 5% update batches.  Returns (results, types, query
     index per request, secs, server info)."""
@@ -159,7 +159,7 @@ def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=
         ups.append(upd_batches[k] if ty else None)
         k += int(ty)
     arrival = np.zeros(nreq) if qps <= 0 else np.cumsum(rng.exponential(1000.0 / qps, nreq))
-    sv = ddp_amd.Server(ctx, dg, heur=heur, heap_cap=heap_cap)
+    sv = ddp_amd.Server(ctx, dg, heur=heur, heap_cap=heap_cap, block=block)
     t0 = time.perf_counter()
     out = sv.replay(arrival, types, src[qi], dst[qi], ups)
     el = time.perf_counter() - t0
@@ -258,7 +258,7 @@ def run_c4(args):
     if not args.no_serve:
         ups = geometric_updates(hg, args.serve_requests // 10, 100, 8)
         o, ty, qi, el, sinfo = serve_trace(ctx, dg, ddp_amd.H_EUCLID, src, dst, args.serve_qps, args.serve_requests,
-                                           ups, 7)
+                                           ups, 7, block=args.serve_block)
         res["served"] = served_summary(o, ty, qi, el, sinfo, args.serve_qps, costs)
         log(f"served: {res['served']}")
     if not args.no_cpu:
@@ -525,8 +525,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep the input node numbering on the device (default: Hilbert/BFS relabelling)")
-    ap.add_argument("--serve-qps", type=float, default=5000.0, help="offered load of the served leg")
+    ap.add_argument("--serve-qps", type=float, default=5000.0,
+                    help="offered load of the served leg (0 = closed loop: every request at once)")
     ap.add_argument("--serve-requests", type=int, default=8192)
+    ap.add_argument("--serve-block", type=int, default=0, help="lanes per serving slot (0 = library default)")
     ap.add_argument("--c3-nodes", type=int, default=1000000)
     ap.add_argument("--c3-batches", type=int, default=20)
     ap.add_argument("--c3-batch", type=int, default=20000, help="lines per batch (half inserts, half deletes)")
