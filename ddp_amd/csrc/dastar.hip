@@ -18,8 +18,13 @@
 //   5. GA* resumes: queued nodes are expanded only while they can still beat
 //      the target, instead of the unbounded Bellman-Ford waves of
 //      insert_propagate / delete_propagate (NC:556-901).
-// The graph is base rows with tombstones plus delta rows holding the inserts
-// in arrival order, so the row of u reads exactly as GA_merge.cu's merged row.
+// The graph lives on the device: base rows with tombstones, plus a log of
+// inserted edges threaded onto per-source lists (insert order) and per-target
+// lists, so the row of u reads exactly as GA_merge.cu's merged row (surviving
+// base edges, then inserts in arrival order).  A batch is applied by kernels:
+// each delete CASes the first live matching instance (base row first, then the
+// source's list), inserts are appended to both lists.  The host does O(batch)
+// work per batch (checking and grouping the lines), never O(N).
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -36,48 +41,121 @@ float ms_since(clk::time_point t0) { return std::chrono::duration<float, std::mi
 
 enum : int { RC_NEXT = 0, RC_MARKED = 1, RC_ROOTS = 2, RC_REPULLED = 3, RC_PUSHES = 4, RC_WORDS = 16 };
 
-__global__ void scatter_tombstones(int32_t *col, const int32_t *pos, int32_t n)
-{
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) col[pos[i]] = -1;
-}
+enum : int { RC_DEL_BASE = 5, RC_DEL_LOG = 6 };
 
-// Roots: deleted u->v where v's parent is u.  Claims v for this epoch.
-__global__ void seed_roots(const int32_t *pairs, int32_t n, const uint64_t *gp, int32_t *mark, int32_t epoch,
-                           int32_t *front, int32_t *marked, uint32_t *rctrl)
+// The log of inserted edges: entry x is u -> v with weight w, live while
+// del[x] == INT32_MAX.  head/next links pack (entry << 32 | 0); the end of a
+// list is ~0.  Lists are only appended between searches.
+struct InsLog {
+    int32_t *u, *v;
+    uint32_t *w;
+    int32_t *del;
+    uint64_t *fhead, *fnext;  // by source
+    uint64_t *rhead, *rnext;  // by target
+    int32_t *ftail, *rtail;   // last entry of each list (appends)
+};
+
+__device__ __forceinline__ bool link_end(uint64_t link) { return (uint32_t)link != 0u; }
+
+// Deletes, one thread per line: the first live instance of u -> v, base row
+// first, then u's inserted edges oldest first (DA_star/GA_merge.cu:1019-1037
+// on the merged row).  Concurrent deletes of the same pair CAS distinct
+// instances, so k deletes remove the first k live ones, as a sequential pass
+// would.  A deleted edge that was v's parent edge makes v a root of the
+// subtree to invalidate.
+__global__ void apply_deletes(const int32_t *pairs, int32_t n, GraphView F, int32_t *col, int32_t *rcol,
+                              const int32_t *roff, const int32_t *reid, InsLog L, const uint64_t *gp, int32_t *mark,
+                              int32_t epoch, int32_t *front, int32_t *marked, uint32_t *rctrl)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int32_t u = pairs[2 * i], v = pairs[2 * i + 1];
-    const uint64_t x = ld_gp(gp + v);
-    if ((uint32_t)(x >> 32) == DDP_INF_F || (uint32_t)x != (uint32_t)u) return;
+    bool found = false;
+    for (int32_t j = F.off[u]; j < F.off[u + 1] && !found; ++j) {
+        if (__hip_atomic_load(col + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != v) continue;
+        if (atomicCAS(col + j, v, -1) != v) continue;
+        found = true;
+        for (int32_t r = roff[v]; r < roff[v + 1]; ++r)
+            if (reid[r] == j) {
+                rcol[r] = -1;
+                break;
+            }
+        atomicAdd(rctrl + RC_DEL_BASE, 1u);
+    }
+    for (uint64_t link = L.fhead[u]; !found && !link_end(link);) {
+        const int32_t x = (int32_t)(link >> 32);
+        link = L.fnext[x];
+        if (L.v[x] == v && atomicCAS(L.del + x, INT32_MAX, 0) == INT32_MAX) {
+            found = true;
+            atomicAdd(rctrl + RC_DEL_LOG, 1u);
+        }
+    }
+    if (!found) return;
+    const uint64_t word = ld_gp(gp + v);
+    if ((uint32_t)(word >> 32) == DDP_INF_F || (uint32_t)word != (uint32_t)u) return;
     if (atomicMax(mark + v, epoch) == epoch) return;
-    const uint32_t k = atomicAdd(rctrl + RC_NEXT, 1u);
-    front[k] = v;
+    front[atomicAdd(rctrl + RC_NEXT, 1u)] = v;
     marked[atomicAdd(rctrl + RC_MARKED, 1u)] = v;
     atomicAdd(rctrl + RC_ROOTS, 1u);
 }
 
+// New log entries: one thread per group of equal keys (entries arrive sorted
+// by key, stably, so a group keeps insert order) appends the group to the
+// key's list.  grp holds (key, entry) pairs.
+__global__ void log_append(const int32_t *grp, int32_t n, uint64_t *head, uint64_t *next, int32_t *tail)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t key = grp[2 * i];
+    if (i > 0 && grp[2 * (i - 1)] == key) return;
+    for (int32_t k = i; k < n && grp[2 * k] == key; ++k) {
+        const int32_t x = grp[2 * k + 1];
+        next[x] = ~0ull;
+        const uint64_t link = (uint64_t)(uint32_t)x << 32;
+        const int32_t last = tail[key];
+        if (last < 0) head[key] = link;
+        else next[last] = link;
+        tail[key] = x;
+    }
+}
+
+__global__ void fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void fill_u64(uint64_t *p, int64_t n, uint64_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 // One level of the parent-subtree walk: children y of frontier node x are the
 // live out-neighbours whose parent is x.
-__global__ void subtree_level(GraphView G, const uint64_t *gp, int32_t *mark, int32_t epoch, const int32_t *cur,
-                              int32_t ncur, int32_t *nxt, int32_t *marked, uint32_t *rctrl)
+__device__ __forceinline__ void claim_child(int32_t y, int32_t x, const uint64_t *gp, int32_t *mark, int32_t epoch,
+                                            int32_t *nxt, int32_t *marked, uint32_t *rctrl)
+{
+    if ((uint32_t)ld_gp(gp + y) != (uint32_t)x) return;
+    if (atomicMax(mark + y, epoch) == epoch) return;
+    nxt[atomicAdd(rctrl + RC_NEXT, 1u)] = y;
+    marked[atomicAdd(rctrl + RC_MARKED, 1u)] = y;
+}
+
+__global__ void subtree_level(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch,
+                              const int32_t *cur, int32_t ncur, int32_t *nxt, int32_t *marked, uint32_t *rctrl)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ncur) return;
     const int32_t x = cur[i];
-    for (int pass = 0; pass < 2; ++pass) {
-        const int32_t *off = pass ? G.doff : G.off;
-        const int32_t *col = pass ? G.dcol : G.col;
-        if (!off) continue;
-        for (int32_t j = off[x]; j < off[x + 1]; ++j) {
-            const int32_t y = col[j];
-            if (y < 0) continue;
-            if ((uint32_t)ld_gp(gp + y) != (uint32_t)x) continue;
-            if (atomicMax(mark + y, epoch) == epoch) continue;
-            nxt[atomicAdd(rctrl + RC_NEXT, 1u)] = y;
-            marked[atomicAdd(rctrl + RC_MARKED, 1u)] = y;
-        }
+    for (int32_t j = G.off[x]; j < G.off[x + 1]; ++j) {
+        const int32_t y = G.col[j];
+        if (y >= 0) claim_child(y, x, gp, mark, epoch, nxt, marked, rctrl);
+    }
+    for (uint64_t link = L.fhead[x]; !link_end(link);) {
+        const int32_t e = (int32_t)(link >> 32);
+        link = L.fnext[e];
+        if (L.del[e] == INT32_MAX) claim_child(L.v[e], x, gp, mark, epoch, nxt, marked, rctrl);
     }
 }
 
@@ -115,7 +193,7 @@ struct RepairPush {
 
 // Invalidated node x: best live in-edge from a node outside the invalidated
 // set with a finite value; x's word is overwritten (INF if none).
-__global__ void repull(GraphView R, const int32_t *h, uint64_t *gp, const int32_t *mark, int32_t epoch,
+__global__ void repull(GraphView R, InsLog L, const int32_t *h, uint64_t *gp, const int32_t *mark, int32_t epoch,
                        const int32_t *marked, int32_t n, RepairPush P)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,22 +202,21 @@ __global__ void repull(GraphView R, const int32_t *h, uint64_t *gp, const int32_
     if (i < n) {
         x = marked[i];
         const uint32_t hx = (uint32_t)h[x];
-        for (int pass = 0; pass < 2; ++pass) {
-            const int32_t *off = pass ? R.doff : R.off;
-            const int32_t *col = pass ? R.dcol : R.col;
-            const uint32_t *w = pass ? R.dw : R.w;
-            if (!off) continue;
-            for (int32_t j = off[x]; j < off[x + 1]; ++j) {
-                const int32_t p = col[j];
-                if (p < 0 || p == x || mark[p] == epoch) continue;
-                const uint32_t fp = (uint32_t)(ld_gp(gp + p) >> 32);
-                if (fp == DDP_INF_F) continue;
-                const uint32_t cand = (fp - (uint32_t)h[p]) + w[j] + hx;  // unsigned, as GA_v2.cu:137
-                if (cand < DDP_INF_F && (cand < best || (cand == best && (uint32_t)p < bp))) {
-                    best = cand;
-                    bp = (uint32_t)p;
-                }
+        auto consider = [&](int32_t p, uint32_t w) {
+            if (p < 0 || p == x || mark[p] == epoch) return;
+            const uint32_t fp = (uint32_t)(ld_gp(gp + p) >> 32);
+            if (fp == DDP_INF_F) return;
+            const uint32_t cand = (fp - (uint32_t)h[p]) + w + hx;  // unsigned, as GA_v2.cu:137
+            if (cand < DDP_INF_F && (cand < best || (cand == best && (uint32_t)p < bp))) {
+                best = cand;
+                bp = (uint32_t)p;
             }
+        };
+        for (int32_t j = R.off[x]; j < R.off[x + 1]; ++j) consider(R.col[j], R.w[j]);
+        for (uint64_t link = L.rhead[x]; !link_end(link);) {
+            const int32_t e = (int32_t)(link >> 32);
+            link = L.rnext[e];
+            if (L.del[e] == INT32_MAX) consider(L.u[e], L.w[e]);
         }
         gp[x] = best == 0xFFFFFFFFu ? DDP_GP_INF : (((uint64_t)best << 32) | bp);
         if (best != 0xFFFFFFFFu) atomicAdd(P.rctrl + RC_REPULLED, 1u);
@@ -196,26 +273,18 @@ int dev_upload(ddp_ctx *ctx, T **dst, size_t *cap, const T *src, size_t n)
 struct ddp_dastar {
     ddp_ctx *ctx = nullptr;
     int device = 0;
-    int32_t N = 0, s = 0, t = 0;
-    // host copy of the current graph version
-    std::vector<int32_t> off, col, roff, rcol, reid;  // off/roff have N+1 entries
-    std::vector<uint32_t> w, rw;
-    struct DEdge {
-        int32_t u, v;
-        uint32_t w;
-    };
-    std::vector<DEdge> delta;  // live inserts in arrival order
-    std::vector<int32_t> doff, dcol, didx, droff, drcol;
-    std::vector<uint32_t> dw, drw;
-    // device
-    ddp_graph *g = nullptr;  // base rows (tombstones scattered in), transposed rows too
+    int32_t N = 0, E = 0, s = 0, t = 0;
+    int64_t live_base = 0, live_log = 0;  // live edges (ddp_dastar_num_edges)
+    // device graph: base rows with tombstones (d->g, transposed rows too) ...
+    ddp_graph *g = nullptr;
+    int32_t *d_reid = nullptr;  // forward edge id of each transposed entry
+    // ... and the log of inserted edges
+    InsLog L{};
+    int32_t log_cap = 0, log_used = 0;
     int32_t *d_h = nullptr;
-    int32_t *d_doff = nullptr, *d_dcol = nullptr, *d_droff = nullptr, *d_drcol = nullptr;
-    uint32_t *d_dw = nullptr, *d_drw = nullptr;
-    size_t c_doff = 0, c_dcol = 0, c_droff = 0, c_drcol = 0, c_dw = 0, c_drw = 0;
     int32_t *d_mark = nullptr, *d_front = nullptr, *d_marked = nullptr;
     uint32_t *d_rctrl = nullptr;
-    int32_t *d_work = nullptr;  // tombstone positions / pairs / inserts
+    int32_t *d_work = nullptr;  // per-batch lines: delete pairs, insert triples, list groups
     size_t c_work = 0;
     int32_t epoch = 0;
     // search
@@ -226,10 +295,21 @@ struct ddp_dastar {
     uint32_t F = DDP_INF_F;
 };
 
+static void log_free(InsLog &L)
+{
+    void *ptrs[] = {L.u, L.v, L.w, L.del, L.fnext, L.rnext};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    L.u = L.v = L.del = nullptr;
+    L.w = nullptr;
+    L.fnext = L.rnext = nullptr;
+}
+
 static void dastar_free(ddp_dastar *d)
 {
     ddp_ga_free(&d->S);
-    void *ptrs[] = {d->d_h, d->d_doff, d->d_dcol, d->d_droff, d->d_drcol, d->d_dw, d->d_drw,
+    log_free(d->L);
+    void *ptrs[] = {d->L.fhead, d->L.rhead, d->L.ftail, d->L.rtail, d->d_reid, d->d_h,
                     d->d_mark, d->d_front, d->d_marked, d->d_rctrl, d->d_work};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -244,14 +324,19 @@ extern "C" void ddp_dastar_close(ddp_dastar *d)
     delete d;
 }
 
+// Inserted edges are visible to the search through the list fields of the
+// view: every link carries version 0 and a live entry's delete word is
+// INT32_MAX, so "version <= ver < delete word" with ver = INT32_MAX - 1 reads
+// exactly the live entries.
 static GraphView fwd_view(const ddp_dastar *d)
 {
     GraphView v = ddp_view(d->g);
-    if (!d->delta.empty()) {
-        v.doff = d->d_doff;
-        v.dcol = d->d_dcol;
-        v.dw = d->d_dw;
-    }
+    v.ver = INT32_MAX - 1;
+    v.lhead = d->L.fhead;
+    v.lnext = d->L.fnext;
+    v.lv = d->L.v;
+    v.lw = d->L.w;
+    v.ldel = d->L.del;
     return v;
 }
 
@@ -262,55 +347,55 @@ static GraphView rev_view(const ddp_dastar *d)
     v.col = d->g->rcol;
     v.w = d->g->rw;
     v.N = d->N;
-    if (!d->delta.empty()) {
-        v.doff = d->d_droff;
-        v.dcol = d->d_drcol;
-        v.dw = d->d_drw;
-    }
     return v;
 }
 
-// Rebuilds the delta rows (forward: grouped by source, arrival order kept;
-// reverse: grouped by target) from the live inserts and uploads them.
-static int rebuild_delta(ddp_dastar *d)
+// Makes room for `need` log entries, keeping the ones in use (device copy).
+static int log_reserve(ddp_dastar *d, int32_t need)
 {
-    const int32_t N = d->N;
-    const size_t D = d->delta.size();
-    d->doff.assign(N + 1, 0);
-    d->droff.assign(N + 1, 0);
-    for (const auto &e : d->delta) {
-        d->doff[e.u + 1]++;
-        d->droff[e.v + 1]++;
-    }
-    for (int32_t i = 0; i < N; ++i) {
-        d->doff[i + 1] += d->doff[i];
-        d->droff[i + 1] += d->droff[i];
-    }
-    d->dcol.assign(D, 0);
-    d->dw.assign(D, 0);
-    d->didx.assign(D, 0);
-    d->drcol.assign(D, 0);
-    d->drw.assign(D, 0);
-    std::vector<int32_t> f(d->doff.begin(), d->doff.end() - 1), r(d->droff.begin(), d->droff.end() - 1);
-    for (size_t k = 0; k < D; ++k) {
-        const auto &e = d->delta[k];
-        const int32_t p = f[e.u]++;
-        d->dcol[p] = e.v;
-        d->dw[p] = e.w;
-        d->didx[p] = (int32_t)k;
-        const int32_t q = r[e.v]++;
-        d->drcol[q] = e.u;
-        d->drw[q] = e.w;
-    }
+    if (need <= d->log_cap) return DDP_OK;
     ddp_ctx *ctx = d->ctx;
-    int rc;
-    if ((rc = dev_upload(ctx, &d->d_doff, &d->c_doff, d->doff.data(), (size_t)N + 1)) ||
-        (rc = dev_upload(ctx, &d->d_dcol, &d->c_dcol, d->dcol.data(), D)) ||
-        (rc = dev_upload(ctx, &d->d_dw, &d->c_dw, d->dw.data(), D)) ||
-        (rc = dev_upload(ctx, &d->d_droff, &d->c_droff, d->droff.data(), (size_t)N + 1)) ||
-        (rc = dev_upload(ctx, &d->d_drcol, &d->c_drcol, d->drcol.data(), D)) ||
-        (rc = dev_upload(ctx, &d->d_drw, &d->c_drw, d->drw.data(), D)))
-        return rc;
+    const int32_t cap = (int32_t)std::min<int64_t>(std::max<int64_t>(need, 2LL * d->log_cap + 65536), INT32_MAX);
+    DDP_CHECK(need <= cap, ctx, DDP_ERR_NOMEM, "DA*: inserted-edge log would exceed %d entries", INT32_MAX);
+    InsLog n = d->L;
+    hipError_t e = hipSuccess;
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    };
+    n.u = n.v = n.del = nullptr;
+    n.w = nullptr;
+    n.fnext = n.rnext = nullptr;
+    al((void **)&n.u, (size_t)cap * 4);
+    al((void **)&n.v, (size_t)cap * 4);
+    al((void **)&n.w, (size_t)cap * 4);
+    al((void **)&n.del, (size_t)cap * 4);
+    al((void **)&n.fnext, (size_t)cap * 8);
+    al((void **)&n.rnext, (size_t)cap * 8);
+    const size_t used = (size_t)d->log_used;
+    if (e == hipSuccess && used) {
+        auto cp = [&](void *dst, const void *src, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+        };
+        cp(n.u, d->L.u, used * 4);
+        cp(n.v, d->L.v, used * 4);
+        cp(n.w, d->L.w, used * 4);
+        cp(n.del, d->L.del, used * 4);
+        cp(n.fnext, d->L.fnext, used * 8);
+        cp(n.rnext, d->L.rnext, used * 8);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        log_free(n);
+        return ddp_hip_err(ctx, e, "DA* log growth", __FILE__, __LINE__);
+    }
+    log_free(d->L);
+    d->L.u = n.u;
+    d->L.v = n.v;
+    d->L.w = n.w;
+    d->L.del = n.del;
+    d->L.fnext = n.fnext;
+    d->L.rnext = n.rnext;
+    d->log_cap = cap;
     return DDP_OK;
 }
 
@@ -354,28 +439,27 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
         delete d;
         return rc;
     }
-    // host copy + stable transpose (DA_star/genOpCSR.cpp order) with forward edge ids
-    d->off.assign(off, off + N);
-    d->off.push_back(E);
-    d->col.assign(col, col + E);
-    d->w.assign(w, w + E);
-    d->roff.assign(N + 1, 0);
+    d->E = E;
+    // stable transpose (DA_star/genOpCSR.cpp order) with forward edge ids
+    std::vector<int32_t> roff(N + 1, 0), rcol, reid;
+    std::vector<uint32_t> rw;
     for (int32_t j = 0; j < E; ++j)
-        if (col[j] >= 0) d->roff[col[j] + 1]++;
-    for (int32_t i = 0; i < N; ++i) d->roff[i + 1] += d->roff[i];
-    const int32_t RE = d->roff[N];
-    d->rcol.assign(RE, 0);
-    d->rw.assign(RE, 0);
-    d->reid.assign(RE, 0);
+        if (col[j] >= 0) roff[col[j] + 1]++;
+    for (int32_t i = 0; i < N; ++i) roff[i + 1] += roff[i];
+    const int32_t RE = roff[N];
+    d->live_base = RE;
+    rcol.assign(RE, 0);
+    rw.assign(RE, 0);
+    reid.assign(RE, 0);
     {
-        std::vector<int32_t> f(d->roff.begin(), d->roff.end() - 1);
+        std::vector<int32_t> f(roff.begin(), roff.end() - 1);
         for (int32_t u = 0; u < N; ++u)
-            for (int32_t j = d->off[u]; j < d->off[u + 1]; ++j) {
+            for (int32_t j = off[u]; j < (u + 1 < N ? off[u + 1] : E); ++j) {
                 if (col[j] < 0) continue;
                 const int32_t p = f[col[j]]++;
-                d->rcol[p] = u;
-                d->rw[p] = w[j];
-                d->reid[p] = j;
+                rcol[p] = u;
+                rw[p] = w[j];
+                reid[p] = j;
             }
     }
     hipError_t e = hipSuccess;
@@ -383,10 +467,25 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
         if (e == hipSuccess) e = hipMalloc(dst, std::max<size_t>(bytes, 16));
         if (e == hipSuccess && bytes) e = hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
     };
-    up((void **)&d->g->roff, d->roff.data(), (size_t)(N + 1) * 4);
-    up((void **)&d->g->rcol, d->rcol.data(), (size_t)RE * 4);
-    up((void **)&d->g->rw, d->rw.data(), (size_t)RE * 4);
+    up((void **)&d->g->roff, roff.data(), (size_t)(N + 1) * 4);
+    up((void **)&d->g->rcol, rcol.data(), (size_t)RE * 4);
+    up((void **)&d->g->rw, rw.data(), (size_t)RE * 4);
+    up((void **)&d->d_reid, reid.data(), (size_t)RE * 4);
     d->g->RE = RE;
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    };
+    al((void **)&d->L.fhead, (size_t)N * 8);
+    al((void **)&d->L.rhead, (size_t)N * 8);
+    al((void **)&d->L.ftail, (size_t)N * 4);
+    al((void **)&d->L.rtail, (size_t)N * 4);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_u64, dim3(1024), dim3(256), 0, ctx->stream, d->L.fhead, (int64_t)N, ~0ull);
+        hipLaunchKernelGGL(fill_u64, dim3(1024), dim3(256), 0, ctx->stream, d->L.rhead, (int64_t)N, ~0ull);
+        hipLaunchKernelGGL(fill_i32, dim3(1024), dim3(256), 0, ctx->stream, d->L.ftail, (int64_t)N, -1);
+        hipLaunchKernelGGL(fill_i32, dim3(1024), dim3(256), 0, ctx->stream, d->L.rtail, (int64_t)N, -1);
+        e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMalloc((void **)&d->d_h, (size_t)N * 4);
     int32_t *h_tmp = nullptr;
     if (e == hipSuccess && h) {
@@ -413,7 +512,7 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
     d->hv.scale = 1;
     d->hv.h = d->d_h;
     if ((rc = ddp_ga_plan(ctx, d->g, opts, &d->P, 2 * (int64_t)N + 65536)) ||
-        (rc = ddp_ga_alloc_owned(ctx, N, d->P, &d->S))) {
+        (rc = ddp_ga_alloc_owned(ctx, N, d->P, &d->S)) || (rc = log_reserve(d, 1))) {
         dastar_free(d);
         delete d;
         return rc;
@@ -445,80 +544,80 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
         DDP_CHECK(l.u >= 0 && l.u < N && l.v >= 0 && l.v < N, ctx, DDP_ERR_ARG, "update line %d: edge %d->%d outside [0,%d)",
                   i, l.u, l.v, N);
     }
-    // ---- 1. host: match deletes (first live instance, base row then inserts)
-    std::vector<int32_t> ftomb, rtomb, pairs, ins;
-    std::vector<char> dead_delta(d->delta.size(), 0);
+    // ---- 1. host, O(batch): split the lines; group new inserts by source and by target
+    std::vector<int32_t> pairs, ins;
+    std::vector<std::pair<int32_t, int32_t>> fg, rg;  // (key, log entry), insert order kept
     for (int32_t i = 0; i < n; ++i) {
         const ddp_update_t &l = lines[i];
-        if (l.flag == 1) {
-            ++S.inserts;
-            continue;
-        }
-        ++S.deletes;
-        bool found = false;
-        for (int32_t j = d->off[l.u]; j < d->off[l.u + 1] && !found; ++j)
-            if (d->col[j] == l.v) {
-                d->col[j] = -1;
-                ftomb.push_back(j);
-                for (int32_t r = d->roff[l.v]; r < d->roff[l.v + 1]; ++r)
-                    if (d->reid[r] == j) {
-                        d->rcol[r] = -1;
-                        rtomb.push_back(r);
-                        break;
-                    }
-                found = true;
-            }
-        if (!found && !d->doff.empty())
-            for (int32_t p = d->doff[l.u]; p < d->doff[l.u + 1] && !found; ++p) {
-                const int32_t k = d->didx[p];
-                if (!dead_delta[k] && d->delta[k].v == l.v) {
-                    dead_delta[k] = 1;
-                    found = true;
-                }
-            }
-        if (found) {
-            ++S.deletes_effective;
+        if (l.flag == 0) {
+            ++S.deletes;
             pairs.push_back(l.u);
             pairs.push_back(l.v);
+        } else {
+            const int32_t x = d->log_used + S.inserts;
+            ++S.inserts;
+            ins.push_back(l.u);
+            ins.push_back(l.v);
+            ins.push_back((int32_t)l.w);
+            fg.emplace_back(l.u, x);
+            rg.emplace_back(l.v, x);
         }
     }
-    // ---- 2. host: drop deleted inserts, append the new ones, rebuild delta rows
-    {
-        std::vector<ddp_dastar::DEdge> nd;
-        nd.reserve(d->delta.size() + S.inserts);
-        for (size_t k = 0; k < d->delta.size(); ++k)
-            if (!dead_delta[k]) nd.push_back(d->delta[k]);
-        for (int32_t i = 0; i < n; ++i)
-            if (lines[i].flag == 1) {
-                nd.push_back({lines[i].u, lines[i].v, lines[i].w});
-                ins.push_back(lines[i].u);
-                ins.push_back(lines[i].v);
-                ins.push_back((int32_t)lines[i].w);
-            }
-        d->delta.swap(nd);
-    }
-    int rc = rebuild_delta(d);
+    auto by_key = [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
+        return a.first < b.first;
+    };
+    std::stable_sort(fg.begin(), fg.end(), by_key);
+    std::stable_sort(rg.begin(), rg.end(), by_key);
+    int rc = log_reserve(d, d->log_used + S.inserts);
     if (rc) return rc;
-    // device work list: [ftomb | rtomb | pairs | ins]
+    // device work list: [pairs | ins | fg | rg]
     std::vector<int32_t> work;
-    work.reserve(ftomb.size() + rtomb.size() + pairs.size() + ins.size());
-    work.insert(work.end(), ftomb.begin(), ftomb.end());
-    work.insert(work.end(), rtomb.begin(), rtomb.end());
+    work.reserve(pairs.size() + ins.size() + 4 * fg.size());
     work.insert(work.end(), pairs.begin(), pairs.end());
     work.insert(work.end(), ins.begin(), ins.end());
+    for (const auto &x : fg) work.insert(work.end(), {x.first, x.second});
+    for (const auto &x : rg) work.insert(work.end(), {x.first, x.second});
     if ((rc = dev_upload(ctx, &d->d_work, &d->c_work, work.data(), work.size()))) return rc;
-    const int32_t *w_ft = d->d_work, *w_rt = w_ft + ftomb.size(), *w_pairs = w_rt + rtomb.size(),
-                  *w_ins = w_pairs + pairs.size();
+    const int32_t *w_pairs = d->d_work, *w_ins = w_pairs + pairs.size(), *w_fg = w_ins + ins.size(),
+                  *w_rg = w_fg + 2 * fg.size();
     const int T = 256;
     auto grid = [&](size_t k) { return dim3((unsigned)std::max<size_t>(1, (k + T - 1) / T)); };
-    if (!ftomb.empty())
-        hipLaunchKernelGGL(scatter_tombstones, grid(ftomb.size()), dim3(T), 0, ctx->stream, d->g->col, w_ft,
-                           (int32_t)ftomb.size());
-    if (!rtomb.empty())
-        hipLaunchKernelGGL(scatter_tombstones, grid(rtomb.size()), dim3(T), 0, ctx->stream, d->g->rcol, w_rt,
-                           (int32_t)rtomb.size());
+    // ---- 2. device: deletes (they never see this batch's inserts), then the appends
+    d->epoch += 1;
+    uint32_t rctrl[RC_WORDS] = {0};
+    DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl, 0, RC_WORDS * 4, ctx->stream));
+    const int32_t npairs = (int32_t)(pairs.size() / 2), nins = S.inserts;
+    if (npairs)
+        hipLaunchKernelGGL(apply_deletes, grid(npairs), dim3(T), 0, ctx->stream, w_pairs, npairs, fwd_view(d),
+                           d->g->col, d->g->rcol, (const int32_t *)d->g->roff, (const int32_t *)d->d_reid, d->L,
+                           (const uint64_t *)d->S.gp, d->d_mark, d->epoch, d->d_front, d->d_marked, d->d_rctrl);
+    if (nins) {
+        std::vector<int32_t> nu(nins), nv(nins), nd(nins, INT32_MAX);
+        std::vector<uint32_t> nw(nins);
+        for (int32_t k = 0; k < nins; ++k) {
+            nu[k] = ins[3 * k];
+            nv[k] = ins[3 * k + 1];
+            nw[k] = (uint32_t)ins[3 * k + 2];
+        }
+        const size_t at = (size_t)d->log_used;
+        DDP_HIP(ctx, hipMemcpyAsync(d->L.u + at, nu.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(d->L.v + at, nv.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(d->L.w + at, nw.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(d->L.del + at, nd.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(log_append, grid(nins), dim3(T), 0, ctx->stream, w_fg, nins, d->L.fhead, d->L.fnext,
+                           d->L.ftail);
+        hipLaunchKernelGGL(log_append, grid(nins), dim3(T), 0, ctx->stream, w_rg, nins, d->L.rhead, d->L.rnext,
+                           d->L.rtail);
+        // the host staging vectors must outlive the copies
+        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     DDP_HIP(ctx, hipGetLastError());
+    DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    d->log_used += nins;
+    S.deletes_effective = (int32_t)(rctrl[RC_DEL_BASE] + rctrl[RC_DEL_LOG]);
+    d->live_base -= rctrl[RC_DEL_BASE];
+    d->live_log += nins - (int64_t)rctrl[RC_DEL_LOG];
     S.host_ms = ms_since(t0);
     int32_t iters = 0;
     float kms = 0;
@@ -527,25 +626,16 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     } else {
         DDP_CHECK(mode == DDP_DASTAR_REPAIR, ctx, DDP_ERR_ARG, "unknown DA* mode %d", mode);
         const auto t1 = clk::now();
-        // ---- 3. invalidate parent subtrees of deleted tree edges
-        d->epoch += 1;
-        uint32_t rctrl[RC_WORDS] = {0};
-        DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl, 0, RC_WORDS * 4, ctx->stream));
+        // ---- 3. invalidate parent subtrees of deleted tree edges (roots seeded by apply_deletes)
         if ((rc = ddp_ga_prepare_resume(ctx, d->S))) return rc;
         const GraphView F = fwd_view(d), R = rev_view(d);
         int32_t *cur = d->d_front, *nxt = d->d_front + N;
-        const int32_t npairs = (int32_t)(pairs.size() / 2);
-        if (npairs)
-            hipLaunchKernelGGL(seed_roots, grid(npairs), dim3(T), 0, ctx->stream, w_pairs, npairs, d->S.gp, d->d_mark,
-                               d->epoch, cur, d->d_marked, d->d_rctrl);
-        DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
-        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         int32_t ncur = (int32_t)rctrl[RC_NEXT];
         S.roots = (int32_t)rctrl[RC_ROOTS];
         while (ncur > 0) {
             DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl + RC_NEXT, 0, 4, ctx->stream));
-            hipLaunchKernelGGL(subtree_level, grid(ncur), dim3(T), 0, ctx->stream, F, d->S.gp, d->d_mark, d->epoch,
-                               cur, ncur, nxt, d->d_marked, d->d_rctrl);
+            hipLaunchKernelGGL(subtree_level, grid(ncur), dim3(T), 0, ctx->stream, F, d->L, d->S.gp, d->d_mark,
+                               d->epoch, cur, ncur, nxt, d->d_marked, d->d_rctrl);
             DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
             DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
             ncur = (int32_t)rctrl[RC_NEXT];
@@ -555,9 +645,8 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
         // ---- 4. re-pull invalidated nodes, 5. relax inserted edges
         RepairPush P{d->S.pbuf + d->P.pcap, d->S.ctrl + 2 * GA_SLOT_WORDS, d->P.pcap, d->d_rctrl, d->t};
         if (S.invalidated)
-            hipLaunchKernelGGL(repull, grid(S.invalidated), dim3(T), 0, ctx->stream, R, d->d_h, d->S.gp, d->d_mark,
-                               d->epoch, d->d_marked, S.invalidated, P);
-        const int32_t nins = (int32_t)(ins.size() / 3);
+            hipLaunchKernelGGL(repull, grid(S.invalidated), dim3(T), 0, ctx->stream, R, d->L, d->d_h, d->S.gp,
+                               d->d_mark, d->epoch, d->d_marked, S.invalidated, P);
         if (nins)
             hipLaunchKernelGGL(relax_inserts, grid(nins), dim3(T), 0, ctx->stream, w_ins, nins, d->d_h, d->S.gp, P);
         DDP_HIP(ctx, hipGetLastError());
@@ -604,30 +693,58 @@ extern "C" int ddp_dastar_path(ddp_dastar *d, int32_t *path, int32_t cap, int32_
 extern "C" int ddp_dastar_num_edges(const ddp_dastar *d, int32_t *E)
 {
     if (!d || !E) return DDP_ERR_ARG;
-    int64_t live = (int64_t)d->delta.size();
-    for (int32_t c : d->col) live += c >= 0;
-    *E = (int32_t)live;
+    *E = (int32_t)(d->live_base + d->live_log);
     return DDP_OK;
 }
 
+// The current graph version as a CSR in the reference layout: per node its
+// surviving base edges in row order, then its live inserted edges in insert
+// order (GA_merge.cu's merged row).  Reads the device state back.
 extern "C" int ddp_dastar_export(const ddp_dastar *d, int32_t *off, int32_t *col, uint32_t *w)
 {
     if (!d || !off || !col || !w) return DDP_ERR_ARG;
+    ddp_ctx *ctx = d->ctx;
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    const int32_t N = d->N, E = d->E, L = d->log_used;
+    std::vector<int32_t> boff(N + 1), bcol(E), lu(L), lv(L), ldel(L);
+    std::vector<uint32_t> bw(E), lw(L);
+    DDP_HIP(ctx, hipMemcpyAsync(boff.data(), d->g->off, (size_t)(N + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (E) {
+        DDP_HIP(ctx, hipMemcpyAsync(bcol.data(), d->g->col, (size_t)E * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(bw.data(), d->g->w, (size_t)E * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (L) {
+        DDP_HIP(ctx, hipMemcpyAsync(lu.data(), d->L.u, (size_t)L * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(lv.data(), d->L.v, (size_t)L * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(lw.data(), d->L.w, (size_t)L * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(ldel.data(), d->L.del, (size_t)L * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // live inserts grouped by source, entry (= insert) order kept
+    std::vector<int32_t> loff(N + 1, 0), lidx;
+    for (int32_t x = 0; x < L; ++x)
+        if (ldel[x] == INT32_MAX) loff[lu[x] + 1]++;
+    for (int32_t i = 0; i < N; ++i) loff[i + 1] += loff[i];
+    lidx.resize(loff[N]);
+    {
+        std::vector<int32_t> f(loff.begin(), loff.end() - 1);
+        for (int32_t x = 0; x < L; ++x)
+            if (ldel[x] == INT32_MAX) lidx[f[lu[x]]++] = x;
+    }
     int32_t k = 0;
-    for (int32_t u = 0; u < d->N; ++u) {
+    for (int32_t u = 0; u < N; ++u) {
         off[u] = k;
-        for (int32_t j = d->off[u]; j < d->off[u + 1]; ++j)
-            if (d->col[j] >= 0) {
-                col[k] = d->col[j];
-                w[k] = d->w[j];
+        for (int32_t j = boff[u]; j < boff[u + 1]; ++j)
+            if (bcol[j] >= 0) {
+                col[k] = bcol[j];
+                w[k] = bw[j];
                 ++k;
             }
-        if (!d->doff.empty())
-            for (int32_t p = d->doff[u]; p < d->doff[u + 1]; ++p) {
-                col[k] = d->dcol[p];
-                w[k] = d->dw[p];
-                ++k;
-            }
+        for (int32_t p = loff[u]; p < loff[u + 1]; ++p) {
+            col[k] = lv[lidx[p]];
+            w[k] = lw[lidx[p]];
+            ++k;
+        }
     }
     return DDP_OK;
 }
