@@ -237,7 +237,7 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
         while (nc * 2 <= total / 4) nc *= 2;
         b->ncap = (uint32_t)nc;
         b->fcap = (uint32_t)std::min<uint64_t>((total - nc) / 2, 0xFFFFFFFFull);
-        b->delta0 = std::max<uint32_t>(1, 2 * g->meanw);
+        b->delta0 = std::max<uint32_t>(1, g->meanw);
     }
     const size_t M = (size_t)max_queries;
     hipError_t e = hipSuccess;

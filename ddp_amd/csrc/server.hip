@@ -463,7 +463,7 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
         while (nc * 2 <= total / 4) nc *= 2;
         sv->ncap = (uint32_t)nc;
         sv->fcap = (uint32_t)std::min<uint64_t>((total - nc) / 2, 0xFFFFFFFFull);
-        sv->delta0 = std::max<uint32_t>(1, 2 * g->meanw);
+        sv->delta0 = std::max<uint32_t>(1, g->meanw);
     }
     sv->ring = (o && o->ring) ? (uint32_t)o->ring : 65536u;
     sv->log_cap = (o && o->max_inserts) ? o->max_inserts : (1 << 20);

@@ -149,8 +149,9 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
 // again after a later improvement, as GA*'s K queues also allow); it returns
 // the optimal cost for an admissible heuristic because an entry is only
 // dropped when it is stale or cannot lead to anything below F(t).  The window
-// width adapts: doubled when a phase fed the lanes fewer than 2B entries,
-// halved when it ran past 32B.
+// width adapts: doubled when a phase fed the lanes fewer than B entries,
+// halved when it ran past 8B (a sweep on the C4 batch: narrower windows
+// expand fewer nodes out of order, down to where rounds run short of work).
 // Overflow: a push that finds the near ring full goes to the far pile; one
 // that finds the far pile full too is dropped and flagged, and the next refill
 // rebuilds the far pile from the slot's touched log (every node whose word
@@ -329,8 +330,8 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             const uint32_t fmin = block_min_u32<B>(mymin, S.red);
             if (fmin >= F) break;  // nothing open can improve t (also: far pile empty)
             if (lane == 0) {
-                if (S.phase_n < 2 * B) S.delta = sat_add_u32(S.delta, S.delta);
-                else if (S.phase_n > 32 * B && S.delta > 1) S.delta >>= 1;
+                if (S.phase_n < B) S.delta = sat_add_u32(S.delta, S.delta);
+                else if (S.phase_n > 8 * B && S.delta > 1) S.delta >>= 1;
                 S.thr = sat_add_u32(fmin, S.delta);
                 S.phase_n = 0;
                 S.far_n2 = 0;
