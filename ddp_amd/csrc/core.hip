@@ -438,9 +438,6 @@ GraphView ddp_view(const ddp_graph *g)
     v.off = g->off;
     v.col = g->col;
     v.w = g->w;
-    v.doff = nullptr;
-    v.dcol = nullptr;
-    v.dw = nullptr;
     v.N = g->N;
     return v;
 }

@@ -91,9 +91,6 @@ struct GraphView {
     const int32_t *__restrict__ off;   // N+1
     const int32_t *__restrict__ col;
     const uint32_t *__restrict__ w;
-    const int32_t *__restrict__ doff;  // delta rows (N+1) or null
-    const int32_t *__restrict__ dcol;
-    const uint32_t *__restrict__ dw;
     int32_t N;
     // Versioned graph (serving): a base edge p is visible iff dver[p] > ver;
     // inserted edges hang off lhead[u] as a list ordered oldest to newest whose

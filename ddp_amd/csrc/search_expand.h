@@ -16,13 +16,29 @@
 template <int B>
 struct ExpandLds {
     int32_t incl[B];   // inclusive prefix of degrees
-    int32_t rowb[B];   // base row start
-    int32_t based[B];  // base row length
-    int32_t drow[B];   // delta row start
+    int32_t rowb[B];   // row start
     uint32_t gu[B];    // g(u) = f(u) - h(u), unsigned as GA_v2.cu:137
     int32_t node[B];
     int32_t tmp[16];
 };
+
+// What expanding u needs besides u itself: its row and g(u).
+struct RowSpan {
+    int32_t b = 0, deg = 0;
+    uint32_t gu = 0;
+};
+
+template <int HK>
+__device__ __forceinline__ RowSpan row_span(const GraphView &G, const HeurView &H, int32_t u, uint32_t fu)
+{
+    RowSpan r;
+    if (u >= 0) {
+        r.gu = fu - (uint32_t)heur_eval<HK>(H, u);
+        r.b = G.off[u];
+        r.deg = G.off[u + 1] - r.b;
+    }
+    return r;
+}
 
 struct ExpandCounters {
     uint64_t expanded = 0, edges = 0, improved = 0;
@@ -41,33 +57,22 @@ constexpr int expand_unroll()
 //   void target(uint32_t f)                    -- t improved to f (t is never queued)
 //   void touch(int32_t c, uint64_t prev)       -- c's word was prev before the update
 //   void operator()(bool has, uint64_t key)    -- block-collective: all lanes call it
+// Each lane brings u (or -1) and its row span (row_span, or the same values
+// loaded earlier by the caller).
 template <int B, int HK, class Push>
-__device__ __forceinline__ void expand_block(const GraphView &G, const HeurView &H, uint64_t *gp,
-                                             int32_t t, int32_t u, uint32_t fu, ExpandLds<B> &L,
-                                             ExpandCounters &C, Push &push)
+__device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
+                                            int32_t u, const RowSpan &rs, ExpandLds<B> &L, ExpandCounters &C,
+                                            Push &push)
 {
     constexpr int R = expand_unroll<B>();
     const int lane = threadIdx.x;
-    int32_t deg = 0, b = 0, bd = 0, db = 0;
-    uint32_t gu = 0;
-    if (u >= 0) {
-        const int32_t hu = heur_eval<HK>(H, u);
-        gu = fu - (uint32_t)hu;
-        b = G.off[u];
-        bd = G.off[u + 1] - b;
-        deg = bd;
-        if (G.doff) {
-            db = G.doff[u];
-            deg += G.doff[u + 1] - db;
-        }
-        C.expanded++;
-    }
+    const int32_t deg = u >= 0 ? rs.deg : 0;
+    const uint32_t gu = rs.gu;
+    if (u >= 0) C.expanded++;
     int32_t total;
     const int32_t incl = block_incl_scan<B>(deg, L.tmp, total);
     L.incl[lane] = incl;
-    L.rowb[lane] = b;
-    L.based[lane] = bd;
-    L.drow[lane] = db;
+    L.rowb[lane] = rs.b;
     L.gu[lane] = gu;
     L.node[lane] = u;
     __syncthreads();
@@ -90,16 +95,10 @@ __device__ __forceinline__ void expand_block(const GraphView &G, const HeurView 
                 }
                 const int32_t k = e - (lo ? L.incl[lo - 1] : 0);
                 jj[r] = lo;
-                if (k < L.based[lo]) {
-                    const int32_t p = L.rowb[lo] + k;
-                    c[r] = G.col[p];
-                    wt[r] = G.w[p];
-                    if (G.dver && G.dver[p] <= G.ver) c[r] = -1;  // deleted at or before this version
-                } else {
-                    const int32_t p = L.drow[lo] + (k - L.based[lo]);
-                    c[r] = G.dcol[p];
-                    wt[r] = G.dw[p];
-                }
+                const int32_t p = L.rowb[lo] + k;
+                c[r] = G.col[p];
+                wt[r] = G.w[p];
+                if (G.dver && G.dver[p] <= G.ver) c[r] = -1;  // deleted at or before this version
                 C.edges++;
             }
         }
@@ -211,4 +210,12 @@ __device__ __forceinline__ void expand_block(const GraphView &G, const HeurView 
         }
     }
     __syncthreads();
+}
+
+template <int B, int HK, class Push>
+__device__ __forceinline__ void expand_block(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
+                                             int32_t u, uint32_t fu, ExpandLds<B> &L, ExpandCounters &C,
+                                             Push &push)
+{
+    expand_rows<B, HK>(G, H, gp, t, u, row_span<HK>(G, H, u, fu), L, C, push);
 }

@@ -369,14 +369,21 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
         // ---- take up to B near entries, one per lane
         const uint32_t n = tail - head < (uint32_t)B ? tail - head : (uint32_t)B;
         int32_t u = -1;
-        uint32_t fu = 0;
+        RowSpan rs;
         if ((uint32_t)lane < n) {
             const uint64_t key = P.near[(head + lane) & nmask];
             const uint32_t f = (uint32_t)(key >> 32);
             const int32_t node = (int32_t)(uint32_t)key;
-            if (f < F && (uint32_t)(ld_gp(gp + node) >> 32) == f) {
+            // the node's word, row and heuristic input in one round trip: the
+            // row is loaded before the entry is known to be current
+            const uint64_t word = ld_gp(gp + node);
+            const int32_t rb = G.off[node], re = G.off[node + 1];
+            const int32_t hn = heur_eval<HK>(hv, node);
+            if (f < F && (uint32_t)(word >> 32) == f) {
                 u = node;
-                fu = f;
+                rs.b = rb;
+                rs.deg = re - rb;
+                rs.gu = f - (uint32_t)hn;
             } else {
                 ++stale;
             }
@@ -387,7 +394,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             S.phase_n += n;
         }
         PilePush<B> push{P.near, far, nmask, P.ncap, P.fcap, &S, touched};
-        expand_block<B, HK>(G, hv, gp, t, u, fu, S.L, C, push);
+        expand_rows<B, HK>(G, hv, gp, t, u, rs, S.L, C, push);
         if (push.err) atomicOr(&S.err, push.err);
         block_sync_global();
         if (S.err) break;
