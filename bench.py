@@ -293,10 +293,14 @@ def run_c1(args):
     minw = int(hg.w.min())
     h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
     r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup)
-    rg, wall_g, kms_g = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup,
-                                    mode=ddp_amd.MODE_WORKGROUP)
-    if rg.cost != r.cost:
-        raise SystemExit(f"C1: workgroup mode {rg.cost} != grid mode {r.cost}")
+    other = {}
+    for name, mode in (("workgroup_mode", ddp_amd.MODE_WORKGROUP), ("pile_mode", ddp_amd.MODE_PILES)):
+        ro, wall_o, kms_o = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup, mode=mode)
+        if ro.cost != r.cost:
+            raise SystemExit(f"C1: {name} {ro.cost} != grid mode {r.cost}")
+        so = ro.stats
+        other[name] = {"wall_ms": round(wall_o, 3), "kernel_ms": round(kms_o, 3), "iterations": so["iterations"],
+                       "expansions": so["expansions"], "blocks": so["blocks"], "block": so["block"]}
     # CPU: the oracle's restatement of A_star_cpu.cpp on the same inputs
     reps, el = 0, 0.0
     while el < min(args.cpu_budget, 10.0) or reps == 0:
@@ -312,11 +316,8 @@ def run_c1(args):
                       "C1 single GA* query: random N=100,000, 6 out-edges, w in [1,100], seed 1, h = reverse-BFS "
                       "hops x min weight, K=1024 (grid mode)")
     res["config"].update({"s": s, "t": t, "K": st["K"], "block": st["block"], "cost": r.cost})
-    sg = rg.stats
+    res.update(other)
     res.update({"kernel_ms": round(kms, 3), "iterations": st["iterations"], "expansions": st["expansions"],
-                "workgroup_mode": {"wall_ms": round(wall_g, 3), "kernel_ms": round(kms_g, 3),
-                              "iterations": sg["iterations"], "expansions": sg["expansions"],
-                              "blocks": sg["blocks"], "block": sg["block"]},
                 "expansions_per_s": round(st["expansions"] / (kms * 1e-3), 1),
                 "edges_per_s": round(st["edges"] / (kms * 1e-3), 1),
                 "roofline": roofline(abytes, kms, "ga_grid_kernel"),
@@ -338,7 +339,8 @@ def run_c2(args):
     sweep = {}
     cost = None
     for name, K, mode in (("512-workgroup", 512, ddp_amd.MODE_WORKGROUP), ("1024-workgroup", 1024,
-                          ddp_amd.MODE_WORKGROUP), ("512-grid", 512, ddp_amd.MODE_GRID),
+                          ddp_amd.MODE_WORKGROUP), ("1024-piles", 1024, ddp_amd.MODE_PILES),
+                          ("512-grid", 512, ddp_amd.MODE_GRID),
                           ("2048-grid", 2048, ddp_amd.MODE_GRID), ("8192-grid", 8192, ddp_amd.MODE_GRID)):
         r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_MANHATTAN, K, args.steps, args.warmup, mode=mode)
         st = r.stats
@@ -346,7 +348,7 @@ def run_c2(args):
             raise SystemExit(f"C2 cost differs across K: {r.cost} vs {cost}")
         cost = r.cost
         abytes = 8 * st["expansions"] + 12 * st["edges"] + 8 * st["improvements"]  # closed-form h reads nothing
-        kern = "ga_grid_kernel" if mode == ddp_amd.MODE_GRID else "ga_wg_kernel"
+        kern = "ga_grid_kernel" if mode == ddp_amd.MODE_GRID else "ga_wg_kernel"  # one-workgroup modes
         sweep[name] = {"K": K, "wall_ms": round(wall, 3), "kernel_ms": round(kms, 3), "iterations": st["iterations"],
                        "expansions": st["expansions"], "edges": st["edges"], "block": st["block"],
                        "blocks": st["blocks"], "roofline": roofline(abytes, kms, kern)}

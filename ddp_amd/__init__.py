@@ -15,13 +15,13 @@ import numpy as np
 from . import _capi
 from ._capi import (DDPError, ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES, GRAPH_REORDER, GRAPH_REVERSE, H_ARRAY, H_EUCLID,
                     H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
-                    MODE_GRID, MODE_WORKGROUP, AstarOpts,
+                    MODE_GRID, MODE_PILES, MODE_WORKGROUP, AstarOpts,
                     AstarStats, BatchOpts, BatchStats, DeviceInfo, Heur, Update, as_i32, as_u32, ptr_f64, ptr_i32,
                     ptr_u32)
 
 __all__ = [
     "DDPError", "INF_COST", "H_ZERO", "H_ARRAY", "H_MANHATTAN", "H_EUCLID", "MODE_AUTO", "MODE_GRID",
-    "MODE_WORKGROUP", "ENGINE_AUTO", "ENGINE_HEAPS", "ENGINE_PILES", "Context", "HostGraph",
+    "MODE_WORKGROUP", "MODE_PILES", "ENGINE_AUTO", "ENGINE_HEAPS", "ENGINE_PILES", "Context", "HostGraph",
     "DeviceGraph", "AStarResult", "astar", "Batch", "astar_batch", "Updates", "GPU_Graph", "GPU_A_Star",
 ]
 

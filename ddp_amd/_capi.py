@@ -22,7 +22,7 @@ ERR_NAMES = {
 INF_COST = 2147483647
 
 H_ZERO, H_ARRAY, H_MANHATTAN, H_EUCLID = 0, 1, 2, 3
-MODE_AUTO, MODE_GRID, MODE_WORKGROUP = 0, 1, 2  # DDP_ASTAR_*
+MODE_AUTO, MODE_GRID, MODE_WORKGROUP, MODE_PILES = 0, 1, 2, 3  # DDP_ASTAR_*
 ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES = 0, 1, 2  # DDP_BATCH_*
 GRAPH_REVERSE = 1
 GRAPH_REORDER = 2

@@ -18,7 +18,9 @@ enum : int {
 
 struct GaGridPlan {
     int32_t K, B, nblocks, lcap, cap, pcap, maxit;
-    bool wg = false;  // workgroup mode: one workgroup, one queue per lane (ddp_astar only)
+    bool wg = false;     // one workgroup (ddp_astar only): lane heaps, or piles when `piles`
+    bool piles = false;
+    uint32_t ncap = 0, fcap = 0, delta0 = 1;  // pile mode: near ring, far buffers, first window
 };
 
 // Device buffers of one GA* search state.

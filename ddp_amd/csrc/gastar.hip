@@ -262,23 +262,30 @@ struct WgArgs {
     GraphView g;
     HeurView h;
     uint64_t *gp;
-    uint64_t *home;      // heap spill: entry i of queue q at home[i*B + q]
+    uint64_t *home;      // heap spill (entry i of queue q at home[i*B + q]), or the piles
     int32_t *touched;    // N: nodes whose word left INF
     uint32_t *ctrl;
     unsigned long long *stats;
     int32_t N, s, t, lcap, cap, maxit;
+    uint32_t ncap, fcap, delta0;  // pile mode: near ring, far buffers, first window
 };
 
-template <int B, int HK>
+// PILES = false: lane heaps (slot_search); true: near/far piles (slot_search_piles)
+template <int B, int HK, bool PILES>
 __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
 {
-    extern __shared__ uint64_t s_heap[];  // lcap * B
+    extern __shared__ uint64_t s_heap[];  // lcap * B (heaps)
     __shared__ SlotLds<B> S;
     const int lane = threadIdx.x;
     const LaneHeap H{s_heap + lane, a.home + lane, B, a.lcap, (int64_t)B, a.cap};
     ExpandCounters C;
     uint64_t stale = 0;
-    slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
+    if constexpr (PILES) {
+        const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap};
+        slot_search_piles<B, HK>(a.g, a.h, a.gp, a.touched, P, a.N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
+    } else {
+        slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
+    }
     if (lane == 0) {
         const uint32_t e = S.err;
         a.ctrl[GA_RES_IT] = (uint32_t)S.iters;
@@ -290,27 +297,30 @@ __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
 
 using WgFn = void (*)(WgArgs);
 
-template <int B>
+template <int B, bool PILES>
 WgFn pick_wg_b(int hk)
 {
     switch (hk) {
-    case DDP_H_ZERO: return ga_wg_kernel<B, DDP_H_ZERO>;
-    case DDP_H_ARRAY: return ga_wg_kernel<B, DDP_H_ARRAY>;
-    case DDP_H_MANHATTAN: return ga_wg_kernel<B, DDP_H_MANHATTAN>;
-    default: return ga_wg_kernel<B, DDP_H_EUCLID>;
+    case DDP_H_ZERO: return ga_wg_kernel<B, DDP_H_ZERO, PILES>;
+    case DDP_H_ARRAY: return ga_wg_kernel<B, DDP_H_ARRAY, PILES>;
+    case DDP_H_MANHATTAN: return ga_wg_kernel<B, DDP_H_MANHATTAN, PILES>;
+    default: return ga_wg_kernel<B, DDP_H_EUCLID, PILES>;
     }
 }
 
-WgFn pick_wg(int B, int hk)
+template <bool PILES>
+WgFn pick_wg_p(int B, int hk)
 {
     switch (B) {
-    case 64: return pick_wg_b<64>(hk);
-    case 128: return pick_wg_b<128>(hk);
-    case 256: return pick_wg_b<256>(hk);
-    case 512: return pick_wg_b<512>(hk);
-    default: return pick_wg_b<1024>(hk);
+    case 64: return pick_wg_b<64, PILES>(hk);
+    case 128: return pick_wg_b<128, PILES>(hk);
+    case 256: return pick_wg_b<256, PILES>(hk);
+    case 512: return pick_wg_b<512, PILES>(hk);
+    default: return pick_wg_b<1024, PILES>(hk);
     }
 }
+
+WgFn pick_wg(int B, int hk, bool piles) { return piles ? pick_wg_p<true>(B, hk) : pick_wg_p<false>(B, hk); }
 
 }  // namespace
 
@@ -321,23 +331,36 @@ int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaG
     int32_t K = (o && o->K) ? o->K : 1024;
     DDP_CHECK(K >= 1, ctx, DDP_ERR_ARG, "K must be >= 1 (got %d)", K);
     const int32_t mode = (o && allow_wg) ? o->mode : DDP_ASTAR_GRID;
-    DDP_CHECK(mode >= DDP_ASTAR_AUTO && mode <= DDP_ASTAR_WORKGROUP, ctx, DDP_ERR_ARG, "unknown mode %d", mode);
-    DDP_CHECK(!(mode == DDP_ASTAR_WORKGROUP && K > 1024), ctx, DDP_ERR_ARG,
-              "workgroup mode holds at most 1024 queues (K=%d)", K);
-    if (mode == DDP_ASTAR_WORKGROUP) {
+    DDP_CHECK(mode >= DDP_ASTAR_AUTO && mode <= DDP_ASTAR_PILES, ctx, DDP_ERR_ARG, "unknown mode %d", mode);
+    DDP_CHECK(!((mode == DDP_ASTAR_WORKGROUP || mode == DDP_ASTAR_PILES) && K > 1024), ctx, DDP_ERR_ARG,
+              "one-workgroup modes hold at most 1024 lanes (K=%d)", K);
+    if (mode == DDP_ASTAR_WORKGROUP || mode == DDP_ASTAR_PILES) {
+        const bool piles = mode == DDP_ASTAR_PILES;
         int32_t WB = 64;
         while (WB < K) WB *= 2;
         DDP_CHECK(B == 0 || B == WB, ctx, DDP_ERR_ARG,
-                  "workgroup mode runs K=%d queues on a block of %d lanes (got block %d)", K, WB, B);
-        const int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(1, std::min(32, 8192 / WB));
+                  "one-workgroup mode runs K=%d on a block of %d lanes (got block %d)", K, WB, B);
+        int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(1, std::min(32, 8192 / WB));
+        if (piles) lcap = 0;
         DDP_CHECK((int64_t)lcap * WB * 8 <= 96 * 1024, ctx, DDP_ERR_ARG,
                   "lds_cap %d x block %d exceeds the LDS budget", lcap, WB);
         int64_t cap = (o && o->heap_cap) ? o->heap_cap : (2 * (int64_t)g->N + WB - 1) / WB + 256;
-        cap = std::max<int64_t>(cap, lcap);
-        DDP_CHECK(cap < (1 << 30), ctx, DDP_ERR_ARG, "heap_cap too large");
+        if (piles && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries
+            cap = std::max<int64_t>(2048, (3 * (int64_t)g->N + WB - 1) / WB);
+        cap = std::max<int64_t>(cap, std::max(lcap, 16));
+        DDP_CHECK(cap < (1 << 30) && cap * WB < (1LL << 32), ctx, DDP_ERR_ARG, "heap_cap too large");
         const int32_t maxit =
             (o && o->max_iters) ? o->max_iters : (int32_t)std::min<int64_t>(4LL * g->N + 1024, 1 << 30);
         *p = GaGridPlan{WB, WB, 1, lcap, (int32_t)cap, 1024, maxit, true};
+        if (piles) {
+            const uint64_t total = (uint64_t)cap * WB;
+            uint64_t nc = 1;
+            while (nc * 2 <= total / 4) nc *= 2;
+            p->piles = true;
+            p->ncap = (uint32_t)nc;
+            p->fcap = (uint32_t)std::min<uint64_t>((total - nc) / 2, 0xFFFFFFFFull);
+            p->delta0 = std::max<uint32_t>(1, g->meanw);
+        }
         return DDP_OK;
     }
     if (B == 0) B = K >= 4096 ? 256 : 64;
@@ -542,8 +565,11 @@ int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const G
     a.lcap = P.lcap;
     a.cap = P.cap;
     a.maxit = P.maxit;
+    a.ncap = P.ncap;
+    a.fcap = P.fcap;
+    a.delta0 = P.delta0;
     const size_t lds = (size_t)P.lcap * P.B * 8;
-    WgFn fn = pick_wg(P.B, hv.kind);
+    WgFn fn = pick_wg(P.B, hv.kind, P.piles);
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(fn, dim3(1), dim3(P.B), lds, ctx->stream, a);

@@ -158,12 +158,13 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     double best_ms = 1e300;
     ddp_astar_opts_t win;
     std::memset(&win, 0, sizeof win);
-    // workgroup mode with 256..1024 queues, grid mode with 1024..8192
+    // workgroup mode with 256..1024 queues, pile mode with 512/1024 lanes, grid mode with 1024..8192
     struct Cand {
         int32_t K, B, mode;
     };
     std::vector<Cand> cands;
     for (int32_t K : {256, 512, 1024}) cands.push_back({K, 0, DDP_ASTAR_WORKGROUP});
+    for (int32_t K : {512, 1024}) cands.push_back({K, 0, DDP_ASTAR_PILES});
     for (int32_t K : {1024, 2048, 4096, 8192})
         for (int32_t B : {64, 256}) cands.push_back({K, B, DDP_ASTAR_GRID});
     for (const Cand &c : cands) {

@@ -149,7 +149,7 @@ typedef struct {
     int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = auto */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
     int32_t max_iters;  /* iteration limit; 0 = auto (bounded) */
-    int32_t mode;       /* DDP_ASTAR_AUTO / _GRID / _WORKGROUP */
+    int32_t mode;       /* DDP_ASTAR_AUTO / _GRID / _WORKGROUP / _PILES */
     int32_t reserved[2];
 } ddp_astar_opts_t;
 
@@ -157,11 +157,13 @@ typedef struct {
  * K/block workgroups that meet at a device-wide barrier every iteration.
  * WORKGROUP keeps all K <= 1024 queues in one workgroup (block = K rounded up
  * to a power of two >= 64): no device-wide barrier, but one CU's worth of
- * memory parallelism, which measures slower on the random and grid-maze
- * workloads, so AUTO selects GRID; ddp_tune_astar times both. */
+ * memory parallelism.  PILES runs the search on one workgroup of K <= 1024
+ * lanes with the near/far piles of the batched engine (DDP_BATCH_PILES)
+ * instead of per-lane heaps.  AUTO selects GRID; ddp_tune_astar times all. */
 #define DDP_ASTAR_AUTO      0
 #define DDP_ASTAR_GRID      1
 #define DDP_ASTAR_WORKGROUP 2
+#define DDP_ASTAR_PILES     3
 
 typedef struct {
     int64_t expansions;     /* nodes popped and expanded */

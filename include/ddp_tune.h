@@ -23,9 +23,9 @@ extern "C" {
 int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
                    const int32_t *dst, int32_t sample, ddp_batch_opts_t *best, float *qps);
 
-/* Times single-query GA* in workgroup mode (K 256..1024) and grid mode
- * (K 1024..8192, block 64/256) on up to `sample` of the queries; writes the
- * fastest options (K, block, mode) and its mean ms. */
+/* Times single-query GA* in workgroup mode (K 256..1024), pile mode (K 512,
+ * 1024) and grid mode (K 1024..8192, block 64/256) on up to `sample` of the
+ * queries; writes the fastest options (K, block, mode) and its mean ms. */
 int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
                    const int32_t *dst, int32_t sample, ddp_astar_opts_t *best, float *mean_ms);
 

@@ -39,7 +39,7 @@ def test_reorder_batch_and_single_geometric(ctx):
         say("reorder geo", i, s, t)
         assert c1[i] == dist[t]
         check_path(hg.off, hg.col, hg.w, p1[i], s, t, int(dist[t]))
-        for mode in (ddp_amd.MODE_GRID, ddp_amd.MODE_WORKGROUP):
+        for mode in (ddp_amd.MODE_GRID, ddp_amd.MODE_WORKGROUP, ddp_amd.MODE_PILES):
             r = ddp_amd.astar(ctx, relab, s, t, heur=ddp_amd.H_EUCLID, K=1024, mode=mode, want_parent=True)
             assert r.cost == dist[t]
             check_path(hg.off, hg.col, hg.w, r.path, s, t, int(dist[t]))

@@ -62,7 +62,8 @@ def test_bfs_hops_config1(ctx):
 @pytest.mark.parametrize("K,block,mode", [(1, 64, ddp_amd.MODE_AUTO), (64, 64, ddp_amd.MODE_AUTO),
                                           (200, 0, ddp_amd.MODE_WORKGROUP), (1024, 0, ddp_amd.MODE_WORKGROUP),
                                           (1024, 64, ddp_amd.MODE_GRID), (1024, 256, ddp_amd.MODE_GRID),
-                                          (4096, 1024, ddp_amd.MODE_AUTO)])
+                                          (4096, 1024, ddp_amd.MODE_AUTO), (100, 0, ddp_amd.MODE_PILES),
+                                          (1024, 0, ddp_amd.MODE_PILES)])
 def test_astar_small_graphs(ctx, K, block, mode):
     """Grid mode (K queues over K/block workgroups) and workgroup mode (all K
     queues in one workgroup) against the oracle's CPU A* and Dijkstra."""
@@ -107,8 +108,9 @@ def test_astar_config1(ctx):
         h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
         say("C1 s", s, "t", t)
         r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=1024)
-        rg = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=1024, mode=ddp_amd.MODE_WORKGROUP)
-        assert rg.cost == r.cost
+        for mode in (ddp_amd.MODE_WORKGROUP, ddp_amd.MODE_PILES):
+            rg = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_ARRAY, K=1024, mode=mode)
+            assert rg.cost == r.cost
         found_q, cost_q, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=True)
         found, cost, _, _ = pyoracle.astar_ref(hg.off, hg.col, hg.w, h, s, t, quirk=False)
         dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s, t)
@@ -126,7 +128,8 @@ def test_astar_grid_manhattan(ctx):
     s, t = 0, 128 * 96 - 1
     dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, s)
     assert dist[t] > 0
-    for K, mode in ((512, ddp_amd.MODE_AUTO), (512, ddp_amd.MODE_WORKGROUP), (2048, ddp_amd.MODE_AUTO),
+    for K, mode in ((512, ddp_amd.MODE_AUTO), (512, ddp_amd.MODE_WORKGROUP), (1024, ddp_amd.MODE_PILES),
+                    (2048, ddp_amd.MODE_AUTO),
                     (8192, ddp_amd.MODE_AUTO)):
         say("grid K", K, "mode", mode)
         r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_MANHATTAN, K=K, mode=mode)
