@@ -23,6 +23,7 @@ __all__ = [
     "DDPError", "INF_COST", "H_ZERO", "H_ARRAY", "H_MANHATTAN", "H_EUCLID", "MODE_AUTO", "MODE_GRID",
     "MODE_WORKGROUP", "MODE_PILES", "ENGINE_AUTO", "ENGINE_HEAPS", "ENGINE_PILES", "Context", "HostGraph",
     "DeviceGraph", "AStarResult", "astar", "Batch", "astar_batch", "Updates", "GPU_Graph", "GPU_A_Star",
+    "DAStar", "Server", "alternative_paths",
 ]
 
 
@@ -522,6 +523,26 @@ class DAStar:
             self.close()
         except Exception:
             pass
+
+
+def alternative_paths(ctx, hg, s, t, num_paths, h=None, K=0):
+    """The loop of k-short/GA_merge.cu:702-945: solve, then num_paths times
+    delete the edge into t on the current best path (parent(t) -> t, first
+    live instance) and re-solve, here by DA* repair.  Returns [(cost, path
+    s..t)], stopping after the first unreachable result."""
+    d = DAStar(ctx, hg, s, t, h=h, K=K)
+    try:
+        out = []
+        cost = d.cost
+        for k in range(num_paths + 1):
+            path = d.path() if cost != INF_COST else np.zeros(0, np.int32)
+            out.append((cost, path))
+            if k == num_paths or len(path) < 2:
+                break
+            cost, _ = d.apply([(0, int(path[-2]), int(t), 0)], mode=DAStar.REPAIR)
+        return out
+    finally:
+        d.close()
 
 
 def _result_dict(r):

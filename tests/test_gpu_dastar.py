@@ -137,3 +137,22 @@ def test_dastar_target_cut_off_and_reconnected(ctx):
         [(0, 2, 3, 0), (0, 2, 3, 0)],      # both parallel 2->3 edges gone: direct edge, cost 4
     ]
     run_batches(ctx, hg, batches, 0, 3, None, K=64)
+
+
+def test_alternative_paths_loop(ctx):
+    """ddp_amd.alternative_paths: each cost equals Dijkstra after deleting the
+    edge into t of every earlier path (the k-short/GA_merge.cu loop)."""
+    hg = ddp_amd.HostGraph.random(5000, 5, 1, 100, 51)
+    src, dst = hg.reachable_pairs(3, 52)
+    for s, t in zip(src, dst):
+        s, t = int(s), int(t)
+        res = ddp_amd.alternative_paths(ctx, hg, s, t, 8, K=1024)
+        cur = (hg.off, hg.col, hg.w)
+        for k, (cost, path) in enumerate(res):
+            if k:
+                prev = res[k - 1][1]
+                cur = pyoracle.apply_batch(*cur, [(0, int(prev[-2]), t, 0)])[:3]
+            dist, _ = pyoracle.dijkstra(*cur, s, t)
+            assert cost == (int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST), (s, t, k)
+            if cost != ddp_amd.INF_COST:
+                assert path[0] == s and path[-1] == t and path_cost(*cur, path) == cost

@@ -81,3 +81,32 @@ def test_da_star_contract():
         paths = [l for l in outs[0].splitlines() if l.startswith("[OUT] Path(in reverse):")]
         assert len(paths) == len(want)
         assert "[INFO] RUNTIME:" in outs[0]
+
+
+def test_k_short_contract():
+    """k_short (k-short/GA_merge.cu): after the first search, each further path
+    deletes the edge into t on the current best path and re-solves; every cost
+    equals Dijkstra on the graph with the same edges deleted (first live
+    instance, as the oracle's batch application does)."""
+    hg = ddp_amd.HostGraph.random(3000, 5, 1, 50, 41)
+    src, dst = hg.reachable_pairs(1, 42)
+    s, t = int(src[0]), int(dst[0])
+    with tempfile.TemporaryDirectory() as d:
+        hg.write_txt(os.path.join(d, "graph.txt"))
+        pyoracle.write_hx_txt(os.path.join(d, "Hx.txt"), np.zeros(hg.N, np.int32))
+        out = run("k_short", d, f"1024\n{s} {t}\n6\n").splitlines()
+    costs = [int(l.split()[-1]) for l in out if l.startswith("[OUT] Cost:")]
+    paths = [[] if "not found" in l else [int(x) for x in l.split(":", 1)[1].split()]
+             for l in out if l.startswith("[OUT] Path(in reverse):")]
+    assert len(costs) == len(paths) >= 2 and out[-1].startswith("[INFO] total run time")
+    cur = (hg.off, hg.col, hg.w)
+    for k, (cost, prev) in enumerate(zip(costs, [None] + paths[:-1])):
+        if prev is not None:  # the previous path, t first: its first edge in reverse is parent(t) -> t
+            cur = pyoracle.apply_batch(*cur, [(0, prev[1], prev[0], 0)])[:3]
+        dist, _ = pyoracle.dijkstra(*cur, s, t)
+        want = int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST
+        assert cost == want, k
+        if cost != ddp_amd.INF_COST:
+            assert paths[k][0] == t and paths[k][-1] == s
+            assert path_cost(*cur, paths[k][::-1]) == cost
+    assert costs == sorted(costs)  # each deletion can only lengthen the best path
