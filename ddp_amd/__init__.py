@@ -122,11 +122,18 @@ class HostGraph:
 
     # -- files
     @classmethod
-    def read_txt(cls, path):
+    def read_txt(cls, path, cache=False, cache_path=None):
+        """graph.txt (reference layout).  cache=True reads through a binary
+        cache (path + ".ddpbin" unless cache_path is given), written on the
+        first read and reused while the text file is unchanged."""
         L = _capi.lib()
         h = c_void_p()
         err = ctypes.create_string_buffer(512)
-        rc = L.ddp_read_graph_txt(os.fsencode(path), byref(h), err, 512)
+        if cache or cache_path:
+            rc = L.ddp_read_graph_cached(os.fsencode(path), None if cache_path is None else os.fsencode(cache_path),
+                                         byref(h), err, 512)
+        else:
+            rc = L.ddp_read_graph_txt(os.fsencode(path), byref(h), err, 512)
         if rc:
             raise DDPError(rc, err.value.decode())
         return cls._from_handle(h)

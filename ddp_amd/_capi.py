@@ -152,6 +152,7 @@ SIGNATURES = {
     "ddp_hgraph_coords": (P_I32, [c_void_p]),
     "ddp_hgraph_set_coords": (c_int, [c_void_p, P_I32]),
     "ddp_read_graph_txt": (c_int, [c_char_p, PP, c_char_p, c_int]),
+    "ddp_read_graph_cached": (c_int, [c_char_p, c_char_p, PP, c_char_p, c_int]),
     "ddp_write_graph_txt": (c_int, [c_char_p, c_void_p]),
     "ddp_read_hx_txt": (c_int, [c_char_p, c_int32, P_I32, c_char_p, c_int]),
     "ddp_write_hx_txt": (c_int, [c_char_p, c_int32, P_I32]),

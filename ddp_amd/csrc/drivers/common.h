@@ -45,10 +45,16 @@ struct HostGraph {
     }
 };
 
+// DDP_GRAPH_CACHE=1 reads graph.txt through the binary cache graph.txt.ddpbin
+// (ddp_read_graph_cached), written on the first run and reused while the text
+// file is unchanged.
 inline void load_graph(const char *path, HostGraph &g)
 {
     char err[512] = {0};
-    if (ddp_read_graph_txt(path, &g.h, err, sizeof err) != DDP_OK) die(path, err, 4);
+    const char *c = std::getenv("DDP_GRAPH_CACHE");
+    const int rc = (c && std::atoi(c)) ? ddp_read_graph_cached(path, nullptr, &g.h, err, sizeof err)
+                                       : ddp_read_graph_txt(path, &g.h, err, sizeof err);
+    if (rc != DDP_OK) die(path, err, 4);
     ddp_hgraph_view(g.h, &g.N, &g.E, &g.off, &g.col, &g.w);
 }
 

@@ -503,3 +503,89 @@ extern "C" int ddp_hgraph_apply_batch(const ddp_hgraph *g, const ddp_update_t *l
     *out = r;
     return DDP_OK;
 }
+
+// ------------------------------------------------------------ binary cache
+//
+// An exact binary image of a parsed graph.txt: a header naming the text file's
+// size and modification time, then off[N], col[E], w[E] as stored in memory.
+// A cache whose header does not match the text file (or is unreadable) is
+// ignored and rewritten; failing to write one is not an error.
+
+namespace {
+
+struct CacheHeader {
+    char magic[8];            // "DDPGRPH1"
+    uint64_t src_size;
+    int64_t src_mtime_s, src_mtime_ns;
+    int32_t N, E;
+};
+
+bool text_identity(const char *path, uint64_t *size, int64_t *sec, int64_t *nsec)
+{
+    struct stat st;
+    if (stat(path, &st) != 0) return false;
+    *size = (uint64_t)st.st_size;
+    *sec = (int64_t)st.st_mtim.tv_sec;
+    *nsec = (int64_t)st.st_mtim.tv_nsec;
+    return true;
+}
+
+bool read_cache(const char *cache, const CacheHeader &want, ddp_hgraph **out)
+{
+    MappedFile f;
+    if (!f.open_(cache) || f.n < sizeof(CacheHeader)) return false;
+    CacheHeader h;
+    std::memcpy(&h, f.p, sizeof h);
+    if (std::memcmp(h.magic, "DDPGRPH1", 8) || h.src_size != want.src_size || h.src_mtime_s != want.src_mtime_s ||
+        h.src_mtime_ns != want.src_mtime_ns || h.N <= 0 || h.E < 0)
+        return false;
+    const size_t need = sizeof h + (size_t)h.N * 4 + (size_t)h.E * 8;
+    if (f.n != need) return false;
+    ddp_hgraph *g = new ddp_hgraph();
+    g->N = h.N;
+    g->E = h.E;
+    const char *p = f.p + sizeof h;
+    g->off.resize(h.N);
+    g->col.resize(h.E);
+    g->w.resize(h.E);
+    std::memcpy(g->off.data(), p, (size_t)h.N * 4);
+    std::memcpy(g->col.data(), p + (size_t)h.N * 4, (size_t)h.E * 4);
+    std::memcpy(g->w.data(), p + (size_t)h.N * 4 + (size_t)h.E * 4, (size_t)h.E * 4);
+    *out = g;
+    return true;
+}
+
+void write_cache(const char *cache, CacheHeader h, const ddp_hgraph *g)
+{
+    const std::string tmp = std::string(cache) + ".tmp";
+    FILE *f = std::fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    std::memcpy(h.magic, "DDPGRPH1", 8);
+    h.N = g->N;
+    h.E = g->E;
+    bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(g->off.data(), 4, g->off.size(), f) == g->off.size() &&
+              std::fwrite(g->col.data(), 4, g->col.size(), f) == g->col.size() &&
+              std::fwrite(g->w.data(), 4, g->w.size(), f) == g->w.size();
+    ok = (std::fclose(f) == 0) && ok;
+    if (ok) ok = std::rename(tmp.c_str(), cache) == 0;  // readers never see a partial cache
+    if (!ok) std::remove(tmp.c_str());
+}
+
+}  // namespace
+
+extern "C" int ddp_read_graph_cached(const char *path, const char *cache_path, ddp_hgraph **out, char *err,
+                                     int errlen)
+{
+    if (!path || !out) return DDP_ERR_ARG;
+    *out = nullptr;
+    const std::string cache = cache_path ? std::string(cache_path) : std::string(path) + ".ddpbin";
+    CacheHeader want{};
+    if (!text_identity(path, &want.src_size, &want.src_mtime_s, &want.src_mtime_ns)) {
+        put_err(err, errlen, "cannot open %s: %s", path, strerror(errno));
+        return DDP_ERR_IO;
+    }
+    if (read_cache(cache.c_str(), want, out)) return DDP_OK;
+    const int rc = ddp_read_graph_txt(path, out, err, errlen);
+    if (rc == DDP_OK) write_cache(cache.c_str(), want, *out);
+    return rc;
+}

@@ -36,6 +36,14 @@ int ddp_hgraph_set_coords(ddp_hgraph *g, const int32_t *xy);
 /* graph.txt reader (replaces the fscanf loops of GA_star/GA_v2.cu:307-348;
  * parallel parse of an mmap'd file).  err receives a message on failure. */
 int ddp_read_graph_txt(const char *path, ddp_hgraph **out, char *err, int errlen);
+/* ddp_read_graph_txt through a binary cache at cache_path (NULL: path +
+ * ".ddpbin").  A cache written for the same text file (its size and
+ * modification time are in the cache header) is read instead of parsing;
+ * otherwise the text is parsed and the cache rewritten (best effort).  No
+ * reference counterpart: the reference parses the text on every run
+ * (GA_star/GA_v2.cu:307-349). */
+int ddp_read_graph_cached(const char *path, const char *cache_path, ddp_hgraph **out, char *err,
+                          int errlen);
 /* graph.txt writer with the reference's token layout (GenerateCSR.cpp:92-132). */
 int ddp_write_graph_txt(const char *path, const ddp_hgraph *g);
 /* Hx.txt: reads exactly n ints (n = N). */

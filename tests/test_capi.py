@@ -184,3 +184,28 @@ def test_hub_pairs_are_connected():
         assert s != t
         dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(s))
         assert dist[t] >= 0
+
+
+def test_binary_graph_cache():
+    """The cache returns exactly what the text parser returns, is reused while
+    the text is unchanged, and is rebuilt when the text changes."""
+    hg = ddp_amd.HostGraph.random(20000, 5, 1, 1000, 61)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "graph.txt")
+        hg.write_txt(p)
+        a = ddp_amd.HostGraph.read_txt(p, cache=True)  # parses, writes graph.txt.ddpbin
+        assert os.path.exists(p + ".ddpbin")
+        b = ddp_amd.HostGraph.read_txt(p, cache=True)  # from the cache
+        for g in (a, b):
+            assert g.N == hg.N and g.E == hg.E
+            assert np.array_equal(g.off, hg.off) and np.array_equal(g.col, hg.col) and np.array_equal(g.w, hg.w)
+        # a cache that does not belong to the text file is ignored
+        other = ddp_amd.HostGraph.random(500, 3, 1, 9, 62)
+        other.write_txt(p)
+        c = ddp_amd.HostGraph.read_txt(p, cache=True)
+        assert c.N == 500 and np.array_equal(c.col, other.col)
+        # a corrupt cache is ignored too
+        with open(p + ".ddpbin", "r+b") as f:
+            f.truncate(100)
+        e = ddp_amd.HostGraph.read_txt(p, cache=True)
+        assert np.array_equal(e.col, other.col)
