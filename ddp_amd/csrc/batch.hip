@@ -37,6 +37,7 @@ struct BatchArgs {
     unsigned long long *stats;
     int32_t N, lcap, cap, maxit;
     uint32_t ncap, fcap, delta0;  // pile engine: near ring, far buffers, first window
+    uint32_t lring;               // pile engine: entries of the LDS near ring (dynamic LDS)
 };
 
 enum { ENG_HEAPS = 0, ENG_PILES = 1 };
@@ -45,7 +46,8 @@ template <int B, int HK, int ENG>
 __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
 {
     extern __shared__ uint64_t s_heap[];  // lcap * B (heap engine)
-    __shared__ SlotLds<B> S;
+    constexpr int SV = (ENG == ENG_PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
+    __shared__ SlotLds<B, SV> S;
     __shared__ int32_t s_q;
 
     const int lane = threadIdx.x;
@@ -55,7 +57,7 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
     int32_t *touched = a.touched + slot * N;
     uint64_t *home = a.home + slot * (int64_t)a.cap * B;  // this slot's queue memory
     const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
-    const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap};
+    const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
     ExpandCounters C;
     uint64_t stale = 0;
     for (;;) {
@@ -71,10 +73,10 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
         else
             slot_search<B, HK>(a.g, a.h, gp, touched, H, s_heap, N, s, t, a.maxit, S, C, stale);
         if (a.path_cap > 0) {
-            slot_path<B>(gp, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
+            slot_path(gp, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
             if (lane == 0) a.plen[q] = S.plen;
         }
-        slot_reset<B>(gp, touched, S);
+        slot_reset(gp, touched, S);
         if (lane == 0) {
             a.cost[q] = (int32_t)S.F;
             a.qerr[q] = S.err;
@@ -138,7 +140,8 @@ struct ddp_batch {
     HeurView hv{};
     int32_t maxq = 0, B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0, path_cap = 0, schedule = 0;
     int32_t eng = ENG_HEAPS;
-    uint32_t ncap = 0, fcap = 0, delta0 = 1;  // pile engine sizes
+    uint32_t ncap = 0, fcap = 0, delta0 = 1, lring = 0;  // pile engine sizes
+    size_t lds = 0;                                       // dynamic LDS bytes per workgroup
     int32_t nq = 0;
     // device
     int32_t *qs = nullptr, *qt = nullptr, *order = nullptr, *cost = nullptr, *plen = nullptr, *path = nullptr,
@@ -196,7 +199,10 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     if (eng == ENG_PILES && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot
         cap = (int32_t)std::min<int64_t>(std::max<int64_t>(cap, (3LL * g->N + B - 1) / B), INT32_MAX / 2);
     cap = std::max(cap, std::max(lcap, 16));
-    const size_t lds = (size_t)lcap * B * 8;
+    // pile engine: the dynamic LDS holds the near ring (8 entries per lane,
+    // 64 KiB at 1,024 lanes, which the one-slot-per-CU occupancy leaves free)
+    const uint32_t lring = eng == ENG_PILES ? 8u * (uint32_t)B : 0u;
+    const size_t lds = eng == ENG_PILES ? (size_t)lring * 8 : (size_t)lcap * B * 8;
     int per_cu = 0;
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)pick_batch(B, hv.kind, eng),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -229,6 +235,8 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     b->path_cap = o ? std::max(0, o->path_cap) : 0;
     b->schedule = o ? o->schedule : 0;
     b->eng = eng;
+    b->lring = lring;
+    b->lds = lds;
     {
         // pile engine: the slot's cap * B queue entries hold a near ring (a
         // power of two, about a quarter) and two far buffers
@@ -367,11 +375,11 @@ static int batch_capture(ddp_batch *b)
     a.ncap = b->ncap;
     a.fcap = b->fcap;
     a.delta0 = b->delta0;
+    a.lring = b->lring;
     const int32_t grid = std::max(1, std::min(b->slots, std::max(1, b->nq)));
     DDP_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, ctx->stream, b->next, b->stats);
-    hipLaunchKernelGGL(pick_batch(b->B, b->hv.kind, b->eng), dim3(grid), dim3(b->B), (size_t)b->lcap * b->B * 8,
-                       ctx->stream, a);
+    hipLaunchKernelGGL(pick_batch(b->B, b->hv.kind, b->eng), dim3(grid), dim3(b->B), b->lds, ctx->stream, a);
     hipError_t e = hipStreamEndCapture(ctx->stream, &b->graph);
     if (e != hipSuccess) return ddp_hip_err(ctx, e, "hipStreamEndCapture", __FILE__, __LINE__);
     DDP_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));

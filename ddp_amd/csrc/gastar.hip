@@ -268,6 +268,7 @@ struct WgArgs {
     unsigned long long *stats;
     int32_t N, s, t, lcap, cap, maxit;
     uint32_t ncap, fcap, delta0;  // pile mode: near ring, far buffers, first window
+    uint32_t lring;               // pile mode: entries of the LDS near ring (dynamic LDS)
 };
 
 // PILES = false: lane heaps (slot_search); true: near/far piles (slot_search_piles)
@@ -275,13 +276,14 @@ template <int B, int HK, bool PILES>
 __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
 {
     extern __shared__ uint64_t s_heap[];  // lcap * B (heaps)
-    __shared__ SlotLds<B> S;
+    constexpr int SV = (PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
+    __shared__ SlotLds<B, SV> S;
     const int lane = threadIdx.x;
     const LaneHeap H{s_heap + lane, a.home + lane, B, a.lcap, (int64_t)B, a.cap};
     ExpandCounters C;
     uint64_t stale = 0;
     if constexpr (PILES) {
-        const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap};
+        const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
         slot_search_piles<B, HK>(a.g, a.h, a.gp, a.touched, P, a.N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
     } else {
         slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
@@ -568,7 +570,8 @@ int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const G
     a.ncap = P.ncap;
     a.fcap = P.fcap;
     a.delta0 = P.delta0;
-    const size_t lds = (size_t)P.lcap * P.B * 8;
+    a.lring = P.piles ? 8u * (uint32_t)P.B : 0u;  // LDS near ring
+    const size_t lds = P.piles ? (size_t)a.lring * 8 : (size_t)P.lcap * P.B * 8;
     WgFn fn = pick_wg(P.B, hv.kind, P.piles);
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));

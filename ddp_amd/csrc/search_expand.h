@@ -45,36 +45,48 @@ struct ExpandCounters {
     uint64_t rebuilds = 0;  // pile engine: far piles rebuilt after an overflow (lane 0 counts)
 };
 
-// Edges handled per lane and round: more overlap for small blocks, fewer
-// registers for large ones.
+// Edges handled per lane and round (memory requests a lane keeps in flight
+// per stage).  Four where registers allow it without lowering occupancy: the
+// 1,024-lane pile kernel needs 110 VGPRs at four (101 at two; eight spills).
 template <int B>
 constexpr int expand_unroll()
 {
-    return B <= 256 ? 4 : 2;
+    return B == 512 ? 2 : 4;
 }
 
 // Push policy interface:
 //   void target(uint32_t f)                    -- t improved to f (t is never queued)
 //   void touch(int32_t c, uint64_t prev)       -- c's word was prev before the update
 //   void operator()(bool has, uint64_t key)    -- block-collective: all lanes call it
-// Each lane brings u (or -1) and its row span (row_span, or the same values
-// loaded earlier by the caller).
-template <int B, int HK, class Push>
+// Each lane brings up to P nodes u[p] (or -1) and their row spans (row_span,
+// or the same values loaded earlier by the caller); node p of lane l is
+// "virtual lane" l*P + p of a B*P-wide expansion, so a wide round pays the
+// scan and barriers once for B*P rows.
+template <int B, int HK, int P, class Push>
 __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
-                                            int32_t u, const RowSpan &rs, ExpandLds<B> &L, ExpandCounters &C,
-                                            Push &push)
+                                            const int32_t (&u)[P], const RowSpan (&rs)[P], ExpandLds<B * P> &L,
+                                            ExpandCounters &C, Push &push)
 {
     constexpr int R = expand_unroll<B>();
+    constexpr int V = B * P;
     const int lane = threadIdx.x;
-    const int32_t deg = u >= 0 ? rs.deg : 0;
-    const uint32_t gu = rs.gu;
-    if (u >= 0) C.expanded++;
+    int32_t deg[P], lsum = 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        deg[p] = u[p] >= 0 ? rs[p].deg : 0;
+        lsum += deg[p];
+        if (u[p] >= 0) C.expanded++;
+    }
     int32_t total;
-    const int32_t incl = block_incl_scan<B>(deg, L.tmp, total);
-    L.incl[lane] = incl;
-    L.rowb[lane] = rs.b;
-    L.gu[lane] = gu;
-    L.node[lane] = u;
+    int32_t run = block_incl_scan<B>(lsum, L.tmp, total) - lsum;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        run += deg[p];
+        L.incl[lane * P + p] = run;
+        L.rowb[lane * P + p] = rs[p].b;
+        L.gu[lane * P + p] = rs[p].gu;
+        L.node[lane * P + p] = u[p];
+    }
     __syncthreads();
     for (int32_t base = 0; base < total; base += B * R) {
         int32_t c[R], jj[R];
@@ -87,7 +99,7 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             jj[r] = 0;
             wt[r] = 0;
             if (e < total) {
-                int32_t lo = 0, hi = B - 1;
+                int32_t lo = 0, hi = V - 1;
                 while (lo < hi) {
                     const int32_t mid = (lo + hi) >> 1;
                     if (L.incl[mid] > e) hi = mid;
@@ -168,12 +180,15 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             push(has, key);
         }
     }
-    if (G.lhead) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        if (!G.lhead) break;
         // Versioned inserts: each lane walks its own node's list, oldest
         // first, stopping at the first entry newer than this version (links
         // carry the inserting version, so newer entries are never read).
-        uint64_t link = u >= 0 ? __hip_atomic_load(G.lhead + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : ~0ull;
+        const uint32_t gu = rs[p].gu;
+        uint64_t link = u[p] >= 0 ? __hip_atomic_load(G.lhead + u[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : ~0ull;
         for (;;) {
             bool active = false, has = false;
             uint64_t key = 0;
@@ -193,7 +208,7 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
                 if (HK != DDP_H_ARRAY || hcn >= 0) {
                     const uint32_t cand = gu + G.lw[x] + (uint32_t)hcn;
                     uint64_t prev;
-                    if (relax_gp(gp, cn, cand, (uint32_t)u, &prev)) {
+                    if (relax_gp(gp, cn, cand, (uint32_t)u[p], &prev)) {
                         C.improved++;
                         push.touch(cn, prev);
                         if (cn == t) {
@@ -217,5 +232,7 @@ __device__ __forceinline__ void expand_block(const GraphView &G, const HeurView 
                                              int32_t u, uint32_t fu, ExpandLds<B> &L, ExpandCounters &C,
                                              Push &push)
 {
-    expand_rows<B, HK>(G, H, gp, t, u, row_span<HK>(G, H, u, fu), L, C, push);
+    const int32_t uu[1] = {u};
+    const RowSpan rs[1] = {row_span<HK>(G, H, u, fu)};
+    expand_rows<B, HK, 1>(G, H, gp, t, uu, rs, L, C, push);
 }

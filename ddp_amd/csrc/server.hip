@@ -78,6 +78,7 @@ struct ServeArgs {
     unsigned long long *stats;
     int32_t N, lcap, cap, maxit;
     uint32_t ncap, fcap, delta0;  // pile engine: near ring, far buffers, first window
+    uint32_t lring;               // pile engine: entries of the LDS near ring (dynamic LDS)
     uint64_t idle_ticks, wait_ticks;
 };
 
@@ -141,7 +142,8 @@ template <int B, int HK, int ENG>
 __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
 {
     extern __shared__ uint64_t s_heap[];
-    __shared__ SlotLds<B> S;
+    constexpr int SV = (ENG == SENG_PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
+    __shared__ SlotLds<B, SV> S;
     __shared__ int32_t s_ticket;
     __shared__ uint32_t s_uerr;
     const int lane = threadIdx.x;
@@ -151,7 +153,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
     int32_t *touched = a.touched + slot * N;
     uint64_t *home = a.home + slot * (int64_t)a.cap * B;  // this slot's queue memory
     const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
-    const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap};
+    const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
     ExpandCounters C;
     uint64_t stale = 0;
     for (;;) {
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                 slot_search<B, HK>(G, a.h, gp, touched, H, s_heap, N, rq.s, rq.t, a.maxit, S, C, stale);
             cost = (int32_t)S.F;
             status = S.err;
-            slot_reset<B>(gp, touched, S);
+            slot_reset(gp, touched, S);
         }
         if (lane == 0) {
             Res *r = a.res + ((uint32_t)ticket & a.ring_mask);
@@ -314,7 +316,8 @@ struct ddp_server {
     HeurView hv{};
     int32_t B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0;
     int32_t eng = SENG_PILES;
-    uint32_t ncap = 0, fcap = 0, delta0 = 1;
+    uint32_t ncap = 0, fcap = 0, delta0 = 1, lring = 0;
+    size_t lds = 0;  // dynamic LDS bytes per workgroup
     uint32_t ring = 0;
     int32_t log_cap = 0, log_used = 0;
     int32_t idle_ms = 2000;
@@ -403,11 +406,12 @@ static int server_launch(ddp_server *sv)
     a.ncap = sv->ncap;
     a.fcap = sv->fcap;
     a.delta0 = sv->delta0;
+    a.lring = sv->lring;
     a.idle_ticks = (uint64_t)sv->idle_ms * 100000ull;
     a.wait_ticks = 100000ull * 60000;  // a version wait longer than a minute is an error
     void *args[] = {&a};
     DDP_HIP(ctx, hipLaunchKernel((const void *)pick_serve(sv->B, sv->hv.kind, sv->eng), dim3(sv->slots), dim3(sv->B),
-                                 args, (size_t)sv->lcap * sv->B * 8, ctx->stream));
+                                 args, sv->lds, ctx->stream));
     DDP_HIP(ctx, hipEventRecord(sv->done_ev, ctx->stream));
     sv->running = true;
     return DDP_OK;
@@ -480,8 +484,10 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
         return fail(ddp_set_err(ctx, DDP_ERR_ARG, "block must be 64..1024 and ring a power of two >= 64"));
     const void *fn = (const void *)pick_serve(sv->B, hv.kind, sv->eng);
     int per_cu = 0;
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sv->lcap * sv->B * 8));
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sv->B, (size_t)sv->lcap * sv->B * 8);
+    sv->lring = sv->eng == SENG_PILES ? 8u * (uint32_t)sv->B : 0u;  // LDS near ring, as for batches
+    sv->lds = sv->eng == SENG_PILES ? (size_t)sv->lring * 8 : (size_t)sv->lcap * sv->B * 8;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sv->lds);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sv->B, sv->lds);
     if (e != hipSuccess || per_cu < 1) return fail(ddp_set_err(ctx, DDP_ERR_HIP, "serve kernel occupancy query failed"));
     int64_t slots = (o && o->slots) ? o->slots : (int64_t)per_cu * ctx->prop.multiProcessorCount;
     slots = std::min<int64_t>(slots, (int64_t)per_cu * ctx->prop.multiProcessorCount);

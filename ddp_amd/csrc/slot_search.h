@@ -10,9 +10,11 @@
 
 enum : uint32_t { QERR_HEAP = 1, QERR_ITERS = 2, QERR_BADQ = 4, QERR_PATH = 8 };
 
-template <int B>
+// V: expansion width (B lanes x entries per lane; the pile engine can take
+// more than one entry per lane and round).
+template <int B, int V = B>
 struct SlotLds {
-    ExpandLds<B> L;
+    ExpandLds<V> L;
     uint32_t red[16];
     uint32_t F;        // best f of the target so far
     uint32_t err;
@@ -20,7 +22,8 @@ struct SlotLds {
     int32_t plen;
     int32_t iters;     // iterations of the last search (pop rounds)
     // pile engine (slot_search_piles)
-    uint32_t near_head, near_tail;  // ring positions (monotonic)
+    uint32_t lring_head, lring_tail;  // LDS near ring positions (monotonic)
+    uint32_t near_head, near_tail;  // HBM near ring positions (monotonic)
     uint32_t far_n, far_n2;         // entries in the current / next far pile
     uint32_t thr, delta;            // near window [.., thr) and its width
     uint32_t phase_n;               // near entries taken since the last refill
@@ -202,18 +205,19 @@ __device__ __forceinline__ uint32_t far_reserve(bool want, uint32_t *count)
 }
 
 struct PilePtrs {
-    uint64_t *near;  // ncap entries (power of two), ring
+    uint64_t *near;  // ncap entries (power of two), ring in HBM
     uint64_t *far0;  // fcap entries
     uint64_t *far1;  // fcap entries
     uint32_t ncap, fcap;
+    uint64_t *lring = nullptr;  // lcap entries (power of two, or 0), ring in LDS
+    uint32_t lcap = 0;
 };
 
-template <int B>
+template <int B, int V>
 struct PilePush {
-    uint64_t *near;
+    const PilePtrs *P;
     uint64_t *far;
-    uint32_t nmask, ncap, fcap;
-    SlotLds<B> *S;
+    SlotLds<B, V> *S;
     int32_t *touched;
     uint32_t err = 0;
 
@@ -222,15 +226,19 @@ struct PilePush {
     {
         if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(&S->ntouch, 1)] = c;
     }
+    // near candidates: the LDS ring, then the HBM ring, then the far pile
     __device__ void operator()(bool has, uint64_t key)
     {
         const uint32_t f = (uint32_t)(key >> 32);
         has = has && f < __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t ni = near_reserve(has && f < S->thr, &S->near_tail, S->near_head, ncap);
-        if (ni != 0xFFFFFFFFu) near[ni & nmask] = key;
-        const uint32_t fi = far_reserve(has && ni == 0xFFFFFFFFu, &S->far_n);
+        const bool nearc = has && f < S->thr;
+        const uint32_t li = near_reserve(nearc, &S->lring_tail, S->lring_head, P->lcap);
+        if (li != 0xFFFFFFFFu) P->lring[li & (P->lcap - 1)] = key;
+        const uint32_t ni = near_reserve(nearc && li == 0xFFFFFFFFu, &S->near_tail, S->near_head, P->ncap);
+        if (ni != 0xFFFFFFFFu) P->near[ni & (P->ncap - 1)] = key;
+        const uint32_t fi = far_reserve(has && li == 0xFFFFFFFFu && ni == 0xFFFFFFFFu, &S->far_n);
         if (fi == 0xFFFFFFFFu) return;
-        if (fi < fcap) far[fi] = key;
+        if (fi < P->fcap) far[fi] = key;
         else S->ovf = 1;  // dropped: the next refill rebuilds the far pile
     }
 };
@@ -240,20 +248,42 @@ __device__ __forceinline__ uint32_t sat_add_u32(uint32_t a, uint32_t b)
     return a > 0xFFFFFFFFu - b ? 0xFFFFFFFFu : a + b;
 }
 
+// Entries each lane takes from the near ring per round.  One: two per lane
+// (2,048 per round at 1,024 lanes) measured slower on the C4 batch (139k
+// against 148k queries/s) -- wider rounds expand more nodes out of f order
+// (80M against 63M) and do not shorten the longest searches, whose rounds are
+// bound by the memory requests one CU keeps in flight, not by per-round costs.
+template <int B>
+constexpr int pile_entries_per_lane()
+{
+    return 1;
+}
+
 // Same contract as slot_search (S.F, S.err, gp and touched describe the search
-// on return).  delta0 is the first window width.
-template <int B, int HK>
+// on return).  delta0 is the first window width.  W entries per lane per round.
+//
+// The near ring has two levels: P.lring in LDS (P.lcap entries) and P.near in
+// HBM for what does not fit.  Rounds take from the LDS ring while it holds
+// entries; an ordinary round then ends with a plain barrier, since everything
+// the next round reads is in LDS or was written by atomics that completed.
+// Reading HBM-resident entries (the HBM ring, the far pile, the touched log)
+// is always preceded by a barrier at which every wave has drained its stores.
+// The ring heads are uniform registers, mirrored in LDS for the pushes'
+// capacity checks.
+template <int B, int HK, int W = pile_entries_per_lane<B>()>
 __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched,
                                   const PilePtrs &P, int32_t N, int32_t s, int32_t t, int32_t maxit,
-                                  uint32_t delta0, SlotLds<B> &S, ExpandCounters &C, uint64_t &stale)
+                                  uint32_t delta0, SlotLds<B, B * W> &S, ExpandCounters &C, uint64_t &stale)
 {
+    constexpr uint32_t V = (uint32_t)B * W;
     const int lane = threadIdx.x;
-    const uint32_t nmask = P.ncap - 1;
+    const uint32_t nmask = P.ncap - 1, lmask = P.lcap ? P.lcap - 1 : 0;
     if (lane == 0) {
         S.F = DDP_INF_F;
         S.ntouch = 0;
         S.err = 0;
         S.iters = 0;
+        S.lring_head = S.lring_tail = 0;
         S.near_head = S.near_tail = 0;
         S.far_n = S.far_n2 = 0;
         S.delta = delta0 > 0 ? delta0 : 1;
@@ -276,8 +306,13 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             gp[s] = ((uint64_t)(uint32_t)hs << 32) | 0xFFFFFFFFull;
             touched[0] = s;
             S.ntouch = 1;
-            P.near[0] = key;
-            S.near_tail = 1;
+            if (P.lcap) {
+                P.lring[0] = key;
+                S.lring_tail = 1;
+            } else {
+                P.near[0] = key;
+                S.near_tail = 1;
+            }
             S.thr = sat_add_u32((uint32_t)hs, S.delta);
         }
     } else if (lane == 0) {
@@ -285,10 +320,17 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
     }
     block_sync_global();
     uint64_t *far = P.far0, *far2 = P.far1;
+    uint32_t lh = 0, hh = 0;  // ring heads (uniform)
     int32_t it = 0;
     while (run) {
-        const uint32_t head = S.near_head, tail = S.near_tail, F = S.F;
-        if (head == tail) {
+        const uint32_t lt = S.lring_tail, F = S.F;
+        bool from_lds = lt != lh;
+        uint32_t avail = lt - lh;
+        if (!from_lds) {
+            block_sync_global();  // HBM entries ahead: every wave's stores have landed
+            avail = S.near_tail - hh;
+        }
+        if (avail == 0) {
             if (S.ovf) {
                 // ---- rebuild the far pile from the touched log (entries were dropped)
                 const int32_t nt = S.ntouch;
@@ -319,7 +361,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
                 __syncthreads();
                 if (S.err) break;
             }
-            // ---- refill the near ring from the far pile
+            // ---- refill the near rings from the far pile
             const uint32_t nf = S.far_n < P.fcap ? S.far_n : P.fcap;
             uint32_t mymin = 0xFFFFFFFFu;
             for (uint32_t i = lane; i < nf; i += B) {
@@ -330,8 +372,8 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             const uint32_t fmin = block_min_u32<B>(mymin, S.red);
             if (fmin >= F) break;  // nothing open can improve t (also: far pile empty)
             if (lane == 0) {
-                if (S.phase_n < B) S.delta = sat_add_u32(S.delta, S.delta);
-                else if (S.phase_n > 8 * B && S.delta > 1) S.delta >>= 1;
+                if (S.phase_n < V) S.delta = sat_add_u32(S.delta, S.delta);
+                else if (S.phase_n > 8 * V && S.delta > 1) S.delta >>= 1;
                 S.thr = sat_add_u32(fmin, S.delta);
                 S.phase_n = 0;
                 S.far_n2 = 0;
@@ -347,9 +389,12 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
                     const uint32_t f = (uint32_t)(key >> 32);
                     live = f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f;  // else hopeless or stale
                 }
-                const uint32_t ni = near_reserve(live && (uint32_t)(key >> 32) < thr, &S.near_tail, head, P.ncap);
+                const bool nearc = live && (uint32_t)(key >> 32) < thr;
+                const uint32_t li = near_reserve(nearc, &S.lring_tail, lh, P.lcap);
+                if (li != 0xFFFFFFFFu) P.lring[li & lmask] = key;
+                const uint32_t ni = near_reserve(nearc && li == 0xFFFFFFFFu, &S.near_tail, hh, P.ncap);
                 if (ni != 0xFFFFFFFFu) P.near[ni & nmask] = key;
-                const uint32_t fi = far_reserve(live && ni == 0xFFFFFFFFu, &S.far_n2);
+                const uint32_t fi = far_reserve(live && li == 0xFFFFFFFFu && ni == 0xFFFFFFFFu, &S.far_n2);
                 if (fi != 0xFFFFFFFFu) far2[fi] = key;  // fewer than nf <= fcap entries
             }
             block_sync_global();
@@ -366,37 +411,46 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             break;
         }
         ++it;
-        // ---- take up to B near entries, one per lane
-        const uint32_t n = tail - head < (uint32_t)B ? tail - head : (uint32_t)B;
-        int32_t u = -1;
-        RowSpan rs;
-        if ((uint32_t)lane < n) {
-            const uint64_t key = P.near[(head + lane) & nmask];
-            const uint32_t f = (uint32_t)(key >> 32);
-            const int32_t node = (int32_t)(uint32_t)key;
-            // the node's word, row and heuristic input in one round trip: the
-            // row is loaded before the entry is known to be current
-            const uint64_t word = ld_gp(gp + node);
-            const int32_t rb = G.off[node], re = G.off[node + 1];
-            const int32_t hn = heur_eval<HK>(hv, node);
-            if (f < F && (uint32_t)(word >> 32) == f) {
-                u = node;
-                rs.b = rb;
-                rs.deg = re - rb;
-                rs.gu = f - (uint32_t)hn;
-            } else {
-                ++stale;
+        // ---- take up to V entries from one ring, W per lane (coalesced: entry w*B + lane)
+        const uint32_t n = avail < V ? avail : V;
+        int32_t u[W];
+        RowSpan rs[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            u[w] = -1;
+            const uint32_t idx = (uint32_t)(w * B + lane);
+            if (idx < n) {
+                const uint64_t key = from_lds ? P.lring[(lh + idx) & lmask] : P.near[(hh + idx) & nmask];
+                const uint32_t f = (uint32_t)(key >> 32);
+                const int32_t node = (int32_t)(uint32_t)key;
+                // the node's word, row and heuristic input in one round trip:
+                // the row is loaded before the entry is known to be current
+                const uint64_t word = ld_gp(gp + node);
+                const int32_t rb = G.off[node], re = G.off[node + 1];
+                const int32_t hn = heur_eval<HK>(hv, node);
+                if (f < F && (uint32_t)(word >> 32) == f) {
+                    u[w] = node;
+                    rs[w].b = rb;
+                    rs[w].deg = re - rb;
+                    rs[w].gu = f - (uint32_t)hn;
+                } else {
+                    ++stale;
+                }
             }
         }
-        __syncthreads();  // every lane has read head/tail and its entry
+        // Pushes of this round start after expand_rows' first barrier, when
+        // every lane has read its entry, so the freed slots may be reused.
+        if (from_lds) lh += n;
+        else hh += n;
         if (lane == 0) {
-            S.near_head = head + n;
+            S.lring_head = lh;
+            S.near_head = hh;
             S.phase_n += n;
         }
-        PilePush<B> push{P.near, far, nmask, P.ncap, P.fcap, &S, touched};
-        expand_rows<B, HK>(G, hv, gp, t, u, rs, S.L, C, push);
+        PilePush<B, B * W> push{&P, far, &S, touched};
+        expand_rows<B, HK, W>(G, hv, gp, t, u, rs, S.L, C, push);
         if (push.err) atomicOr(&S.err, push.err);
-        block_sync_global();
+        __syncthreads();
         if (S.err) break;
     }
     if (lane == 0) S.iters = it;
@@ -405,9 +459,9 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
 
 // Lane 0 writes the path s..t into out[0..cap) and returns its length in
 // S.plen (0 if unreachable, -len if longer than cap).
-template <int B>
+template <int B, int V>
 __device__ void slot_path(const uint64_t *gp, int32_t N, int32_t s, int32_t t, int32_t *out, int32_t cap,
-                          SlotLds<B> &S)
+                          SlotLds<B, V> &S)
 {
     if (threadIdx.x == 0) {
         int32_t n = 0;
@@ -439,8 +493,8 @@ __device__ void slot_path(const uint64_t *gp, int32_t N, int32_t s, int32_t t, i
 // Clears every (f, parent) word the query touched.  Starts with a barrier so
 // lane 0's path walk has finished reading parents; ends with one so the next
 // query's atomics see the cleared words.
-template <int B>
-__device__ void slot_reset(uint64_t *gp, const int32_t *touched, SlotLds<B> &S)
+template <int B, int V>
+__device__ void slot_reset(uint64_t *gp, const int32_t *touched, SlotLds<B, V> &S)
 {
     __syncthreads();
     const int32_t nt = S.ntouch;
