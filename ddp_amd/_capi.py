@@ -63,7 +63,8 @@ class BatchOpts(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("expansions", c_int64), ("edges", c_int64), ("improvements", c_int64),
                 ("stale_pops", c_int64), ("slots", c_int32), ("block", c_int32), ("errors", c_int32),
-                ("kernel_ms", c_float), ("total_ms", c_float), ("pile_rebuilds", c_int64)]
+                ("kernel_ms", c_float), ("total_ms", c_float), ("pile_rebuilds", c_int64),
+                ("pile_refills", c_int64), ("pile_scanned", c_int64)]
 
 
 class Update(ctypes.Structure):

@@ -411,13 +411,15 @@ extern "C" int ddp_batch_wait(ddp_batch *b, ddp_batch_stats_t *st)
     b->last_ms = ms;
     if (st) {
         std::memset(st, 0, sizeof *st);
-        unsigned long long v[5];
-        DDP_HIP(ctx, hipMemcpy(v, b->stats, 40, hipMemcpyDeviceToHost));
+        unsigned long long v[7];
+        DDP_HIP(ctx, hipMemcpy(v, b->stats, sizeof v, hipMemcpyDeviceToHost));
         st->expansions = (int64_t)v[0];
         st->edges = (int64_t)v[1];
         st->improvements = (int64_t)v[2];
         st->stale_pops = (int64_t)v[3];
         st->pile_rebuilds = (int64_t)v[4];
+        st->pile_refills = (int64_t)v[5];
+        st->pile_scanned = (int64_t)v[6];
         st->slots = b->slots;
         st->block = b->B;
         st->kernel_ms = ms;

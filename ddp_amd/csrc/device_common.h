@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "ddp_internal.h"
+#include "isqrt.h"
 
 #define DDP_INF_F 0x7FFFFFFFu                      // Cx "infinity" (INT_MAX)
 #define DDP_GP_INF 0x7FFFFFFFFFFFFFFFull           // packed (f=INT_MAX, parent=-1)
@@ -27,22 +28,37 @@ __device__ __forceinline__ void st_u32(uint32_t *p, uint32_t v)
 }
 
 // ------------------------------------------------------------- heuristics
-__host__ __device__ inline uint32_t isqrt_u64(uint64_t x)
+
+// A heuristic value in two steps, so that a lane can issue the loads for
+// several nodes before the first value is computed: heur_load reads what h(v)
+// needs from HBM (the h array entry, or v's coordinates), heur_from finishes.
+// (No initialisers: a lane that skips heur_load never reads its HeurIn, and
+// leaving it undefined spares the compiler a merge that would wait on the load.)
+struct HeurIn {
+    int32_t a, b;
+};
+
+template <int HK>
+__device__ __forceinline__ HeurIn heur_load(const HeurView &H, int32_t v)
 {
-    // floor(sqrt(x)), exact: double estimate then integer fix-up.
-    uint64_t r = (uint64_t)sqrt((double)x);
-    while (r * r > x) --r;
-    while ((r + 1) * (r + 1) <= x) ++r;
-    return (uint32_t)r;
+    HeurIn in{0, 0};
+    if constexpr (HK == DDP_H_ARRAY) {
+        in.a = H.h[v];
+    } else if constexpr (HK == DDP_H_EUCLID) {
+        const int2 p = reinterpret_cast<const int2 *>(H.xy)[v];
+        in.a = p.x;
+        in.b = p.y;
+    }
+    return in;
 }
 
 template <int HK>
-__device__ __forceinline__ int32_t heur_eval(const HeurView &H, int32_t v)
+__device__ __forceinline__ int32_t heur_from(const HeurView &H, int32_t v, HeurIn in)
 {
     if constexpr (HK == DDP_H_ZERO) {
         return 0;
     } else if constexpr (HK == DDP_H_ARRAY) {
-        return H.h[v];
+        return in.a;
     } else if constexpr (HK == DDP_H_MANHATTAN) {
         int32_t r = v / H.width, c = v - r * H.width;
         int32_t dr = r - H.ty, dc = c - H.tx;
@@ -50,9 +66,15 @@ __device__ __forceinline__ int32_t heur_eval(const HeurView &H, int32_t v)
         dc = dc < 0 ? -dc : dc;
         return (dr + dc) * H.scale;
     } else {
-        int64_t dx = (int64_t)H.xy[2 * v] - H.tx, dy = (int64_t)H.xy[2 * v + 1] - H.ty;
+        const int64_t dx = (int64_t)in.a - H.tx, dy = (int64_t)in.b - H.ty;
         return (int32_t)isqrt_u64((uint64_t)(dx * dx + dy * dy)) * H.scale;
     }
+}
+
+template <int HK>
+__device__ __forceinline__ int32_t heur_eval(const HeurView &H, int32_t v)
+{
+    return heur_from<HK>(H, v, heur_load<HK>(H, v));
 }
 
 // ---------------------------------------------------------- relaxation
@@ -138,14 +160,18 @@ __device__ __forceinline__ bool heap_pop(const LaneHeap &H, int32_t &size, uint6
 }
 
 // ----------------------------------------------------- wave / block helpers
+// Wave-wide inclusive prefix sum with DPP row shifts and row broadcasts (no
+// LDS traffic): shifts by 1, 2, 4 and 8 scan each 16-lane row, then lane 15
+// of rows 0 and 2 is added into rows 1 and 3, and lane 31 into rows 2 and 3.
+// Lanes whose source is outside the row, or whose row is masked off, add 0.
 __device__ __forceinline__ int32_t wave_incl_scan(int32_t v)
 {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        int32_t o = __shfl_up(v, d, 64);
-        if (lane >= d) v += o;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return v;
 }
 
@@ -173,7 +199,7 @@ __device__ __forceinline__ int32_t block_incl_scan(int32_t v, int32_t *tmp, int3
 {
     int32_t x = wave_incl_scan(v);
     if constexpr (B == 64) {
-        total = __shfl(x, 63, 64);
+        total = __builtin_amdgcn_readlane(x, 63);
         return x;
     } else {
         const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -214,6 +240,31 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t mask)
 {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated append of up to R values per lane: one atomicAdd on *count
+// (LDS or global) reserves room for every selected slot of the wave; slot r
+// of a lane lands after the wave's selected slots r' < r and after slot r of
+// the lanes below it.  pos[r] = UINT32_MAX for unselected slots.  Every lane
+// of the wave must call it.
+template <int R>
+__device__ __forceinline__ void wave_append(const bool (&sel)[R], uint32_t *count, uint32_t (&pos)[R])
+{
+    uint64_t m[R];
+    uint32_t off[R], tot = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        m[r] = __ballot(sel[r]);
+        off[r] = tot;
+        tot += (uint32_t)__popcll(m[r]);
+    }
+    uint32_t base = 0;
+    if (tot) {
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(count, tot);
+        base = __builtin_amdgcn_readfirstlane(base);  // lane 0's value (every lane is active)
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) pos[r] = sel[r] ? base + off[r] + lane_rank(m[r]) : 0xFFFFFFFFu;
 }
 
 // On gfx950 __syncthreads() is a bare s_barrier: a wave's global stores may

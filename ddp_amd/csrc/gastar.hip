@@ -75,6 +75,22 @@ struct GridPush {
             pushmin = f < pushmin ? f : pushmin;
         }
     }
+    template <int R>
+    __device__ void touch_all(const bool (&)[R], const int32_t (&)[R]) {}
+    template <int R>
+    __device__ void all(const bool (&has)[R], const uint64_t (&key)[R])
+    {
+        uint32_t pos[R];
+        wave_append<R>(has, pushcnt, pos);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!has[r]) continue;
+            if (pos[r] < (uint32_t)pcap) buf[pos[r]] = key[r];
+            else err |= ERR_PBUF;
+            const uint32_t f = (uint32_t)(key[r] >> 32);
+            pushmin = f < pushmin ? f : pushmin;
+        }
+    }
 };
 
 template <int B, int HK>

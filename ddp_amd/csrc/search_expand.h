@@ -13,14 +13,21 @@
 
 #include "device_common.h"
 
-template <int B>
+template <int V>
 struct ExpandLds {
-    int32_t incl[B];   // inclusive prefix of degrees
-    int32_t rowb[B];   // row start
-    uint32_t gu[B];    // g(u) = f(u) - h(u), unsigned as GA_v2.cu:137
-    int32_t node[B];
+    static_assert(V <= 65536, "owner entries are 16-bit");
+    int32_t incl[V];     // inclusive prefix of degrees
+    int32_t rowb[V];     // row start
+    uint32_t gu[V];      // g(u) = f(u) - h(u), unsigned as GA_v2.cu:137
+    int32_t node[V];
     int32_t tmp[16];
+    uint16_t own[4 * V];  // edge slot of the current window -> virtual lane (a hint)
 };
+
+// Owner entries a lane writes per row and window: rows up to this long (every
+// row of a road or geometric graph) are located with one LDS read per edge;
+// the rest of a longer row falls back to a binary search over the prefix.
+constexpr int32_t kOwnRun = 32;
 
 // What expanding u needs besides u itself: its row and g(u).
 struct RowSpan {
@@ -43,6 +50,8 @@ __device__ __forceinline__ RowSpan row_span(const GraphView &G, const HeurView &
 struct ExpandCounters {
     uint64_t expanded = 0, edges = 0, improved = 0;
     uint64_t rebuilds = 0;  // pile engine: far piles rebuilt after an overflow (lane 0 counts)
+    uint64_t refills = 0;   // pile engine: near-ring refills from the far pile (lane 0 counts)
+    uint64_t scanned = 0;   // pile engine: far-pile entries read by refills
 };
 
 // Edges handled per lane and round (memory requests a lane keeps in flight
@@ -54,10 +63,13 @@ constexpr int expand_unroll()
     return B == 512 ? 2 : 4;
 }
 
-// Push policy interface:
+// Push policy interface (block-collective members: every lane calls them):
 //   void target(uint32_t f)                    -- t improved to f (t is never queued)
 //   void touch(int32_t c, uint64_t prev)       -- c's word was prev before the update
-//   void operator()(bool has, uint64_t key)    -- block-collective: all lanes call it
+//   void operator()(bool has, uint64_t key)    -- block-collective
+//   void touch_all<R>(first[R], c[R])          -- block-collective: touch() of R edges
+//                                                 whose previous f was infinite
+//   void all<R>(has[R], key[R])                -- block-collective: operator() of R edges
 // Each lane brings up to P nodes u[p] (or -1) and their row spans (row_span,
 // or the same values loaded earlier by the caller); node p of lane l is
 // "virtual lane" l*P + p of a B*P-wide expansion, so a wide round pays the
@@ -78,54 +90,101 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
         if (u[p] >= 0) C.expanded++;
     }
     int32_t total;
-    int32_t run = block_incl_scan<B>(lsum, L.tmp, total) - lsum;
+    const int32_t run0 = block_incl_scan<B>(lsum, L.tmp, total) - lsum;
+    int32_t ends[P];
+    {
+        int32_t run = run0;
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-        run += deg[p];
-        L.incl[lane * P + p] = run;
-        L.rowb[lane * P + p] = rs[p].b;
-        L.gu[lane * P + p] = rs[p].gu;
-        L.node[lane * P + p] = u[p];
+        for (int p = 0; p < P; ++p) {
+            run += deg[p];
+            ends[p] = run;
+            L.incl[lane * P + p] = run;
+            L.rowb[lane * P + p] = rs[p].b;
+            L.gu[lane * P + p] = rs[p].gu;
+            L.node[lane * P + p] = u[p];
+        }
     }
+    // Owner hints for the edge slots [base, base + B*R): each row writes its
+    // virtual lane into the slots of its first kOwnRun edges in the window.
+    // A hint is checked against the prefix before use, so stale entries from
+    // an earlier window only cost the binary search.
+    auto fill_owners = [&](int32_t base) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const int32_t st = ends[p] - deg[p];
+            const int32_t from = st > base ? st : base;
+            int32_t to = ends[p] < base + B * R ? ends[p] : base + B * R;
+            to = to < from + kOwnRun ? to : from + kOwnRun;
+            for (int32_t x = from; x < to; ++x) L.own[x - base] = (uint16_t)(lane * P + p);
+        }
+    };
+    fill_owners(0);
     __syncthreads();
     for (int32_t base = 0; base < total; base += B * R) {
+        if (base > 0) {  // every lane is done with the previous window's hints
+            __syncthreads();
+            fill_owners(base);
+            __syncthreads();
+        }
         int32_t c[R], jj[R];
         uint32_t wt[R];
-        // stage 1: locate each edge and load its column and weight
+        // stage 1: locate each edge (LDS only), then issue its column and
+        // weight loads for all R edges before any of them is used
+        int32_t pos[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int32_t e = base + r * B + lane;
-            c[r] = -1;
+            pos[r] = -1;
             jj[r] = 0;
-            wt[r] = 0;
             if (e < total) {
-                int32_t lo = 0, hi = V - 1;
-                while (lo < hi) {
-                    const int32_t mid = (lo + hi) >> 1;
-                    if (L.incl[mid] > e) hi = mid;
-                    else lo = mid + 1;
+                int32_t lo = L.own[e - base];
+                int32_t ex = 0;
+                if (lo < V) ex = lo ? L.incl[lo - 1] : 0;
+                if (!(lo < V && ex <= e && e < L.incl[lo])) {
+                    int32_t a = 0, hi = V - 1;
+                    while (a < hi) {
+                        const int32_t mid = (a + hi) >> 1;
+                        if (L.incl[mid] > e) hi = mid;
+                        else a = mid + 1;
+                    }
+                    lo = a;
+                    ex = lo ? L.incl[lo - 1] : 0;
                 }
-                const int32_t k = e - (lo ? L.incl[lo - 1] : 0);
                 jj[r] = lo;
-                const int32_t p = L.rowb[lo] + k;
-                c[r] = G.col[p];
-                wt[r] = G.w[p];
-                if (G.dver && G.dver[p] <= G.ver) c[r] = -1;  // deleted at or before this version
+                pos[r] = L.rowb[lo] + (e - ex);
                 C.edges++;
             }
         }
-        // stage 2: heuristic inputs and the current (f, parent) words
-        int32_t hc[R];
+        int32_t dv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            c[r] = -1;
+            wt[r] = 0;
+            dv[r] = INT32_MAX;
+            if (pos[r] >= 0) {
+                c[r] = G.col[pos[r]];
+                wt[r] = G.w[pos[r]];
+                if (G.dver) dv[r] = G.dver[pos[r]];
+            }
+        }
+        // stage 2: heuristic inputs and the current (f, parent) words, all
+        // loads issued before the first heuristic is computed.  hin, cur and
+        // got below are only written and read by lanes whose edge takes part:
+        // giving the others a value would make the compiler merge the two and
+        // wait for each load (or CAS) before issuing the next.
+        HeurIn hin[R];
         uint64_t cur[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            hc[r] = 0;
-            cur[r] = 0;
+            if (G.dver && dv[r] <= G.ver) c[r] = -1;  // deleted at or before this version
             if (c[r] >= 0) {  // negative id = deleted edge
-                hc[r] = heur_eval<HK>(H, c[r]);
+                hin[r] = heur_load<HK>(H, c[r]);
                 cur[r] = ld_gp(gp + c[r]);
             }
         }
+        int32_t hc[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) hc[r] = c[r] >= 0 ? heur_from<HK>(H, c[r], hin[r]) : 0;
         // stage 3: one CAS attempt per improving edge, all issued together
         uint64_t want[R], got[R];
         bool tried[R];
@@ -139,16 +198,22 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
                 tried[r] = (uint32_t)(cur[r] >> 32) > cand;
             }
         }
+        // Every load above has returned by now (the comparisons used them);
+        // saying so keeps the compiler from waiting again before each CAS.
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            got[r] = tried[r] ? (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
-                                                   (unsigned long long)want[r])
-                              : 0;
+            if (tried[r])
+                got[r] = (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
+                                             (unsigned long long)want[r]);
         // stage 4: settle contended CASes, then hand improvements to the push policy
+        bool has[R], first[R];
+        uint64_t key[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            bool has = false;
-            uint64_t key = 0;
+            has[r] = false;
+            first[r] = false;
+            key[r] = 0;
             if (tried[r]) {
                 const uint32_t cand = (uint32_t)(want[r] >> 32);
                 bool ok = got[r] == cur[r];
@@ -168,17 +233,18 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
                 }
                 if (ok) {
                     C.improved++;
-                    push.touch(c[r], prev);
+                    first[r] = (uint32_t)(prev >> 32) == DDP_INF_F;
                     if (c[r] == t) {
                         push.target(cand);
                     } else {
-                        has = true;
-                        key = ((uint64_t)cand << 32) | (uint32_t)c[r];
+                        has[r] = true;
+                        key[r] = ((uint64_t)cand << 32) | (uint32_t)c[r];
                     }
                 }
             }
-            push(has, key);
         }
+        push.template touch_all<R>(first, c);
+        push.template all<R>(has, key);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {

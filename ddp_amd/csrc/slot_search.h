@@ -48,6 +48,21 @@ struct SlotPush {
     {
         if (has && !heap_push(*H, *size, key)) err |= QERR_HEAP;
     }
+    template <int R>
+    __device__ void touch_all(const bool (&first)[R], const int32_t (&c)[R])
+    {
+        uint32_t pos[R];
+        wave_append<R>(first, (uint32_t *)s_ntouch, pos);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (first[r]) touched[pos[r]] = c[r];
+    }
+    template <int R>
+    __device__ void all(const bool (&has)[R], const uint64_t (&key)[R])
+    {
+#pragma unroll
+        for (int r = 0; r < R; ++r) (*this)(has[r], key[r]);
+    }
 };
 
 template <int HK>
@@ -193,6 +208,23 @@ __device__ __forceinline__ uint32_t near_reserve(bool want, uint32_t *tail, uint
     return (want && r < got) ? base + r : 0xFFFFFFFFu;
 }
 
+// Single-lane form of near_reserve's reservation: up to `want` slots of a
+// ring whose tail may run at most cap ahead of head.
+__device__ __forceinline__ void ring_reserve(uint32_t *tail, uint32_t head, uint32_t cap, uint32_t want,
+                                             uint32_t &base, uint32_t &got)
+{
+    uint32_t cur = __hip_atomic_load(tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (;;) {
+        const uint32_t used = cur - head;
+        got = used < cap ? (cap - used < want ? cap - used : want) : 0u;
+        if (got == 0) break;
+        if (__hip_atomic_compare_exchange_weak(tail, &cur, cur + got, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+            break;
+    }
+    base = cur;
+}
+
 __device__ __forceinline__ uint32_t far_reserve(bool want, uint32_t *count)
 {
     const uint64_t m = __ballot(want);
@@ -229,17 +261,67 @@ struct PilePush {
     // near candidates: the LDS ring, then the HBM ring, then the far pile
     __device__ void operator()(bool has, uint64_t key)
     {
-        const uint32_t f = (uint32_t)(key >> 32);
-        has = has && f < __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const bool nearc = has && f < S->thr;
-        const uint32_t li = near_reserve(nearc, &S->lring_tail, S->lring_head, P->lcap);
-        if (li != 0xFFFFFFFFu) P->lring[li & (P->lcap - 1)] = key;
-        const uint32_t ni = near_reserve(nearc && li == 0xFFFFFFFFu, &S->near_tail, S->near_head, P->ncap);
-        if (ni != 0xFFFFFFFFu) P->near[ni & (P->ncap - 1)] = key;
-        const uint32_t fi = far_reserve(has && li == 0xFFFFFFFFu && ni == 0xFFFFFFFFu, &S->far_n);
-        if (fi == 0xFFFFFFFFu) return;
-        if (fi < P->fcap) far[fi] = key;
-        else S->ovf = 1;  // dropped: the next refill rebuilds the far pile
+        const bool h[1] = {has};
+        const uint64_t k[1] = {key};
+        all<1>(h, k);
+    }
+    template <int R>
+    __device__ void touch_all(const bool (&first)[R], const int32_t (&c)[R])
+    {
+        uint32_t pos[R];
+        wave_append<R>(first, (uint32_t *)&S->ntouch, pos);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (first[r]) touched[pos[r]] = c[r];
+    }
+    // R candidates per lane, one reservation per level and wave: the wave's
+    // near candidates fill the LDS ring, then the HBM ring, and what does not
+    // fit joins the far candidates in the far pile.
+    template <int R>
+    __device__ void all(const bool (&has)[R], const uint64_t (&key)[R])
+    {
+        const uint32_t F = __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t thr = S->thr;
+        bool nr[R], fr[R];
+        uint64_t m[R];
+        uint32_t off[R], tot = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t f = (uint32_t)(key[r] >> 32);
+            const bool live = has[r] && f < F;
+            nr[r] = live && f < thr;
+            fr[r] = live && !nr[r];
+            m[r] = __ballot(nr[r]);
+            off[r] = tot;
+            tot += (uint32_t)__popcll(m[r]);
+        }
+        if (tot) {
+            uint32_t lb = 0, lg = 0, hb = 0, hg = 0;
+            if ((threadIdx.x & 63) == 0) {
+                if (P->lcap) ring_reserve(&S->lring_tail, S->lring_head, P->lcap, tot, lb, lg);
+                if (lg < tot) ring_reserve(&S->near_tail, S->near_head, P->ncap, tot - lg, hb, hg);
+            }
+            lb = __builtin_amdgcn_readfirstlane(lb);  // lane 0's values (every lane is active)
+            lg = __builtin_amdgcn_readfirstlane(lg);
+            hb = __builtin_amdgcn_readfirstlane(hb);
+            hg = __builtin_amdgcn_readfirstlane(hg);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (!nr[r]) continue;
+                const uint32_t q = off[r] + lane_rank(m[r]);
+                if (q < lg) P->lring[(lb + q) & (P->lcap - 1)] = key[r];
+                else if (q < lg + hg) P->near[(hb + q - lg) & (P->ncap - 1)] = key[r];
+                else fr[r] = true;  // both rings full
+            }
+        }
+        uint32_t pos[R];
+        wave_append<R>(fr, &S->far_n, pos);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!fr[r]) continue;
+            if (pos[r] < P->fcap) far[pos[r]] = key[r];
+            else S->ovf = 1;  // dropped: the next refill rebuilds the far pile
+        }
     }
 };
 
@@ -363,6 +445,8 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             }
             // ---- refill the near rings from the far pile
             const uint32_t nf = S.far_n < P.fcap ? S.far_n : P.fcap;
+            if (lane == 0) C.refills++;
+            C.scanned += 2 * ((nf > (uint32_t)lane) ? (nf - 1 - lane) / B + 1 : 0);
             uint32_t mymin = 0xFFFFFFFFu;
             for (uint32_t i = lane; i < nf; i += B) {
                 const uint64_t key = far[i];
@@ -507,12 +591,15 @@ template <int B>
 __device__ void slot_flush_stats(unsigned long long *stats, const ExpandCounters &C, uint64_t stale)
 {
     const uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
-                   v3 = wave_sum_u64(stale), v4 = wave_sum_u64(C.rebuilds);
+                   v3 = wave_sum_u64(stale), v4 = wave_sum_u64(C.rebuilds), v5 = wave_sum_u64(C.refills),
+                   v6 = wave_sum_u64(C.scanned);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(stats + 0, (unsigned long long)v0);
         atomicAdd(stats + 1, (unsigned long long)v1);
         atomicAdd(stats + 2, (unsigned long long)v2);
         atomicAdd(stats + 3, (unsigned long long)v3);
         if (v4) atomicAdd(stats + 4, (unsigned long long)v4);
+        if (v5) atomicAdd(stats + 5, (unsigned long long)v5);
+        if (v6) atomicAdd(stats + 6, (unsigned long long)v6);
     }
 }

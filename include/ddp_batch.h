@@ -47,6 +47,8 @@ typedef struct {
     float kernel_ms;    /* launch-to-finish device time (HIP events on the ctx stream) */
     float total_ms;
     int64_t pile_rebuilds;  /* pile engine: far piles rebuilt after an overflow */
+    int64_t pile_refills;   /* pile engine: near-ring refills from the far pile */
+    int64_t pile_scanned;   /* pile engine: far-pile entries read by those refills */
 } ddp_batch_stats_t;
 
 /* Plan: allocates the query slots ((f,parent) arrays, touched logs, heaps) once
