@@ -500,7 +500,7 @@ int ddp_heur_at(const ddp_graph *g, const HeurView &hv, int32_t v, int32_t *out)
     default: {
         int64_t dx = (int64_t)g->host_xy[2 * (size_t)v] - hv.tx;
         int64_t dy = (int64_t)g->host_xy[2 * (size_t)v + 1] - hv.ty;
-        *out = (int32_t)isqrt_u64((uint64_t)(dx * dx + dy * dy)) * hv.scale;
+        *out = (int32_t)isqrt_sum_sq(dx, dy) * hv.scale;
         return DDP_OK;
     }
     }

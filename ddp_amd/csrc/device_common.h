@@ -66,8 +66,7 @@ __device__ __forceinline__ int32_t heur_from(const HeurView &H, int32_t v, HeurI
         dc = dc < 0 ? -dc : dc;
         return (dr + dc) * H.scale;
     } else {
-        const int64_t dx = (int64_t)in.a - H.tx, dy = (int64_t)in.b - H.ty;
-        return (int32_t)isqrt_u64((uint64_t)(dx * dx + dy * dy)) * H.scale;
+        return (int32_t)isqrt_sum_sq((int64_t)in.a - H.tx, (int64_t)in.b - H.ty) * H.scale;
     }
 }
 
