@@ -21,7 +21,7 @@ struct ExpandLds {
     uint32_t gu[V];      // g(u) = f(u) - h(u), unsigned as GA_v2.cu:137
     int32_t node[V];
     int32_t tmp[16];
-    uint16_t own[4 * V];  // edge slot of the current window -> virtual lane (a hint)
+    uint16_t own[16 * V];  // edge slot of the round -> virtual lane (a hint)
 };
 
 // Owner entries a lane writes per row and window: rows up to this long (every
@@ -104,28 +104,23 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             L.node[lane * P + p] = u[p];
         }
     }
-    // Owner hints for the edge slots [base, base + B*R): each row writes its
-    // virtual lane into the slots of its first kOwnRun edges in the window.
-    // A hint is checked against the prefix before use, so stale entries from
-    // an earlier window only cost the binary search.
-    auto fill_owners = [&](int32_t base) {
+    // Owner hints, when the round's edges fit in four windows of B*R slots
+    // (16V): each row writes its virtual lane into the slots of its first
+    // kOwnRun edges.  A hint is checked against the prefix before use, so a
+    // stale one only costs the binary search.  Larger rounds (hub rows of
+    // power-law graphs) skip the hints: refilling them per window cost two
+    // barriers each and measured slower than searching.
+    const bool use_own = total <= 4 * B * R;
+    if (use_own) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const int32_t st = ends[p] - deg[p];
-            const int32_t from = st > base ? st : base;
-            int32_t to = ends[p] < base + B * R ? ends[p] : base + B * R;
-            to = to < from + kOwnRun ? to : from + kOwnRun;
-            for (int32_t x = from; x < to; ++x) L.own[x - base] = (uint16_t)(lane * P + p);
+            const int32_t to = ends[p] < st + kOwnRun ? ends[p] : st + kOwnRun;
+            for (int32_t x = st; x < to; ++x) L.own[x] = (uint16_t)(lane * P + p);
         }
-    };
-    fill_owners(0);
+    }
     __syncthreads();
     for (int32_t base = 0; base < total; base += B * R) {
-        if (base > 0) {  // every lane is done with the previous window's hints
-            __syncthreads();
-            fill_owners(base);
-            __syncthreads();
-        }
         int32_t c[R], jj[R];
         uint32_t wt[R];
         // stage 1: locate each edge (LDS only), then issue its column and
@@ -137,7 +132,7 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             pos[r] = -1;
             jj[r] = 0;
             if (e < total) {
-                int32_t lo = L.own[e - base];
+                int32_t lo = use_own ? L.own[e] : V;
                 int32_t ex = 0;
                 if (lo < V) ex = lo ? L.incl[lo - 1] : 0;
                 if (!(lo < V && ex <= e && e < L.incl[lo])) {

@@ -305,6 +305,9 @@ def test_batch_piles_parity(ctx, block):
     for i in range(24):
         dist, _ = pyoracle.dijkstra(mz.off, mz.col, mz.w, int(ms[i]), int(mt[i]))
         assert cost[i] == dist[mt[i]]
+    # a maze search runs through many windows: refills are counted, each reads
+    # the far pile twice (minimum pass, then the split)
+    assert st["pile_refills"] >= 1 and st["pile_scanned"] % 2 == 0
     dg.close()
 
 
