@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
     const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
     const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
     ExpandCounters C;
-    uint64_t stale = 0;
+    uint32_t stale = 0;
     for (;;) {
         if (lane == 0) s_q = (int32_t)atomicAdd(a.next, 1u);
         __syncthreads();
@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
             a.qerr[q] = S.err;
             a.qtime[2 * (int64_t)q + 1] = dev_clock_100mhz();
         }
+        if (__ballot(C.edges >= (1u << 30))) slot_flush_stats<B>(a.stats, C, stale);  // before 32 bits wrap
     }
     slot_flush_stats<B>(a.stats, C, stale);
 }

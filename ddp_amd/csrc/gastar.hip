@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
     int32_t rr = a.rr0;
     uint32_t err_local = 0, err_seen = 0;
     ExpandCounters C;
-    uint64_t stale = 0;
+    uint32_t stale = 0;
     int32_t it = 0;
     for (;; ++it) {
         // -- drain the previous iteration's pushes into this lane's queue
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
     const int lane = threadIdx.x;
     const LaneHeap H{s_heap + lane, a.home + lane, B, a.lcap, (int64_t)B, a.cap};
     ExpandCounters C;
-    uint64_t stale = 0;
+    uint32_t stale = 0;
     if constexpr (PILES) {
         const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
         slot_search_piles<B, HK>(a.g, a.h, a.gp, a.touched, P, a.N, a.s, a.t, a.maxit, a.delta0, S, C, stale);

@@ -47,11 +47,15 @@ __device__ __forceinline__ RowSpan row_span(const GraphView &G, const HeurView &
     return r;
 }
 
+// Per-lane statistics, 32-bit to spare registers.  Kernels add them to the
+// 64-bit device totals at exit, and slot kernels also whenever a wave's
+// edge count passes 2^30 (no counter grows faster than the edge count,
+// except rebuilds/refills/scanned, which lane 0 counts once per refill).
 struct ExpandCounters {
-    uint64_t expanded = 0, edges = 0, improved = 0;
-    uint64_t rebuilds = 0;  // pile engine: far piles rebuilt after an overflow (lane 0 counts)
-    uint64_t refills = 0;   // pile engine: near-ring refills from the far pile (lane 0 counts)
-    uint64_t scanned = 0;   // pile engine: far-pile entries read by refills
+    uint32_t expanded = 0, edges = 0, improved = 0;
+    uint32_t rebuilds = 0;  // pile engine: far piles rebuilt after an overflow (lane 0 counts)
+    uint32_t refills = 0;   // pile engine: near-ring refills from the far pile (lane 0 counts)
+    uint32_t scanned = 0;   // pile engine: far-pile entries read by refills
 };
 
 // Edges handled per lane and round (memory requests a lane keeps in flight

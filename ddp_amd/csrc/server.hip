@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
     const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
     const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
     ExpandCounters C;
-    uint64_t stale = 0;
+    uint32_t stale = 0;
     for (;;) {
         // ---- claim the next published request (bounded idle wait).  Idle
         // slots watch device words only; one of them at a time holds the
@@ -248,6 +248,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             cost = (int32_t)S.F;
             status = S.err;
             slot_reset(gp, touched, S);
+            if (__ballot(C.edges >= (1u << 30))) slot_flush_stats<B>(a.stats, C, stale);  // before 32 bits wrap
         }
         if (lane == 0) {
             Res *r = a.res + ((uint32_t)ticket & a.ring_mask);

@@ -83,7 +83,7 @@ __device__ __forceinline__ void set_target(HeurView &hv, int32_t t)
 template <int B, int HK>
 __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched, const LaneHeap &H,
                             uint64_t *s_heap, int32_t N, int32_t s, int32_t t, int32_t maxit, SlotLds<B> &S,
-                            ExpandCounters &C, uint64_t &stale)
+                            ExpandCounters &C, uint32_t &stale)
 {
     const int lane = threadIdx.x;
     if (lane == 0) {
@@ -360,7 +360,7 @@ constexpr int pile_entries_per_lane()
 template <int B, int HK, int W = pile_entries_per_lane<B>()>
 __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched,
                                   const PilePtrs &P, int32_t N, int32_t s, int32_t t, int32_t maxit,
-                                  uint32_t delta0, SlotLds<B, B * W> &S, ExpandCounters &C, uint64_t &stale)
+                                  uint32_t delta0, SlotLds<B, B * W> &S, ExpandCounters &C, uint32_t &stale)
 {
     constexpr uint32_t V = (uint32_t)B * W;
     const int lane = threadIdx.x;
@@ -452,8 +452,30 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             const uint32_t nf = S.far_n < P.fcap ? S.far_n : P.fcap;
             if (lane == 0) C.refills++;
             C.scanned += 2 * ((nf > (uint32_t)lane) ? (nf - 1 - lane) / B + 1 : 0);
+            // The first KC*B entries stay in registers between the two passes
+            // (loads issued together: entries, then their (f, parent) words);
+            // only a far pile longer than that is read twice.
+            constexpr int KC = 4;
+            uint64_t kc[KC], wv[KC];
+            bool lc[KC];
             uint32_t mymin = 0xFFFFFFFFu;
-            for (uint32_t i = lane; i < nf; i += B) {
+#pragma unroll
+            for (int k = 0; k < KC; ++k)
+                if ((uint32_t)k * B + lane < nf) kc[k] = far[(uint32_t)k * B + lane];
+#pragma unroll
+            for (int k = 0; k < KC; ++k)
+                if ((uint32_t)k * B + lane < nf && (uint32_t)(kc[k] >> 32) < F)
+                    wv[k] = ld_gp(gp + (uint32_t)kc[k]);
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+                lc[k] = false;
+                if ((uint32_t)k * B + lane < nf) {
+                    const uint32_t f = (uint32_t)(kc[k] >> 32);
+                    lc[k] = f < F && (uint32_t)(wv[k] >> 32) == f;  // else hopeless or stale
+                    if (lc[k]) mymin = f < mymin ? f : mymin;
+                }
+            }
+            for (uint32_t i = (uint32_t)KC * B + lane; i < nf; i += B) {
                 const uint64_t key = far[i];
                 const uint32_t f = (uint32_t)(key >> 32);
                 if (f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f) mymin = f < mymin ? f : mymin;
@@ -469,7 +491,23 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             }
             __syncthreads();
             const uint32_t thr = S.thr;
-            for (uint32_t i0 = 0; i0 < nf; i0 += B) {  // uniform trip count: the reservations are wave-wide
+            // second pass: near entries to the rings, the rest compacted into far2
+            auto place = [&](uint64_t key, bool live) {
+                const bool nearc = live && (uint32_t)(key >> 32) < thr;
+                const uint32_t li = near_reserve(nearc, &S.lring_tail, lh, P.lcap);
+                if (li != 0xFFFFFFFFu) P.lring[li & lmask] = key;
+                const uint32_t ni = near_reserve(nearc && li == 0xFFFFFFFFu, &S.near_tail, hh, P.ncap);
+                if (ni != 0xFFFFFFFFu) P.near[ni & nmask] = key;
+                const uint32_t fi = far_reserve(live && li == 0xFFFFFFFFu && ni == 0xFFFFFFFFu, &S.far_n2);
+                if (fi != 0xFFFFFFFFu) far2[fi] = key;  // fewer than nf <= fcap entries
+            };
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+                if ((uint32_t)k * B >= nf) break;  // uniform: the reservations are wave-wide
+                const bool in = (uint32_t)k * B + lane < nf;
+                place(in ? kc[k] : 0, in && lc[k]);
+            }
+            for (uint32_t i0 = (uint32_t)KC * B; i0 < nf; i0 += B) {  // uniform trip count
                 const uint32_t i = i0 + lane;
                 uint64_t key = 0;
                 bool live = false;
@@ -478,13 +516,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
                     const uint32_t f = (uint32_t)(key >> 32);
                     live = f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f;  // else hopeless or stale
                 }
-                const bool nearc = live && (uint32_t)(key >> 32) < thr;
-                const uint32_t li = near_reserve(nearc, &S.lring_tail, lh, P.lcap);
-                if (li != 0xFFFFFFFFu) P.lring[li & lmask] = key;
-                const uint32_t ni = near_reserve(nearc && li == 0xFFFFFFFFu, &S.near_tail, hh, P.ncap);
-                if (ni != 0xFFFFFFFFu) P.near[ni & nmask] = key;
-                const uint32_t fi = far_reserve(live && li == 0xFFFFFFFFu && ni == 0xFFFFFFFFu, &S.far_n2);
-                if (fi != 0xFFFFFFFFu) far2[fi] = key;  // fewer than nf <= fcap entries
+                place(key, live);
             }
             block_sync_global();
             if (lane == 0) S.far_n = S.far_n2;
@@ -593,7 +625,7 @@ __device__ void slot_reset(uint64_t *gp, const int32_t *touched, SlotLds<B, V> &
 }
 
 template <int B>
-__device__ void slot_flush_stats(unsigned long long *stats, const ExpandCounters &C, uint64_t stale)
+__device__ void slot_flush_stats(unsigned long long *stats, ExpandCounters &C, uint32_t &stale)
 {
     const uint64_t v0 = wave_sum_u64(C.expanded), v1 = wave_sum_u64(C.edges), v2 = wave_sum_u64(C.improved),
                    v3 = wave_sum_u64(stale), v4 = wave_sum_u64(C.rebuilds), v5 = wave_sum_u64(C.refills),
@@ -607,4 +639,6 @@ __device__ void slot_flush_stats(unsigned long long *stats, const ExpandCounters
         if (v5) atomicAdd(stats + 5, (unsigned long long)v5);
         if (v6) atomicAdd(stats + 6, (unsigned long long)v6);
     }
+    C = ExpandCounters{};
+    stale = 0;
 }
