@@ -311,6 +311,37 @@ def test_batch_piles_parity(ctx, block):
     dg.close()
 
 
+@pytest.mark.parametrize("block", [64, 256, 1024])
+def test_batch_power_law_hub_rows(ctx, block):
+    """R-MAT graph (hub rows of thousands of edges, Dijkstra order): rounds
+    whose edges span many windows and rows longer than the owner-map runs
+    take the binary-search path, smaller rounds the owner hints; both
+    engines, both numberings, costs equal to Dijkstra and paths real."""
+    hg = ddp_amd.HostGraph.rmat(13, 16 << 13, 0.57, 0.19, 0.19, 1, 100, 41)
+    src, dst = hg.hub_pairs(48, 42)
+    maxdeg = int(np.diff(np.append(hg.off, hg.E)).max())
+    assert maxdeg > 1000  # rows far longer than the 32-edge owner runs
+    want = []
+    for i in range(48):
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
+        want.append(int(dist[dst[i]]) if dist[dst[i]] >= 0 else ddp_amd.INF_COST)
+    for reorder in (False, True):
+        dg = ddp_amd.DeviceGraph(ctx, hg, reorder=reorder)
+        say("power law", block, reorder, "maxdeg", maxdeg)
+        cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, block=block,
+                                              path_cap=8192, engine=ddp_amd.ENGINE_PILES)
+        assert st["errors"] == 0
+        assert [int(c) for c in cost] == want
+        for i in range(0, 48, 7):
+            if want[i] != ddp_amd.INF_COST:
+                check_path(hg.off, hg.col, hg.w, paths[i], int(src[i]), int(dst[i]), want[i])
+        if block <= 256:
+            heaps, _, _ = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, block=block,
+                                              engine=ddp_amd.ENGINE_HEAPS)
+            assert [int(c) for c in heaps] == want
+        dg.close()
+
+
 def test_batch_piles_overflow_is_reported(ctx):
     """Queue memory too small for a query: the query is flagged, never
     answered wrongly; the others stay exact."""
