@@ -191,9 +191,11 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
     return v;
 }
 
-// Block-wide inclusive scan; `tmp` must hold B/64 ints of LDS.  Ends with a
-// barrier, so `tmp` may be reused right after.  Returns the block total too.
-template <int B>
+// Block-wide inclusive scan; `tmp` must hold B/64 ints of LDS.  Returns the
+// block total too.  With TAIL_SYNC it ends with a barrier, so `tmp` may be
+// reused right after; without, the caller must pass another barrier before
+// anything writes `tmp` again.
+template <int B, bool TAIL_SYNC = true>
 __device__ __forceinline__ int32_t block_incl_scan(int32_t v, int32_t *tmp, int32_t &total)
 {
     int32_t x = wave_incl_scan(v);
@@ -212,7 +214,7 @@ __device__ __forceinline__ int32_t block_incl_scan(int32_t v, int32_t *tmp, int3
             tot += s;
         }
         total = tot;
-        __syncthreads();
+        if constexpr (TAIL_SYNC) __syncthreads();
         return x + add;
     }
 }

@@ -78,7 +78,11 @@ constexpr int expand_unroll()
 // or the same values loaded earlier by the caller); node p of lane l is
 // "virtual lane" l*P + p of a B*P-wide expansion, so a wide round pays the
 // scan and barriers once for B*P rows.
-template <int B, int HK, int P, class Push>
+//
+// Without FINAL_SYNC the function returns without its closing barrier; the
+// caller must pass a barrier before the next expand_rows (or before reading
+// what the pushes wrote).
+template <int B, int HK, int P, class Push, bool FINAL_SYNC = true>
 __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
                                             const int32_t (&u)[P], const RowSpan (&rs)[P], ExpandLds<B * P> &L,
                                             ExpandCounters &C, Push &push)
@@ -94,7 +98,9 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
         if (u[p] >= 0) C.expanded++;
     }
     int32_t total;
-    const int32_t run0 = block_incl_scan<B>(lsum, L.tmp, total) - lsum;
+    // no barrier after the scan: the next write to L.tmp comes after the
+    // barrier that publishes L below
+    const int32_t run0 = block_incl_scan<B, false>(lsum, L.tmp, total) - lsum;
     int32_t ends[P];
     {
         int32_t run = run0;
@@ -289,7 +295,7 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             if (!__syncthreads_or(active)) break;
         }
     }
-    __syncthreads();
+    if constexpr (FINAL_SYNC) __syncthreads();
 }
 
 template <int B, int HK, class Push>

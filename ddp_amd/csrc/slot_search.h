@@ -569,9 +569,9 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             S.phase_n += n;
         }
         PilePush<B, B * W> push{&P, far, &S, touched};
-        expand_rows<B, HK, W>(G, hv, gp, t, u, rs, S.L, C, push);
+        expand_rows<B, HK, W, PilePush<B, B * W>, false>(G, hv, gp, t, u, rs, S.L, C, push);
         if (push.err) atomicOr(&S.err, push.err);
-        __syncthreads();
+        __syncthreads();  // the round's pushes and errors are visible to the next round
         if (S.err) break;
     }
     if (lane == 0) S.iters = it;
