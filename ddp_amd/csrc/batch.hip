@@ -353,6 +353,7 @@ static int batch_capture(ddp_batch *b)
     }
     BatchArgs a{};
     a.g = ddp_view(b->g);
+    a.g.relax_min = b->g->minw >= 1;
     a.h = b->hv;
     a.qs = b->qs;
     a.qt = b->qt;

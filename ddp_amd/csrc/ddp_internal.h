@@ -98,6 +98,12 @@ struct GraphView {
     // visible iff its link version <= ver and ldel[x] > ver.
     const int32_t *__restrict__ dver;
     int32_t ver;
+    // 1: relax with one 64-bit atomicMin on the packed (f, parent) word
+    // instead of a strict-decrease CAS loop.  Only for graphs whose weights
+    // are all >= 1 and that do not change during the search: a parent then
+    // always has a strictly smaller g, so parent chains stay acyclic even
+    // though an equal f may swap the parent (to the smaller id).
+    int32_t relax_min;
     const uint64_t *lhead;
     const uint64_t *lnext;
     const int32_t *lv;

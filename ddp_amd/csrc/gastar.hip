@@ -655,7 +655,9 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
         int32_t iters = 0;
         float kms = 0;
         if (P.wg) {
-            rc = ddp_ga_run_wg(ctx, ddp_view(g), hv, P, S, g->N, s, t, &F, &iters, &kms);
+            GraphView gv = ddp_view(g);
+            gv.relax_min = g->minw >= 1;
+            rc = ddp_ga_run_wg(ctx, gv, hv, P, S, g->N, s, t, &F, &iters, &kms);
         } else {
             if ((rc = ddp_ga_seed(ctx, P, S, s, (uint32_t)hs))) return rc;
             rc = ddp_ga_run(ctx, ddp_view(g), hv, P, S, t, DDP_INF_F, 0, &F, nullptr, &iters, &kms);

@@ -206,11 +206,21 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
         // Every load above has returned by now (the comparisons used them);
         // saying so keeps the compiler from waiting again before each CAS.
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        const bool by_min = G.relax_min != 0;  // uniform
+        if (by_min) {
+            // one atomicMin per improving edge: the smallest (f, parent) of
+            // the round wins with no retry round trips
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            if (tried[r])
-                got[r] = (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
-                                             (unsigned long long)want[r]);
+            for (int r = 0; r < R; ++r)
+                if (tried[r]) got[r] = (uint64_t)atomicMin((unsigned long long *)(gp + c[r]),
+                                                           (unsigned long long)want[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (tried[r])
+                    got[r] = (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
+                                                 (unsigned long long)want[r]);
+        }
         // stage 4: settle contended CASes, then hand improvements to the push policy
         bool has[R], first[R];
         uint64_t key[R];
@@ -221,9 +231,9 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             key[r] = 0;
             if (tried[r]) {
                 const uint32_t cand = (uint32_t)(want[r] >> 32);
-                bool ok = got[r] == cur[r];
-                uint64_t prev = cur[r];
-                if (!ok) {
+                bool ok = by_min ? (uint32_t)(got[r] >> 32) > cand : got[r] == cur[r];
+                uint64_t prev = by_min ? got[r] : cur[r];
+                if (!ok && !by_min) {
                     uint64_t now = got[r];
                     while ((uint32_t)(now >> 32) > cand) {
                         const uint64_t old = atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)now,

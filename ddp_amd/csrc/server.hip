@@ -39,6 +39,7 @@ struct Req {       // pinned, written by the host
     int32_t version;
     int32_t del_off, del_n;  // updates: (u,v) pairs in the arena
     int32_t ins_off, ins_n;  // updates: (u,v,w,entry) quads, sorted by u
+    int32_t relax_min;       // queries: every weight at this version is >= 1
 };
 
 struct Res {       // pinned, written by the device
@@ -211,6 +212,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         rq.del_n = sys_ld_i(&rp->del_n);
         rq.ins_off = sys_ld_i(&rp->ins_off);
         rq.ins_n = sys_ld_i(&rp->ins_n);
+        rq.relax_min = sys_ld_i(&rp->relax_min);
         const uint64_t tstart = dev_clock_100mhz();
         // ---- wait for the graph version this request needs (bounded)
         const uint32_t need = rq.type == DDP_REQ_UPDATE ? (uint32_t)rq.version - 1 : (uint32_t)rq.version;
@@ -241,6 +243,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         } else if (!status) {
             GraphView G = a.g;
             G.ver = rq.version;
+            G.relax_min = a.g.relax_min && rq.relax_min;
             if constexpr (ENG == SENG_PILES)
                 slot_search_piles<B, HK>(G, a.h, gp, touched, P, N, rq.s, rq.t, a.maxit, a.delta0, S, C, stale);
             else
@@ -343,6 +346,7 @@ struct ddp_server {
     // bookkeeping
     uint32_t submitted = 0, collected = 0;
     int32_t version = 0;
+    int32_t zero_w_version = INT32_MAX;  // first version with an inserted weight-0 edge
     std::vector<uint64_t> tags;
     std::vector<double> submit_ms;
     std::vector<int32_t> types;
@@ -374,6 +378,7 @@ static int server_launch(ddp_server *sv)
     ddp_ctx *ctx = sv->ctx;
     ServeArgs a{};
     a.g = ddp_view(sv->g);
+    a.g.relax_min = sv->g->minw >= 1;
     a.g.dver = sv->dver;
     a.g.lhead = sv->lhead;
     a.g.lnext = sv->lnext;
@@ -575,6 +580,7 @@ extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uin
     r.s = ddp_to_dev(sv->g, s);
     r.t = ddp_to_dev(sv->g, t);
     r.version = sv->version;
+    r.relax_min = sv->version < sv->zero_w_version;  // the base graph's own minimum is in a.g
     publish(sv, r, tag, DDP_REQ_QUERY);
     return server_ensure(sv);
 }
@@ -596,6 +602,7 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
             dels.push_back(v);
         } else {
             ins.push_back({u, v, (int32_t)l.w});
+            if (l.w == 0) sv->zero_w_version = std::min(sv->zero_w_version, sv->version + 1);
         }
     }
     DDP_CHECK(sv->log_used + (int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_NOMEM,
