@@ -178,38 +178,8 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
 // search, so results stay exact; only a far pile smaller than the touched set
 // is an error (QERR_HEAP).
 
-// Wave-aggregated reservations (one LDS atomic per wave).  near_reserve
-// takes ring slots only while the ring has room (the tail never runs more
-// than ncap ahead of head) and may grant fewer than asked; far_reserve always
-// grants.  Both return the lane's index, or UINT32_MAX when it got none.
-// Every lane of the wave must call them.
-__device__ __forceinline__ uint32_t near_reserve(bool want, uint32_t *tail, uint32_t head, uint32_t ncap)
-{
-    const uint64_t m = __ballot(want);
-    if (m == 0) return 0xFFFFFFFFu;
-    const int leader = __builtin_ctzll(m);
-    uint32_t base = 0, got = 0;
-    if ((int)(threadIdx.x & 63) == leader) {
-        const uint32_t cnt = (uint32_t)__popcll(m);
-        uint32_t cur = __hip_atomic_load(tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        for (;;) {
-            const uint32_t used = cur - head;
-            got = used < ncap ? (ncap - used < cnt ? ncap - used : cnt) : 0u;
-            if (got == 0) break;
-            if (__hip_atomic_compare_exchange_weak(tail, &cur, cur + got, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_WORKGROUP))
-                break;
-        }
-        base = cur;
-    }
-    base = __shfl(base, leader, 64);
-    got = __shfl(got, leader, 64);
-    const uint32_t r = lane_rank(m);
-    return (want && r < got) ? base + r : 0xFFFFFFFFu;
-}
-
-// Single-lane form of near_reserve's reservation: up to `want` slots of a
-// ring whose tail may run at most cap ahead of head.
+// Ring reservation by one lane for its wave: up to `want` slots of a ring
+// whose tail may never run more than cap ahead of head (it may grant fewer).
 __device__ __forceinline__ void ring_reserve(uint32_t *tail, uint32_t head, uint32_t cap, uint32_t want,
                                              uint32_t &base, uint32_t &got)
 {
@@ -225,6 +195,8 @@ __device__ __forceinline__ void ring_reserve(uint32_t *tail, uint32_t head, uint
     base = cur;
 }
 
+// Wave-aggregated append (one LDS atomic per wave); returns the lane's index,
+// or UINT32_MAX if it wanted none.  Every lane of the wave must call it.
 __device__ __forceinline__ uint32_t far_reserve(bool want, uint32_t *count)
 {
     const uint64_t m = __ballot(want);
@@ -244,6 +216,58 @@ struct PilePtrs {
     uint64_t *lring = nullptr;  // lcap entries (power of two, or 0), ring in LDS
     uint32_t lcap = 0;
 };
+
+// Places up to R entries per lane with one reservation per level and wave:
+// entries with f < thr fill the LDS ring (if lds_ok), then the HBM ring;
+// what does not fit, and entries with f >= thr, are appended to fdst at
+// *fcount.  Returns true if some entry found fdst full and was dropped.
+// Every lane of the wave must call it.
+template <int R>
+__device__ __forceinline__ bool pile_place(const PilePtrs &P, uint32_t thr, bool lds_ok, uint32_t lhead,
+                                           uint32_t hhead, uint32_t *ltail, uint32_t *htail, uint64_t *fdst,
+                                           uint32_t *fcount, const bool (&put)[R], const uint64_t (&key)[R])
+{
+    bool nr[R], fr[R];
+    uint64_t m[R];
+    uint32_t off[R], tot = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        nr[r] = put[r] && (uint32_t)(key[r] >> 32) < thr;
+        fr[r] = put[r] && !nr[r];
+        m[r] = __ballot(nr[r]);
+        off[r] = tot;
+        tot += (uint32_t)__popcll(m[r]);
+    }
+    if (tot) {
+        uint32_t lb = 0, lg = 0, hb = 0, hg = 0;
+        if ((threadIdx.x & 63) == 0) {
+            if (lds_ok) ring_reserve(ltail, lhead, P.lcap, tot, lb, lg);
+            if (lg < tot) ring_reserve(htail, hhead, P.ncap, tot - lg, hb, hg);
+        }
+        lb = __builtin_amdgcn_readfirstlane(lb);  // lane 0's values (every lane is active)
+        lg = __builtin_amdgcn_readfirstlane(lg);
+        hb = __builtin_amdgcn_readfirstlane(hb);
+        hg = __builtin_amdgcn_readfirstlane(hg);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (!nr[r]) continue;
+            const uint32_t q = off[r] + lane_rank(m[r]);
+            if (q < lg) P.lring[(lb + q) & (P.lcap - 1)] = key[r];
+            else if (q < lg + hg) P.near[(hb + q - lg) & (P.ncap - 1)] = key[r];
+            else fr[r] = true;  // both rings full
+        }
+    }
+    uint32_t pos[R];
+    wave_append<R>(fr, fcount, pos);
+    bool dropped = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!fr[r]) continue;
+        if (pos[r] < P.fcap) fdst[pos[r]] = key[r];
+        else dropped = true;
+    }
+    return dropped;
+}
 
 template <int B, int V>
 struct PilePush {
@@ -274,59 +298,22 @@ struct PilePush {
         for (int r = 0; r < R; ++r)
             if (first[r]) touched[pos[r]] = c[r];
     }
-    // R candidates per lane, one reservation per level and wave: the wave's
-    // near candidates fill the LDS ring, then the HBM ring, and what does not
-    // fit joins the far candidates in the far pile.  The two rings act as one
-    // FIFO: while the HBM ring holds entries (pushed when the LDS ring was
-    // full), new ones queue behind them there, since rounds drain the LDS ring
+    // R candidates per lane; see pile_place.  The two rings act as one FIFO:
+    // while the HBM ring holds entries (pushed when the LDS ring was full),
+    // new ones queue behind them there, since rounds drain the LDS ring
     // first.  (Letting later pushes overtake through the LDS ring made a
     // Dijkstra-order search on R-MAT expand 65% more nodes.)
     template <int R>
     __device__ void all(const bool (&has)[R], const uint64_t (&key)[R])
     {
         const uint32_t F = __hip_atomic_load(&S->F, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t thr = S->thr;
+        bool put[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) put[r] = has[r] && (uint32_t)(key[r] >> 32) < F;
         const bool lds_ok = P->lcap && S->near_tail == S->near_head;
-        bool nr[R], fr[R];
-        uint64_t m[R];
-        uint32_t off[R], tot = 0;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint32_t f = (uint32_t)(key[r] >> 32);
-            const bool live = has[r] && f < F;
-            nr[r] = live && f < thr;
-            fr[r] = live && !nr[r];
-            m[r] = __ballot(nr[r]);
-            off[r] = tot;
-            tot += (uint32_t)__popcll(m[r]);
-        }
-        if (tot) {
-            uint32_t lb = 0, lg = 0, hb = 0, hg = 0;
-            if ((threadIdx.x & 63) == 0) {
-                if (lds_ok) ring_reserve(&S->lring_tail, S->lring_head, P->lcap, tot, lb, lg);
-                if (lg < tot) ring_reserve(&S->near_tail, S->near_head, P->ncap, tot - lg, hb, hg);
-            }
-            lb = __builtin_amdgcn_readfirstlane(lb);  // lane 0's values (every lane is active)
-            lg = __builtin_amdgcn_readfirstlane(lg);
-            hb = __builtin_amdgcn_readfirstlane(hb);
-            hg = __builtin_amdgcn_readfirstlane(hg);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (!nr[r]) continue;
-                const uint32_t q = off[r] + lane_rank(m[r]);
-                if (q < lg) P->lring[(lb + q) & (P->lcap - 1)] = key[r];
-                else if (q < lg + hg) P->near[(hb + q - lg) & (P->ncap - 1)] = key[r];
-                else fr[r] = true;  // both rings full
-            }
-        }
-        uint32_t pos[R];
-        wave_append<R>(fr, &S->far_n, pos);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (!fr[r]) continue;
-            if (pos[r] < P->fcap) far[pos[r]] = key[r];
-            else S->ovf = 1;  // dropped: the next refill rebuilds the far pile
-        }
+        if (pile_place<R>(*P, S->thr, lds_ok, S->lring_head, S->near_head, &S->lring_tail, &S->near_tail, far,
+                          &S->far_n, put, key))
+            S->ovf = 1;  // dropped: the next refill rebuilds the far pile
     }
 };
 
@@ -452,33 +439,40 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             const uint32_t nf = S.far_n < P.fcap ? S.far_n : P.fcap;
             if (lane == 0) C.refills++;
             C.scanned += 2 * ((nf > (uint32_t)lane) ? (nf - 1 - lane) / B + 1 : 0);
-            // The first KC*B entries stay in registers between the two passes
-            // (loads issued together: entries, then their (f, parent) words);
-            // only a far pile longer than that is read twice.
-            constexpr int KC = 4;
-            uint64_t kc[KC], wv[KC];
-            bool lc[KC];
-            uint32_t mymin = 0xFFFFFFFFu;
+            // The far pile is read in chunks of KC*B entries, each chunk's
+            // loads issued together (entries, then their (f, parent) words),
+            // once for the minimum and again for the split.  (Keeping the
+            // first chunk in registers between the passes pushed the
+            // 1,024-lane kernel into spills.)
+            constexpr int KC = 2;
+            auto load_chunk = [&](uint32_t base, uint64_t (&k)[KC], bool (&live)[KC]) {
+                uint64_t wv[KC];
 #pragma unroll
-            for (int k = 0; k < KC; ++k)
-                if ((uint32_t)k * B + lane < nf) kc[k] = far[(uint32_t)k * B + lane];
+                for (int j = 0; j < KC; ++j)
+                    if (base + (uint32_t)j * B + lane < nf) k[j] = far[base + (uint32_t)j * B + lane];
 #pragma unroll
-            for (int k = 0; k < KC; ++k)
-                if ((uint32_t)k * B + lane < nf && (uint32_t)(kc[k] >> 32) < F)
-                    wv[k] = ld_gp(gp + (uint32_t)kc[k]);
+                for (int j = 0; j < KC; ++j)
+                    if (base + (uint32_t)j * B + lane < nf && (uint32_t)(k[j] >> 32) < F)
+                        wv[j] = ld_gp(gp + (uint32_t)k[j]);
 #pragma unroll
-            for (int k = 0; k < KC; ++k) {
-                lc[k] = false;
-                if ((uint32_t)k * B + lane < nf) {
-                    const uint32_t f = (uint32_t)(kc[k] >> 32);
-                    lc[k] = f < F && (uint32_t)(wv[k] >> 32) == f;  // else hopeless or stale
-                    if (lc[k]) mymin = f < mymin ? f : mymin;
+                for (int j = 0; j < KC; ++j) {
+                    live[j] = false;
+                    if (base + (uint32_t)j * B + lane < nf) {
+                        const uint32_t f = (uint32_t)(k[j] >> 32);
+                        live[j] = f < F && (uint32_t)(wv[j] >> 32) == f;  // else hopeless or stale
+                    } else {
+                        k[j] = 0;
+                    }
                 }
-            }
-            for (uint32_t i = (uint32_t)KC * B + lane; i < nf; i += B) {
-                const uint64_t key = far[i];
-                const uint32_t f = (uint32_t)(key >> 32);
-                if (f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f) mymin = f < mymin ? f : mymin;
+            };
+            uint32_t mymin = 0xFFFFFFFFu;
+            for (uint32_t base = 0; base < nf; base += (uint32_t)KC * B) {
+                uint64_t k2[KC];
+                bool l2[KC];
+                load_chunk(base, k2, l2);
+#pragma unroll
+                for (int j = 0; j < KC; ++j)
+                    if (l2[j]) mymin = (uint32_t)(k2[j] >> 32) < mymin ? (uint32_t)(k2[j] >> 32) : mymin;
             }
             const uint32_t fmin = block_min_u32<B>(mymin, S.red);
             if (fmin >= F) break;  // nothing open can improve t (also: far pile empty)
@@ -491,32 +485,14 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             }
             __syncthreads();
             const uint32_t thr = S.thr;
-            // second pass: near entries to the rings, the rest compacted into far2
-            auto place = [&](uint64_t key, bool live) {
-                const bool nearc = live && (uint32_t)(key >> 32) < thr;
-                const uint32_t li = near_reserve(nearc, &S.lring_tail, lh, P.lcap);
-                if (li != 0xFFFFFFFFu) P.lring[li & lmask] = key;
-                const uint32_t ni = near_reserve(nearc && li == 0xFFFFFFFFu, &S.near_tail, hh, P.ncap);
-                if (ni != 0xFFFFFFFFu) P.near[ni & nmask] = key;
-                const uint32_t fi = far_reserve(live && li == 0xFFFFFFFFu && ni == 0xFFFFFFFFu, &S.far_n2);
-                if (fi != 0xFFFFFFFFu) far2[fi] = key;  // fewer than nf <= fcap entries
-            };
-#pragma unroll
-            for (int k = 0; k < KC; ++k) {
-                if ((uint32_t)k * B >= nf) break;  // uniform: the reservations are wave-wide
-                const bool in = (uint32_t)k * B + lane < nf;
-                place(in ? kc[k] : 0, in && lc[k]);
-            }
-            for (uint32_t i0 = (uint32_t)KC * B; i0 < nf; i0 += B) {  // uniform trip count
-                const uint32_t i = i0 + lane;
-                uint64_t key = 0;
-                bool live = false;
-                if (i < nf) {
-                    key = far[i];
-                    const uint32_t f = (uint32_t)(key >> 32);
-                    live = f < F && (uint32_t)(ld_gp(gp + (uint32_t)key) >> 32) == f;  // else hopeless or stale
-                }
-                place(key, live);
+            // second pass: near entries to the rings (both empty here), the rest
+            // compacted into far2 (fewer than nf <= fcap entries: nothing drops)
+            for (uint32_t base = 0; base < nf; base += (uint32_t)KC * B) {  // uniform trips
+                uint64_t k2[KC];
+                bool l2[KC];
+                load_chunk(base, k2, l2);
+                (void)pile_place<KC>(P, thr, P.lcap != 0, lh, hh, &S.lring_tail, &S.near_tail, far2, &S.far_n2, l2,
+                                     k2);
             }
             block_sync_global();
             if (lane == 0) S.far_n = S.far_n2;
