@@ -130,6 +130,7 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
         }
     }
     __syncthreads();
+    int32_t prow = 0;  // row of this lane's previous edge (edges are visited in increasing order)
     for (int32_t base = 0; base < total; base += B * R) {
         int32_t c[R], jj[R];
         uint32_t wt[R];
@@ -142,11 +143,15 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
             pos[r] = -1;
             jj[r] = 0;
             if (e < total) {
-                int32_t lo = use_own ? L.own[e] : V;
+                // the owner hint, else the row of this lane's previous edge
+                // (a lane's edges only move forward, so that row or a later
+                // one holds e: hub rows keep it for many windows), else a
+                // binary search from that row on
+                int32_t lo = use_own ? L.own[e] : prow;
                 int32_t ex = 0;
                 if (lo < V) ex = lo ? L.incl[lo - 1] : 0;
                 if (!(lo < V && ex <= e && e < L.incl[lo])) {
-                    int32_t a = 0, hi = V - 1;
+                    int32_t a = prow, hi = V - 1;
                     while (a < hi) {
                         const int32_t mid = (a + hi) >> 1;
                         if (L.incl[mid] > e) hi = mid;
@@ -155,6 +160,7 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
                     lo = a;
                     ex = lo ? L.incl[lo - 1] : 0;
                 }
+                prow = lo;
                 jj[r] = lo;
                 pos[r] = L.rowb[lo] + (e - ex);
                 C.edges++;
