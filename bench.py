@@ -504,8 +504,9 @@ def run_c5(args):
                           "slots": bst["slots"]})
     res.update({"p50_ms": lat_at["p50_ms"], "p99_ms": lat_at["p99_ms"], "latency_at_qps": lat_at["offered_qps"],
                 "sweep": sweep, "batched_queries_per_s": round(len(src) / (bst["kernel_ms"] * 1e-3), 2),
-                "one_at_a_time": {"queries_per_s": round(1000.0 / float(np.mean(lat1)), 3),
-                                  "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": 8192}})
+                "one_at_a_time": ({"queries_per_s": round(1000.0 / float(np.mean(lat1)), 3),
+                                   "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": 8192}
+                                  if lat1 else None)})
     ctx.close()
     return res
 
