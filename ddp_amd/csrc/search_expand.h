@@ -5,10 +5,10 @@
 // walked R*B edges per round, so lanes read consecutive CSR entries and a long
 // row is split across the whole block instead of serialising one lane (the
 // reference gives each thread its own row: GA_star/GA_v2.cu:109-167).
-// Within a round every lane owns R edges and issues their loads stage by
+// Within a window every lane owns R edges and issues their loads stage by
 // stage (row entries, then heuristic inputs and (f, parent) words, then the
-// CAS attempts), so R independent memory round trips overlap instead of
-// running back to back.
+// CAS or atomicMin attempts), so R independent memory round trips overlap
+// instead of running back to back.
 #pragma once
 
 #include "device_common.h"
@@ -58,9 +58,10 @@ struct ExpandCounters {
     uint32_t scanned = 0;   // pile engine: far-pile entries read by refills
 };
 
-// Edges handled per lane and round (memory requests a lane keeps in flight
-// per stage).  Four where registers allow it without lowering occupancy: the
-// 1,024-lane pile kernel needs 110 VGPRs at four (101 at two; eight spills).
+// Edges handled per lane and window (memory requests a lane keeps in flight
+// per stage).  Four where registers allow it without lowering occupancy (the
+// 1,024-lane pile kernel fits 128 VGPRs at four); two and three measured no
+// faster on the C4 batch.
 template <int B>
 constexpr int expand_unroll()
 {

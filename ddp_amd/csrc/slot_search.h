@@ -153,8 +153,9 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
 // ---------------------------------------------------------------- piles
 //
 // A second queue discipline for one-slot searches: no heaps.  Open entries
-// (f << 32 | node) sit in two piles in HBM: a "near" ring holding the entries
-// with f below a threshold and a "far" pile with the rest.  Each round the
+// (f << 32 | node) sit in two piles: a "near" ring holding the entries with f
+// below a threshold (its front in LDS, the rest in HBM) and a "far" pile in
+// HBM with the rest.  Each round the
 // workgroup takes up to B near entries at once (one per lane, a coalesced
 // read), drops stale ones (f no longer the node's word) and those with
 // f >= F(t), and expands the rest cooperatively; an improvement goes to the
@@ -337,8 +338,9 @@ constexpr int pile_entries_per_lane()
 // on return).  delta0 is the first window width.  W entries per lane per round.
 //
 // The near ring has two levels: P.lring in LDS (P.lcap entries) and P.near in
-// HBM for what does not fit.  Rounds take from the LDS ring while it holds
-// entries; an ordinary round then ends with a plain barrier, since everything
+// HBM for what does not fit, kept in FIFO order together (see PilePush).
+// Rounds take from the LDS ring while it holds entries; an ordinary round then
+// ends with a plain barrier, since everything
 // the next round reads is in LDS or was written by atomics that completed.
 // Reading HBM-resident entries (the HBM ring, the far pile, the touched log)
 // is always preceded by a barrier at which every wave has drained its stores.
