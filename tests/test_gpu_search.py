@@ -388,3 +388,47 @@ def test_batch_piles_rebuild_after_overflow(ctx):
     assert st["errors"] == 0 and st["pile_rebuilds"] >= 1
     assert out["cost"].tolist() == [3, 2]
     assert out["paths"][0].tolist() == [0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("engine,block", [(ddp_amd.ENGINE_PILES, 64), (ddp_amd.ENGINE_PILES, 1024),
+                                          (ddp_amd.ENGINE_HEAPS, 64)])
+def test_batch_edge_cases(ctx, engine, block):
+    """Batches with s == t, unreachable targets, out-of-range ids (flagged,
+    not answered), a one-node graph without edges, and more queries than
+    slots, answered twice from the same upload."""
+    one = ddp_amd.HostGraph(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint32))
+    d1 = ddp_amd.DeviceGraph(ctx, one)
+    b = ddp_amd.Batch(ctx, d1, 2, heur=ddp_amd.H_ZERO, block=block, engine=engine, path_cap=8)
+    b.upload(np.array([0, 0], np.int32), np.array([0, 0], np.int32))
+    b.launch()
+    b.wait()
+    out = b.download(paths=True)
+    b.close()
+    d1.close()
+    assert list(out["cost"]) == [0, 0] and list(out["qerr"]) == [0, 0]
+    assert [list(p) for p in out["paths"]] == [[0], [0]]
+    # chain 0 -> 1 -> 2 plus isolated node 3
+    off = np.array([0, 1, 2, 2], np.int32)
+    col = np.array([1, 2], np.int32)
+    w = np.array([5, 7], np.uint32)
+    hg = ddp_amd.HostGraph(off, col, w)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src = np.array([0, 2, 0, 3, 1, -1, 0, 0] * 40, np.int32)
+    dst = np.array([2, 0, 3, 3, 1, 2, 7, 1] * 40, np.int32)
+    want = [12, ddp_amd.INF_COST, ddp_amd.INF_COST, 0, 0, None, None, 5] * 40
+    b = ddp_amd.Batch(ctx, dg, len(src), heur=ddp_amd.H_ZERO, block=block, engine=engine, path_cap=8, slots=4)
+    b.upload(src, dst)
+    for rep in range(2):
+        b.launch()
+        st = b.wait()
+        out = b.download(paths=True)
+        say("edge cases", engine, block, rep, st["errors"])
+        assert st["errors"] == 80  # the two out-of-range queries of each group of eight
+        for i, wv in enumerate(want):
+            if wv is None:
+                assert out["qerr"][i] & 4, i
+            else:
+                assert out["qerr"][i] == 0 and out["cost"][i] == wv, (i, out["cost"][i], wv)
+        assert list(out["paths"][0]) == [0, 1, 2] and len(out["paths"][1]) == 0
+    b.close()
+    dg.close()
