@@ -28,6 +28,7 @@ struct SlotLds {
     uint32_t thr, delta;            // near window [.., thr) and its width
     uint32_t phase_n;               // near entries taken since the last refill
     uint32_t ovf;                   // a push found both piles full and was dropped
+    uint32_t no_early;              // the far pile holds nothing near the window: no early refill
 };
 
 template <int B>
@@ -160,10 +161,10 @@ __device__ void slot_search(const GraphView &G, HeurView hv, uint64_t *gp, int32
 // read), drops stale ones (f no longer the node's word) and those with
 // f >= F(t), and expands the rest cooperatively; an improvement goes to the
 // near ring when its f is below the threshold, else to the far pile.  When the
-// near ring runs dry, one pass over the far pile finds the smallest live f,
-// the threshold moves to it plus a window width, and a second pass moves the
-// entries under it to the near ring and compacts the rest into the other far
-// buffer.  The search ends when both piles are empty or every live far entry
+// near ring runs dry (or thin, kThinRing, while the next window adjoins), one
+// pass over the far pile finds the smallest live f, the threshold moves to it
+// plus a window width, and a second pass moves the entries under it to the
+// near ring and compacts the rest into the other far buffer.  The search ends when both piles are empty or every live far entry
 // has f >= F(t).  This is a label-correcting search (a node can be expanded
 // again after a later improvement, as GA*'s K queues also allow); it returns
 // the optimal cost for an admissible heuristic because an entry is only
@@ -346,6 +347,13 @@ constexpr int pile_entries_per_lane()
 // is always preceded by a barrier at which every wave has drained its stores.
 // The ring heads are uniform registers, mirrored in LDS for the pushes'
 // capacity checks.
+// A near ring holding fewer than V / kThinRing entries is refilled early
+// (when the next window adjoins the current one).  Swept on the C4 batch:
+// 32 and 64 were fastest (64 expands 3% more nodes than no early refills,
+// 32 19%); 16 and 4 ran the window ahead of f order; 128 helped less and
+// 256 (three or fewer entries) refilled so often that it was slower.
+constexpr uint32_t kThinRing = 64;
+
 template <int B, int HK, int W = pile_entries_per_lane<B>()>
 __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched,
                                   const PilePtrs &P, int32_t N, int32_t s, int32_t t, int32_t maxit,
@@ -366,6 +374,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
         S.thr = 0;
         S.phase_n = 0;
         S.ovf = 0;
+        S.no_early = 0;
     }
     __syncthreads();
     bool run = s >= 0 && s < N && t >= 0 && t < N;
@@ -406,7 +415,14 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             block_sync_global();  // HBM entries ahead: every wave's stores have landed
             avail = S.near_tail - hh;
         }
-        if (avail == 0) {
+        // Early refill: a near ring this thin holds stragglers of the window
+        // while the far pile waits; move the next window in beside them.
+        // (Not while a rebuild is pending: it re-queues every touched node
+        // under F(t), and with the near ring drained F(t) prunes far more.)
+        const bool early = avail > 0 && (lt - lh) + (S.near_tail - hh) < V / kThinRing && S.far_n > 0 && !S.ovf &&
+                           !S.no_early;
+        if (early && from_lds) block_sync_global();  // the far pile is read below
+        if (avail == 0 || early) {
             if (S.ovf) {
                 // ---- rebuild the far pile from the touched log (entries were dropped)
                 const int32_t nt = S.ntouch;
@@ -477,23 +493,43 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
                     if (l2[j]) mymin = (uint32_t)(k2[j] >> 32) < mymin ? (uint32_t)(k2[j] >> 32) : mymin;
             }
             const uint32_t fmin = block_min_u32<B>(mymin, S.red);
-            if (fmin >= F) break;  // nothing open can improve t (also: far pile empty)
+            if (fmin >= F) {  // nothing in the far pile can improve t (also: far pile empty)
+                if (!early) break;
+                if (lane == 0) S.far_n = 0;  // F only falls: those entries are hopeless for good
+                __syncthreads();
+                continue;
+            }
+            // An early refill only takes the next window when it adjoins the
+            // current one; entries further out would run ahead of f order (a
+            // maze's detours).  Otherwise wait for the ring to drain.
+            if (early && fmin >= sat_add_u32(S.thr, S.delta)) {
+                if (lane == 0) S.no_early = 1;
+                __syncthreads();
+                continue;
+            }
             if (lane == 0) {
-                if (S.phase_n < V) S.delta = sat_add_u32(S.delta, S.delta);
-                else if (S.phase_n > 8 * V && S.delta > 1) S.delta >>= 1;
-                S.thr = sat_add_u32(fmin, S.delta);
-                S.phase_n = 0;
+                S.no_early = 0;
+                if (!early) {
+                    if (S.phase_n < V) S.delta = sat_add_u32(S.delta, S.delta);
+                    else if (S.phase_n > 8 * V && S.delta > 1) S.delta >>= 1;
+                    S.phase_n = 0;
+                }
+                const uint32_t nt = sat_add_u32(fmin, S.delta);
+                S.thr = nt > S.thr ? nt : S.thr;
                 S.far_n2 = 0;
             }
             __syncthreads();
             const uint32_t thr = S.thr;
+            // FIFO with what the rings still hold (early refills): the LDS ring
+            // takes entries only while the HBM ring is empty
+            const bool lds_ok = P.lcap != 0 && S.near_tail == hh;
             // second pass: near entries to the rings (both empty here), the rest
             // compacted into far2 (fewer than nf <= fcap entries: nothing drops)
             for (uint32_t base = 0; base < nf; base += (uint32_t)KC * B) {  // uniform trips
                 uint64_t k2[KC];
                 bool l2[KC];
                 load_chunk(base, k2, l2);
-                (void)pile_place<KC>(P, thr, P.lcap != 0, lh, hh, &S.lring_tail, &S.near_tail, far2, &S.far_n2, l2,
+                (void)pile_place<KC>(P, thr, lds_ok, lh, hh, &S.lring_tail, &S.near_tail, far2, &S.far_n2, l2,
                                      k2);
             }
             block_sync_global();
