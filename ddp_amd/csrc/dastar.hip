@@ -37,11 +37,30 @@
 namespace {
 
 using clk = std::chrono::steady_clock;
+
+// Stable sort of (node, entry) pairs by node id in [0, n): LSD radix with
+// 11-bit digits.  An update batch holds thousands of pairs; this is several
+// times faster than a comparison sort on them and keeps equal nodes in input
+// order, which the insert lists rely on.
+void sort_by_node(std::vector<std::pair<int32_t, int32_t>> &v, int32_t n)
+{
+    if (v.size() < 2) return;
+    std::vector<std::pair<int32_t, int32_t>> tmp(v.size());
+    int bits = 1;
+    while (bits < 31 && (1ll << bits) < (int64_t)n) ++bits;
+    for (int sh = 0; sh < bits; sh += 11) {
+        uint32_t cnt[2049] = {0};
+        for (const auto &p : v) cnt[(((uint32_t)p.first >> sh) & 2047u) + 1]++;
+        for (int i = 0; i < 2048; ++i) cnt[i + 1] += cnt[i];
+        for (const auto &p : v) tmp[cnt[((uint32_t)p.first >> sh) & 2047u]++] = p;
+        v.swap(tmp);
+    }
+}
 float ms_since(clk::time_point t0) { return std::chrono::duration<float, std::milli>(clk::now() - t0).count(); }
 
 enum : int { RC_NEXT = 0, RC_MARKED = 1, RC_ROOTS = 2, RC_REPULLED = 3, RC_PUSHES = 4, RC_WORDS = 16 };
 
-enum : int { RC_DEL_BASE = 5, RC_DEL_LOG = 6 };
+enum : int { RC_DEL_BASE = 5, RC_DEL_LOG = 6, RC_WALKED = 7 };
 
 // The log of inserted edges: entry x is u -> v with weight w, live while
 // del[x] == INT32_MAX.  head/next links pack (entry << 32 | 0); the end of a
@@ -119,6 +138,18 @@ __global__ void log_append(const int32_t *grp, int32_t n, uint64_t *head, uint64
     }
 }
 
+// New log entries at + k from the uploaded (u, v, w) triples, live (del =
+// INT32_MAX) until a later batch deletes them.
+__global__ void log_fill(const int32_t *ins, int32_t n, InsLog L, int64_t at)
+{
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    L.u[at + k] = ins[3 * k];
+    L.v[at + k] = ins[3 * k + 1];
+    L.w[at + k] = (uint32_t)ins[3 * k + 2];
+    L.del[at + k] = INT32_MAX;
+}
+
 __global__ void fill_i32(int32_t *p, int64_t n, int32_t v)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -142,12 +173,10 @@ __device__ __forceinline__ void claim_child(int32_t y, int32_t x, const uint64_t
     marked[atomicAdd(rctrl + RC_MARKED, 1u)] = y;
 }
 
-__global__ void subtree_level(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch,
-                              const int32_t *cur, int32_t ncur, int32_t *nxt, int32_t *marked, uint32_t *rctrl)
+__device__ __forceinline__ void claim_children(const GraphView &G, const InsLog &L, int32_t x, const uint64_t *gp,
+                                               int32_t *mark, int32_t epoch, int32_t *nxt, int32_t *marked,
+                                               uint32_t *rctrl)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ncur) return;
-    const int32_t x = cur[i];
     for (int32_t j = G.off[x]; j < G.off[x + 1]; ++j) {
         const int32_t y = G.col[j];
         if (y >= 0) claim_child(y, x, gp, mark, epoch, nxt, marked, rctrl);
@@ -157,6 +186,53 @@ __global__ void subtree_level(GraphView G, InsLog L, const uint64_t *gp, int32_t
         link = L.fnext[e];
         if (L.del[e] == INT32_MAX) claim_child(L.v[e], x, gp, mark, epoch, nxt, marked, rctrl);
     }
+}
+
+__global__ void subtree_level(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch,
+                              const int32_t *cur, int32_t ncur, int32_t *nxt, int32_t *marked, uint32_t *rctrl)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncur) claim_children(G, L, cur[i], gp, mark, epoch, nxt, marked, rctrl);
+}
+
+// The narrow levels of the subtree walk in one workgroup, so that they cost
+// no launch and host round trip each.  Starts from the rctrl[RC_NEXT] nodes in
+// cur and runs level after level while a level holds at most `wide` nodes.
+// On exit rctrl[RC_NEXT] is the size of the level left to walk (0 when done),
+// and rctrl[RC_WALKED] the number of levels walked: when it is odd, that level
+// sits in the buffer passed as nxt.  Every node is marked at most once per
+// epoch, so the loop ends after at most N levels.
+constexpr int kWalkBlock = 1024;
+
+__global__ void __launch_bounds__(kWalkBlock)
+    subtree_walk(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch, int32_t *cur,
+                 int32_t *nxt, int32_t *marked, uint32_t *rctrl, int32_t wide)
+{
+    __shared__ int32_t s_n;
+    if (threadIdx.x == 0) s_n = (int32_t)__hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    int32_t ncur = s_n;
+    uint32_t walked = 0;
+    while (ncur > 0 && ncur <= wide) {
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(rctrl + RC_NEXT, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+        }
+        __syncthreads();
+        for (int32_t i = threadIdx.x; i < ncur; i += kWalkBlock)
+            claim_children(G, L, cur[i], gp, mark, epoch, nxt, marked, rctrl);
+        __threadfence();  // nxt entries and counter updates, before the level ends
+        __syncthreads();
+        if (threadIdx.x == 0)
+            s_n = (int32_t)__hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        ncur = s_n;
+        int32_t *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        ++walked;
+    }
+    if (threadIdx.x == 0) rctrl[RC_WALKED] = walked;
 }
 
 struct RepairPush {
@@ -563,11 +639,8 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
             rg.emplace_back(l.v, x);
         }
     }
-    auto by_key = [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
-        return a.first < b.first;
-    };
-    std::stable_sort(fg.begin(), fg.end(), by_key);
-    std::stable_sort(rg.begin(), rg.end(), by_key);
+    sort_by_node(fg, N);
+    sort_by_node(rg, N);
     int rc = log_reserve(d, d->log_used + S.inserts);
     if (rc) return rc;
     // device work list: [pairs | ins | fg | rg]
@@ -592,24 +665,11 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
                            d->g->col, d->g->rcol, (const int32_t *)d->g->roff, (const int32_t *)d->d_reid, d->L,
                            (const uint64_t *)d->S.gp, d->d_mark, d->epoch, d->d_front, d->d_marked, d->d_rctrl);
     if (nins) {
-        std::vector<int32_t> nu(nins), nv(nins), nd(nins, INT32_MAX);
-        std::vector<uint32_t> nw(nins);
-        for (int32_t k = 0; k < nins; ++k) {
-            nu[k] = ins[3 * k];
-            nv[k] = ins[3 * k + 1];
-            nw[k] = (uint32_t)ins[3 * k + 2];
-        }
-        const size_t at = (size_t)d->log_used;
-        DDP_HIP(ctx, hipMemcpyAsync(d->L.u + at, nu.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
-        DDP_HIP(ctx, hipMemcpyAsync(d->L.v + at, nv.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
-        DDP_HIP(ctx, hipMemcpyAsync(d->L.w + at, nw.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
-        DDP_HIP(ctx, hipMemcpyAsync(d->L.del + at, nd.data(), (size_t)nins * 4, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(log_fill, grid(nins), dim3(T), 0, ctx->stream, w_ins, nins, d->L, (int64_t)d->log_used);
         hipLaunchKernelGGL(log_append, grid(nins), dim3(T), 0, ctx->stream, w_fg, nins, d->L.fhead, d->L.fnext,
                            d->L.ftail);
         hipLaunchKernelGGL(log_append, grid(nins), dim3(T), 0, ctx->stream, w_rg, nins, d->L.rhead, d->L.rnext,
                            d->L.rtail);
-        // the host staging vectors must outlive the copies
-        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
     DDP_HIP(ctx, hipGetLastError());
     DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -632,14 +692,23 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
         int32_t *cur = d->d_front, *nxt = d->d_front + N;
         int32_t ncur = (int32_t)rctrl[RC_NEXT];
         S.roots = (int32_t)rctrl[RC_ROOTS];
+        // Narrow levels (up to kWalkWide nodes) run inside one subtree_walk
+        // launch; a wider level takes a launch of its own over the whole grid.
+        constexpr int32_t kWalkWide = 8 * kWalkBlock;
         while (ncur > 0) {
-            DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl + RC_NEXT, 0, 4, ctx->stream));
-            hipLaunchKernelGGL(subtree_level, grid(ncur), dim3(T), 0, ctx->stream, F, d->L, d->S.gp, d->d_mark,
-                               d->epoch, cur, ncur, nxt, d->d_marked, d->d_rctrl);
+            if (ncur <= kWalkWide) {
+                hipLaunchKernelGGL(subtree_walk, dim3(1), dim3(kWalkBlock), 0, ctx->stream, F, d->L, d->S.gp,
+                                   d->d_mark, d->epoch, cur, nxt, d->d_marked, d->d_rctrl, kWalkWide);
+            } else {
+                DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl + RC_NEXT, 0, 4, ctx->stream));
+                hipLaunchKernelGGL(subtree_level, grid(ncur), dim3(T), 0, ctx->stream, F, d->L, d->S.gp, d->d_mark,
+                                   d->epoch, cur, ncur, nxt, d->d_marked, d->d_rctrl);
+            }
             DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
             DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            const bool walked = ncur <= kWalkWide;
             ncur = (int32_t)rctrl[RC_NEXT];
-            std::swap(cur, nxt);
+            if (!walked || (rctrl[RC_WALKED] & 1u)) std::swap(cur, nxt);
         }
         S.invalidated = (int32_t)rctrl[RC_MARKED];
         // ---- 4. re-pull invalidated nodes, 5. relax inserted edges

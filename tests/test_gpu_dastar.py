@@ -156,3 +156,40 @@ def test_alternative_paths_loop(ctx):
             assert cost == (int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST), (s, t, k)
             if cost != ddp_amd.INF_COST:
                 assert path[0] == s and path[-1] == t and path_cost(*cur, path) == cost
+
+
+def test_dastar_wide_and_narrow_subtree_levels(ctx):
+    """Deleting s->a invalidates a's whole subtree: a, a level of 20,000
+    leaves (wider than one workgroup's walk takes, so it gets a launch of its
+    own) and a 50-node chain below one leaf (narrow levels, walked in one
+    launch).  The leaf is then re-pulled through a costlier detour; a second
+    batch puts s->a back."""
+    nl, nc = 20000, 50
+    a, leaf0, chain0 = 1, 2, 2 + nl
+    b = chain0 + nc
+    n = b + 1
+    t = chain0 + nc - 1
+    rows = [[] for _ in range(n)]
+    rows[0] = [(a, 1), (b, 10)]
+    rows[a] = [(leaf0 + i, 1) for i in range(nl)]
+    rows[leaf0] = [(chain0, 1)]
+    for k in range(nc - 1):
+        rows[chain0 + k] = [(chain0 + k + 1, 1)]
+    rows[b] = [(leaf0, 1)]
+    off = np.cumsum([0] + [len(r) for r in rows[:-1]]).astype(np.int32)
+    col = np.array([v for r in rows for v, _ in r], np.int32)
+    w = np.array([x for r in rows for _, x in r], np.uint32)
+    hg = ddp_amd.HostGraph(off, col, w)
+    da = ddp_amd.DAStar(ctx, hg, 0, t, h=None, K=1024)
+    assert da.cost == 2 + nc
+    cost, st = da.apply([(0, 0, a, 0)], mode=ddp_amd.DAStar.REPAIR)
+    say("cut", cost, st)
+    assert st["invalidated"] == 1 + nl + nc
+    assert cost == 10 + 1 + nc
+    p = da.path().tolist()
+    assert p[:3] == [0, b, leaf0] and p[-1] == t and len(p) == nc + 3
+    cost, st = da.apply([(1, 0, a, 1)], mode=ddp_amd.DAStar.REPAIR)
+    say("restored", cost, st)
+    assert cost == 2 + nc
+    assert da.path().tolist()[:3] == [0, a, leaf0]
+    da.close()
