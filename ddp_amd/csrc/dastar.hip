@@ -162,29 +162,47 @@ __global__ void fill_u64(uint64_t *p, int64_t n, uint64_t v)
         p[i] = v;
 }
 
-// One level of the parent-subtree walk: children y of frontier node x are the
-// live out-neighbours whose parent is x.
-__device__ __forceinline__ void claim_child(int32_t y, int32_t x, const uint64_t *gp, int32_t *mark, int32_t epoch,
-                                            int32_t *nxt, int32_t *marked, uint32_t *rctrl)
+// y is a child of a frontier node: marks it once per epoch and lists it;
+// nxt[k - base] receives the child that drew k from rctrl[RC_NEXT].
+__device__ __forceinline__ void claim(int32_t y, int32_t *mark, int32_t epoch, int32_t *nxt, uint32_t base,
+                                      int32_t *marked, uint32_t *rctrl)
 {
-    if ((uint32_t)ld_gp(gp + y) != (uint32_t)x) return;
     if (atomicMax(mark + y, epoch) == epoch) return;
-    nxt[atomicAdd(rctrl + RC_NEXT, 1u)] = y;
+    nxt[atomicAdd(rctrl + RC_NEXT, 1u) - base] = y;
     marked[atomicAdd(rctrl + RC_MARKED, 1u)] = y;
 }
 
-__device__ __forceinline__ void claim_children(const GraphView &G, const InsLog &L, int32_t x, const uint64_t *gp,
-                                               int32_t *mark, int32_t epoch, int32_t *nxt, int32_t *marked,
-                                               uint32_t *rctrl)
+__device__ __forceinline__ void claim_child(int32_t y, int32_t x, const uint64_t *gp, int32_t *mark, int32_t epoch,
+                                            int32_t *nxt, uint32_t base, int32_t *marked, uint32_t *rctrl)
 {
-    for (int32_t j = G.off[x]; j < G.off[x + 1]; ++j) {
-        const int32_t y = G.col[j];
-        if (y >= 0) claim_child(y, x, gp, mark, epoch, nxt, marked, rctrl);
+    if ((uint32_t)ld_gp(gp + y) == (uint32_t)x) claim(y, mark, epoch, nxt, base, marked, rctrl);
+}
+
+// One node of a level of the parent-subtree walk: the children y of frontier
+// node x are its live out-neighbours whose parent is x.
+__device__ __forceinline__ void claim_children(const GraphView &G, const InsLog &L, int32_t x, const uint64_t *gp,
+                                               int32_t *mark, int32_t epoch, int32_t *nxt, uint32_t base,
+                                               int32_t *marked, uint32_t *rctrl)
+{
+    // the row in groups of 8: all column loads, then all parent-word loads,
+    // then the claims (one round trip per stage instead of two per edge)
+    constexpr int kG = 8;
+    const int32_t rb = G.off[x], re = G.off[x + 1];
+    for (int32_t j0 = rb; j0 < re; j0 += kG) {
+        int32_t y[kG];
+        uint32_t par[kG];
+#pragma unroll
+        for (int k = 0; k < kG; ++k) y[k] = j0 + k < re ? G.col[j0 + k] : -1;
+#pragma unroll
+        for (int k = 0; k < kG; ++k) par[k] = y[k] >= 0 ? (uint32_t)ld_gp(gp + y[k]) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < kG; ++k)
+            if (y[k] >= 0 && par[k] == (uint32_t)x) claim(y[k], mark, epoch, nxt, base, marked, rctrl);
     }
     for (uint64_t link = L.fhead[x]; !link_end(link);) {
         const int32_t e = (int32_t)(link >> 32);
         link = L.fnext[e];
-        if (L.del[e] == INT32_MAX) claim_child(L.v[e], x, gp, mark, epoch, nxt, marked, rctrl);
+        if (L.del[e] == INT32_MAX) claim_child(L.v[e], x, gp, mark, epoch, nxt, base, marked, rctrl);
     }
 }
 
@@ -192,47 +210,50 @@ __global__ void subtree_level(GraphView G, InsLog L, const uint64_t *gp, int32_t
                               const int32_t *cur, int32_t ncur, int32_t *nxt, int32_t *marked, uint32_t *rctrl)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncur) claim_children(G, L, cur[i], gp, mark, epoch, nxt, marked, rctrl);
+    if (i < ncur) claim_children(G, L, cur[i], gp, mark, epoch, nxt, 0u, marked, rctrl);
 }
 
 // The narrow levels of the subtree walk in one workgroup, so that they cost
 // no launch and host round trip each.  Starts from the rctrl[RC_NEXT] nodes in
 // cur and runs level after level while a level holds at most `wide` nodes.
-// On exit rctrl[RC_NEXT] is the size of the level left to walk (0 when done),
-// and rctrl[RC_WALKED] the number of levels walked: when it is odd, that level
-// sits in the buffer passed as nxt.  Every node is marked at most once per
-// epoch, so the loop ends after at most N levels.
+// rctrl[RC_NEXT] keeps counting across levels (a level's children take the
+// draws from its base on), so no reset has to be ordered against the next
+// level's draws, and the barriers order everything else: one workgroup, no
+// device-wide fence.  On exit rctrl[RC_NEXT] is the size of the level left to
+// walk (0 when done), and rctrl[RC_WALKED] the number of levels walked: when it
+// is odd, that level sits in the buffer passed as nxt.  Every node is marked at
+// most once per epoch, so the loop ends after at most N levels and the count
+// stays below 2N.
 constexpr int kWalkBlock = 1024;
 
 __global__ void __launch_bounds__(kWalkBlock)
     subtree_walk(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch, int32_t *cur,
                  int32_t *nxt, int32_t *marked, uint32_t *rctrl, int32_t wide)
 {
-    __shared__ int32_t s_n;
-    if (threadIdx.x == 0) s_n = (int32_t)__hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = __hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    int32_t ncur = s_n;
+    uint32_t base = s_n;  // draws so far; the current level holds ncur nodes
+    int32_t ncur = (int32_t)base;
     uint32_t walked = 0;
     while (ncur > 0 && ncur <= wide) {
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(rctrl + RC_NEXT, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-        }
-        __syncthreads();
         for (int32_t i = threadIdx.x; i < ncur; i += kWalkBlock)
-            claim_children(G, L, cur[i], gp, mark, epoch, nxt, marked, rctrl);
-        __threadfence();  // nxt entries and counter updates, before the level ends
+            claim_children(G, L, cur[i], gp, mark, epoch, nxt, base, marked, rctrl);
+        __syncthreads();  // every draw has returned and every nxt entry is stored
+        if (threadIdx.x == 0) s_n = __hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
-        if (threadIdx.x == 0)
-            s_n = (int32_t)__hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        ncur = s_n;
+        const uint32_t total = s_n;
+        ncur = (int32_t)(total - base);
+        base = total;
         int32_t *tmp = cur;
         cur = nxt;
         nxt = tmp;
-        ++walked;
+        ++walked;  // thread 0 writes s_n again only after the next level's first barrier
     }
-    if (threadIdx.x == 0) rctrl[RC_WALKED] = walked;
+    if (threadIdx.x == 0) {
+        rctrl[RC_NEXT] = (uint32_t)ncur;
+        rctrl[RC_WALKED] = walked;
+    }
 }
 
 struct RepairPush {
