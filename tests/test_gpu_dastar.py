@@ -193,3 +193,38 @@ def test_dastar_wide_and_narrow_subtree_levels(ctx):
     assert cost == 2 + nc
     assert da.path().tolist()[:3] == [0, a, leaf0]
     da.close()
+
+
+def test_dastar_many_roots_at_once(ctx):
+    """One batch deletes 9,000 tree edges s->m_i: the first two levels
+    (9,000 roots, then their 9,000 leaves) are wider than one workgroup's walk
+    takes, the chain below leaf 0 is narrow.  The leaves are re-pulled
+    through x (s->x weighs 5), the m_i become unreachable."""
+    m = 9000
+    nc = 40
+    x = 1
+    m0, leaf0, chain0 = 2, 2 + m, 2 + 2 * m
+    n = chain0 + nc
+    t = n - 1
+    rows = [[] for _ in range(n)]
+    rows[0] = [(x, 5)] + [(m0 + i, 1) for i in range(m)]
+    rows[x] = [(leaf0 + i, 1) for i in range(m)]
+    for i in range(m):
+        rows[m0 + i] = [(leaf0 + i, 1)]
+    rows[leaf0] = [(chain0, 1)]
+    for k in range(nc - 1):
+        rows[chain0 + k] = [(chain0 + k + 1, 1)]
+    off = np.cumsum([0] + [len(r) for r in rows[:-1]]).astype(np.int32)
+    col = np.array([v for r in rows for v, _ in r], np.int32)
+    w = np.array([c for r in rows for _, c in r], np.uint32)
+    hg = ddp_amd.HostGraph(off, col, w)
+    da = ddp_amd.DAStar(ctx, hg, 0, t, h=None, K=1024)
+    assert da.cost == 2 + nc
+    cost, st = da.apply([(0, 0, m0 + i, 0) for i in range(m)], mode=ddp_amd.DAStar.REPAIR)
+    say("cut", cost, st)
+    assert st["roots"] == m
+    assert st["invalidated"] == 2 * m + nc
+    assert cost == 5 + 1 + nc
+    p = da.path().tolist()
+    assert p[:3] == [0, x, leaf0] and p[-1] == t and len(p) == nc + 3
+    da.close()
