@@ -432,3 +432,55 @@ def test_batch_edge_cases(ctx, engine, block):
         assert list(out["paths"][0]) == [0, 1, 2] and len(out["paths"][1]) == 0
     b.close()
     dg.close()
+
+
+def test_batch_config4_full_size(ctx):
+    """The C4 batch at full size (2,000,000-node geometric graph, 4,096 queries
+    of log-uniform separation, Euclidean heuristic, relabelled device graph):
+    - every path is a chain of real edges from s to t whose weights sum to
+      the returned cost (vectorised over all 4,096 paths);
+    - the pile engine (1,024-lane slots) and the heap engine (512-lane slots)
+      return the same 4,096 costs;
+    - 64 uniformly drawn queries match the oracle's restatement of
+      A_star_cpu.cpp and 8 match a plain Dijkstra."""
+    hg = ddp_amd.HostGraph.geometric(2000000, 10, 100000, 4)
+    src, dst = hg.geo_pairs(4096, 100, 100000, 4)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=True)
+    say("c4 full: piles")
+    cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=1024, path_cap=4096,
+                                          engine=ddp_amd.ENGINE_PILES)
+    assert st["errors"] == 0
+    say("c4 full: heaps")
+    heaps, _, st_h = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=512,
+                                         engine=ddp_amd.ENGINE_HEAPS)
+    assert st_h["errors"] == 0
+    assert np.array_equal(cost, heaps)
+    dg.close()
+    say("c4 full: paths")
+    N = hg.N
+    o = np.append(hg.off, hg.E).astype(np.int64)
+    u = np.repeat(np.arange(N, dtype=np.int64), np.diff(o))
+    key = u * N + hg.col.astype(np.int64)
+    order = np.lexsort((hg.w, key))  # by (u, v), the lightest parallel edge first
+    ks, ws = key[order], hg.w[order].astype(np.int64)
+    found = cost != ddp_amd.INF_COST
+    say("c4 full: reachable", int(found.sum()), "of", len(src))
+    assert found.mean() > 0.95  # the graph's giant component holds nearly every node
+    assert all(len(paths[i]) == 0 for i in np.flatnonzero(~found))
+    live = np.flatnonzero(found)
+    assert all(paths[i][0] == src[i] and paths[i][-1] == dst[i] for i in live)
+    pid = np.concatenate([np.full(len(paths[i]) - 1, i, np.int64) for i in live])
+    hop = np.concatenate([paths[i][:-1].astype(np.int64) * N + paths[i][1:] for i in live])
+    at = np.minimum(np.searchsorted(ks, hop), len(ks) - 1)
+    assert (ks[at] == hop).all(), "a path step is not an edge"
+    sums = np.bincount(pid, weights=ws[at], minlength=len(paths)).astype(np.int64)
+    assert np.array_equal(sums[live], cost[live].astype(np.int64))
+    say("c4 full: oracle sample")
+    rng = np.random.default_rng(12)
+    pick = np.sort(rng.choice(len(src), size=64, replace=False))
+    got, _, _ = pyoracle.astar_batch_euclid(hg.off, hg.col, hg.w, hg.xy, src[pick], dst[pick], 8)
+    assert np.array_equal(got, cost[pick])
+    for i in pick[:8]:
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
+        want = int(dist[dst[i]]) if dist[dst[i]] >= 0 else ddp_amd.INF_COST
+        assert cost[i] == want, (i, cost[i], want)
