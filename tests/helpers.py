@@ -25,8 +25,11 @@ def full_offsets(off, E):
     return np.append(np.asarray(off, np.int64), E)
 
 
-def edge_min_weight(off, col, w, a, b):
-    o = full_offsets(off, len(col))
+def edge_min_weight(off, col, w, a, b, o=None):
+    """Weight of the cheapest a -> b edge, None if there is none.  o: the
+    full offsets (N + 1 entries), when the caller already has them."""
+    if o is None:
+        o = full_offsets(off, len(col))
     best = None
     for j in range(o[a], o[a + 1]):
         if col[j] == b and (best is None or w[j] < best):
@@ -37,9 +40,10 @@ def edge_min_weight(off, col, w, a, b):
 def path_cost(off, col, w, path):
     """Sum over consecutive nodes of the cheapest parallel edge; None if an
     edge is missing."""
+    o = full_offsets(off, len(col))
     total = 0
     for a, b in zip(path[:-1], path[1:]):
-        x = edge_min_weight(off, col, w, int(a), int(b))
+        x = edge_min_weight(off, col, w, int(a), int(b), o)
         if x is None:
             return None
         total += x

@@ -228,3 +228,50 @@ def test_dastar_many_roots_at_once(ctx):
     p = da.path().tolist()
     assert p[:3] == [0, x, leaf0] and p[-1] == t and len(p) == nc + 3
     da.close()
+
+
+def test_dastar_config3_full_size(ctx):
+    """C3 at full size (N = 1,000,000, out-degree 8, w in [1,1000], seed 3),
+    first 4 of its batches of 10,000 inserts + 10,000 deletes, h = reverse-BFS
+    hops x min weight on the union of every version.  Each batch: repair ==
+    recompute.  After the last: both equal the oracle's Dijkstra on the
+    oracle's own merged graph, the path is a real path of that cost, and the
+    library's graph equals the oracle's edge for edge."""
+    hg = ddp_amd.HostGraph.random(1000000, 8, 1, 1000, 3)
+    upd = hg.gen_updates(4, 10000, 10000, 1, 1000, 33).batches
+    src, dst = hg.reachable_pairs(1, 3)
+    s, t = int(src[0]), int(dst[0])
+    o = full_offsets(hg.off, hg.E)
+    us = [np.repeat(np.arange(hg.N, dtype=np.int64), np.diff(o))]
+    vs, ws = [hg.col.astype(np.int64)], [hg.w.astype(np.int64)]
+    for b in upd:
+        a = np.array([l for l in b if l[0] == 1], np.int64)
+        us.append(a[:, 1])
+        vs.append(a[:, 2])
+        ws.append(a[:, 3])
+    us, vs, ws = np.concatenate(us), np.concatenate(vs), np.concatenate(ws)
+    order = np.argsort(us, kind="stable")
+    uoff = np.searchsorted(us[order], np.arange(hg.N)).astype(np.int32)
+    roff, rcol, _ = pyoracle.transpose(uoff, vs[order].astype(np.int32), ws[order].astype(np.uint32))
+    hops, _ = pyoracle.bfs_hops(roff, rcol, t)
+    h = np.where(hops < 0, -1, hops * int(ws.min())).astype(np.int32)
+    da = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=4096)
+    rc = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=4096)
+    cur = (hg.off, hg.col, hg.w)
+    for bi, lines in enumerate(upd):
+        cost, st = da.apply(lines, mode=ddp_amd.DAStar.REPAIR)
+        cost_rc, _ = rc.apply(lines, mode=ddp_amd.DAStar.RECOMPUTE)
+        say("c3 full batch", bi, cost, cost_rc, st["invalidated"])
+        assert cost == cost_rc
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
+    dist, _ = pyoracle.dijkstra(*cur, s, t)
+    want = int(dist[t]) if dist[t] >= 0 else ddp_amd.INF_COST
+    assert cost == want
+    if want != ddp_amd.INF_COST:
+        p = da.path()
+        assert p[0] == s and p[-1] == t
+        assert path_cost(*cur, p) == want
+    g = da.graph()
+    assert np.array_equal(g.off, cur[0]) and np.array_equal(g.col, cur[1]) and np.array_equal(g.w, cur[2])
+    da.close()
+    rc.close()

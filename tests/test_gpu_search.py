@@ -484,3 +484,25 @@ def test_batch_config4_full_size(ctx):
         dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
         want = int(dist[dst[i]]) if dist[dst[i]] >= 0 else ddp_amd.INF_COST
         assert cost[i] == want, (i, cost[i], want)
+
+
+def test_astar_config2_full_size(ctx):
+    """C2 at full size: the 4096x4096 maze (p_block 0.30, seed 2), corner to
+    corner, Manhattan heuristic.  Pile mode (K = 1,024) and grid mode
+    (K = 2,048) return the cost of the oracle's indexed-heap A* and real
+    paths of that cost."""
+    W = 4096
+    hg = ddp_amd.HostGraph.grid_maze(W, W, 300, 2)
+    s, t = 0, W * W - 1
+    xs = np.arange(W * W, dtype=np.int64)
+    h = ((W - 1 - xs // W) + (W - 1 - xs % W)).astype(np.int32)
+    found, want, _ = pyoracle.astar_indexed(hg.off, hg.col, hg.w, h, s, t)
+    assert found and want >= 2 * (W - 1)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg.set_grid(W)
+    for K, mode in ((1024, ddp_amd.MODE_PILES), (2048, ddp_amd.MODE_GRID)):
+        say("c2 full", K, mode)
+        r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_MANHATTAN, K=K, mode=mode)
+        assert r.cost == want
+        check_path(hg.off, hg.col, hg.w, r.path, s, t, want)
+    dg.close()
