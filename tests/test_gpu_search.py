@@ -506,3 +506,25 @@ def test_astar_config2_full_size(ctx):
         assert r.cost == want
         check_path(hg.off, hg.col, hg.w, r.path, s, t, want)
     dg.close()
+
+
+def test_batch_config5_full_size(ctx):
+    """C5's graph at full size (R-MAT scale 23: 8,388,608 nodes, 134,217,728
+    edges, a/b/c/d = 0.57/0.19/0.19/0.05, w in [1,1000], seed 5), h = 0, 8
+    hub pairs in one batch on the pile engine: every cost matches a plain
+    Dijkstra and every path is a real path of that cost."""
+    hg = ddp_amd.HostGraph.rmat(23, 16 << 23, 0.57, 0.19, 0.19, 1, 1000, 5)
+    src, dst = hg.hub_pairs(8, 5)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=True)
+    say("c5 full: batch")
+    cost, paths, st = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, block=1024, path_cap=4096,
+                                          engine=ddp_amd.ENGINE_PILES)
+    assert st["errors"] == 0
+    dg.close()
+    for i in range(len(src)):
+        dist, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[i]), int(dst[i]))
+        want = int(dist[dst[i]]) if dist[dst[i]] >= 0 else ddp_amd.INF_COST
+        say("c5 full: query", i, cost[i], want)
+        assert cost[i] == want
+        if want != ddp_amd.INF_COST:
+            check_path(hg.off, hg.col, hg.w, paths[i], int(src[i]), int(dst[i]), want)
