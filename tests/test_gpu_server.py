@@ -151,3 +151,47 @@ def test_server_ring_backpressure(ctx):
         d, _ = pyoracle.dijkstra(hg.off, hg.col, hg.w, int(src[k]))
         assert done[k]["cost"] == d[dst[k]]
     sv.close()
+
+
+def test_server_config4_full_size(ctx):
+    """The C4 workload through the continuous-batching server at full size:
+    4,096 queries on the 2,000,000-node geometric graph submitted at once,
+    with one update batch (100 deletes + 100 inserts that keep the Euclidean
+    heuristic admissible) in the middle.  Queries before the update return
+    the batch kernel's costs on the original graph, queries after it the
+    batch kernel's costs on the updated graph: the same answer alone and in
+    the mix."""
+    hg = ddp_amd.HostGraph.geometric(2000000, 10, 100000, 4)
+    src, dst = hg.geo_pairs(4096, 100, 100000, 4)
+    ub = geometric_updates(hg, 1, 100, 8)
+    mid = len(src) // 2
+    n = len(src) + 1
+    types = np.zeros(n, np.int32)
+    types[mid] = 1
+    qs, qt = np.insert(src, mid, 0), np.insert(dst, mid, 0)
+    ups = [None] * n
+    ups[mid] = ub[0]
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=True)
+    say("c4 served: batch before")
+    before, _, st = ddp_amd.astar_batch(ctx, dg, src[:mid], dst[:mid], heur=ddp_amd.H_EUCLID, block=1024,
+                                        engine=ddp_amd.ENGINE_PILES)
+    assert st["errors"] == 0
+    say("c4 served: server")
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_EUCLID)
+    res = sv.replay(np.zeros(n), types, qs, qt, ups)
+    say("  info", sv.info())
+    sv.close()
+    dg.close()
+    say("c4 served: batch after")
+    g1 = ddp_amd.HostGraph(*pyoracle.apply_batch(hg.off, hg.col, hg.w, ub[0])[:3], xy=hg.xy)
+    dg1 = ddp_amd.DeviceGraph(ctx, g1, reorder=True)
+    after, _, st = ddp_amd.astar_batch(ctx, dg1, src[mid:], dst[mid:], heur=ddp_amd.H_EUCLID, block=1024,
+                                       engine=ddp_amd.ENGINE_PILES)
+    assert st["errors"] == 0
+    dg1.close()
+    assert [r["tag"] for r in res] == list(range(n)) and all(r["status"] == 0 for r in res)
+    got = np.array([r["cost"] for r in res])
+    ver = np.array([r["version"] for r in res])
+    assert (ver[:mid] == 0).all() and (ver[mid:] == 1).all()
+    assert np.array_equal(got[:mid], before)
+    assert np.array_equal(got[mid + 1:], after)
