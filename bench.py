@@ -535,7 +535,7 @@ def main():
     ap.add_argument("--c3-nodes", type=int, default=1000000)
     ap.add_argument("--c3-batches", type=int, default=20)
     ap.add_argument("--c3-batch", type=int, default=20000, help="lines per batch (half inserts, half deletes)")
-    ap.add_argument("--c3-K", type=int, default=4096)
+    ap.add_argument("--c3-K", type=int, default=16384, help="queues of both DA* sessions (repair and recompute)")
     ap.add_argument("--rmat-scale", type=int, default=23)
     ap.add_argument("--c5-queries", type=int, default=256)
     ap.add_argument("--c5-requests", type=int, default=1024)
