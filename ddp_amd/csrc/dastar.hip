@@ -239,7 +239,10 @@ __global__ void __launch_bounds__(kWalkBlock)
     while (ncur > 0 && ncur <= wide) {
         for (int32_t i = threadIdx.x; i < ncur; i += kWalkBlock)
             claim_children(G, L, cur[i], gp, mark, epoch, nxt, base, marked, rctrl);
-        __syncthreads();  // every draw has returned and every nxt entry is stored
+        // every draw has returned; waiting on the memory counter before the
+        // barrier (a bare s_barrier) lands this wave's nxt stores, so the next
+        // level reads them
+        block_sync_global();
         if (threadIdx.x == 0) s_n = __hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __syncthreads();
         const uint32_t total = s_n;
