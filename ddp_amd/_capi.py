@@ -17,7 +17,7 @@ DDP_OK = 0
 ERR_NAMES = {
     1: "DDP_ERR_ARG", 2: "DDP_ERR_HIP", 3: "DDP_ERR_NOMEM", 4: "DDP_ERR_OVERFLOW",
     5: "DDP_ERR_IO", 6: "DDP_ERR_FORMAT", 7: "DDP_ERR_TIMEOUT", 8: "DDP_ERR_STATE",
-    9: "DDP_ERR_NODEVICE",
+    9: "DDP_ERR_NODEVICE", 10: "DDP_ERR_CAPACITY",
 }
 INF_COST = 2147483647
 

@@ -302,6 +302,15 @@ extern "C" int ddp_batch_upload(ddp_batch *b, int32_t nq, const int32_t *src, co
     DDP_CHECK(nq >= 0 && nq <= b->maxq, ctx, DDP_ERR_ARG, "nq=%d outside [0,%d]", nq, b->maxq);
     DDP_CHECK(nq == 0 || (src && dst), ctx, DDP_ERR_ARG, "ddp_batch_upload: null query arrays");
     DDP_HIP(ctx, hipSetDevice(ctx->device));
+    if (b->hv.kind == DDP_H_ARRAY && b->g->h_target >= 0) {
+        // an h array from ddp_bfs_hops estimates the distance to one target;
+        // for any other target it may be inadmissible or mark reachable
+        // nodes unreachable, so such queries are refused
+        const int32_t ht = ddp_to_user(b->g, b->g->h_target);
+        for (int32_t i = 0; i < nq; ++i)
+            DDP_CHECK(dst[i] == ht, ctx, DDP_ERR_ARG,
+                      "query %d: target %d, but the DDP_H_ARRAY heuristic was computed for target %d", i, dst[i], ht);
+    }
     std::vector<int32_t> ms, md;
     if (!b->g->perm.empty()) {  // caller ids -> device ids (out-of-range ids stay, the kernel flags them)
         const int32_t N = b->g->N;

@@ -165,6 +165,7 @@ extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t rever
         if (store_as_h) {
             if (!g->h) DDP_HIP(ctx, hipMalloc((void **)&g->h, (size_t)N * 4));
             scaled = g->h;
+            g->h_target = t;
         } else {
             scaled = front;  // frontier buffers are free now
         }

@@ -394,6 +394,7 @@ extern "C" int ddp_graph_set_h(ddp_graph *g, const int32_t *h)
         h = hp.data();
     }
     if (!g->h) DDP_HIP(ctx, hipMalloc((void **)&g->h, (size_t)g->N * 4));
+    g->h_target = -1;  // a caller's array: its target is not known here
     DDP_HIP(ctx, hipMemcpyAsync(g->h, h, (size_t)g->N * 4, hipMemcpyHostToDevice, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return DDP_OK;

@@ -5,8 +5,8 @@
 // Invariant kept between batches (the one A* itself maintains): every finite
 // g is the cost of a real path, and every node that is not in a queue has
 // relaxed all of its live out-edges with its current g.  A batch restores it:
-//   1. deletes (matched on the host, first live instance of u->v as in
-//      DA_star/GA_merge.cu:1019-1037) become tombstones in HBM;
+//   1. deletes (matched on the device by apply_deletes, first live instance
+//      of u->v as in DA_star/GA_merge.cu:1019-1037) become tombstones in HBM;
 //   2. if a deleted u->v was v's parent edge, v's whole parent subtree is
 //      invalidated (found by walking children: y with parent(y) == x), which
 //      replaces the ancestor walks and serial cycle fixing of
@@ -634,6 +634,10 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
 {
     if (!d || !cost || (n && !lines)) return DDP_ERR_ARG;
     ddp_ctx *ctx = d->ctx;
+    // checked before the batch touches the device: a batch applied without
+    // its repair would leave g values over edges that no longer exist
+    DDP_CHECK(mode == DDP_DASTAR_REPAIR || mode == DDP_DASTAR_RECOMPUTE, ctx, DDP_ERR_ARG, "unknown DA* mode %d",
+              mode);
     DDP_HIP(ctx, hipSetDevice(ctx->device));
     const auto t0 = clk::now();
     const int32_t N = d->N;
@@ -708,7 +712,6 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     if (mode == DDP_DASTAR_RECOMPUTE) {
         if ((rc = fresh_search(d, &iters, &kms))) return rc;
     } else {
-        DDP_CHECK(mode == DDP_DASTAR_REPAIR, ctx, DDP_ERR_ARG, "unknown DA* mode %d", mode);
         const auto t1 = clk::now();
         // ---- 3. invalidate parent subtrees of deleted tree edges (roots seeded by apply_deletes)
         if ((rc = ddp_ga_prepare_resume(ctx, d->S))) return rc;

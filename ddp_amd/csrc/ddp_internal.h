@@ -35,6 +35,7 @@ struct ddp_graph {
     int32_t RE = 0;
     // heuristics
     int32_t *h = nullptr;
+    int32_t h_target = -1;  // device id the h array was computed for (ddp_bfs_hops); -1 = not known
     int32_t *xy = nullptr;
     int32_t grid_w = 0;
     // host copy of the coordinates for work estimates (batched scheduling)

@@ -52,8 +52,9 @@ struct Res {       // pinned, written by the device
 
 enum : uint32_t { SERR_WAIT = 16, SERR_LOG = 32 };
 // ctrl words, one 64-byte line apart: claimed count, device copy of the host's
-// published count, poller lease, stop flag copy, applied version
-enum : int { CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_STOP = 80, CT_WORDS = 96 };
+// published count, applied version, poller lease, sticky error, stop flag copy.
+// Counts are unsigned and compared by difference, so they may wrap.
+enum : int { CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_ERR = 64, CT_STOP = 80, CT_WORDS = 96 };
 
 struct ServeArgs {
     GraphView g;           // base rows + versioned view fields (ver set per query)
@@ -145,7 +146,8 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
     extern __shared__ uint64_t s_heap[];
     constexpr int SV = (ENG == SENG_PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
     __shared__ SlotLds<B, SV> S;
-    __shared__ int32_t s_ticket;
+    __shared__ uint32_t s_ticket;
+    __shared__ int32_t s_have;
     __shared__ uint32_t s_uerr;
     const int lane = threadIdx.x;
     const int64_t slot = blockIdx.x;
@@ -165,16 +167,18 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         // slots neither flood the host link nor invalidate caches that the
         // running searches depend on.
         if (lane == 0) {
-            int32_t ticket = -1;
+            uint32_t ticket = 0;
+            bool have = false;
             const uint64_t t0 = dev_clock_100mhz();
             bool poller = false;
             int backoff = 0;  // non-pollers back off to ~15 us
             for (;;) {
                 const uint32_t h = ld_u32(a.ctrl + CT_HEAD);
                 const uint32_t tail = ld_u32(a.ctrl + CT_TAIL);
-                if (h < tail) {
+                if ((int32_t)(tail - h) > 0) {
                     if (atomicCAS(a.ctrl + CT_HEAD, h, h + 1) == h) {
-                        ticket = (int32_t)h;
+                        ticket = h;
+                        have = true;
                         break;
                     }
                     continue;
@@ -182,8 +186,9 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                 if (ld_u32(a.ctrl + CT_STOP) || dev_clock_100mhz() - t0 > a.idle_ticks) break;
                 if (!poller) poller = atomicCAS(a.ctrl + CT_POLL, 0u, 1u) == 0u;
                 if (poller) {
+                    // the lease holder is the only writer of the tail copy
                     const uint32_t ht = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                    if (ht > tail) atomicMax(a.ctrl + CT_TAIL, ht);
+                    if ((int32_t)(ht - tail) > 0) st_u32(a.ctrl + CT_TAIL, ht);
                     if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
                         atomicExch(a.ctrl + CT_STOP, 1u);
                     __builtin_amdgcn_s_sleep(16);
@@ -195,14 +200,15 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             if (poller) atomicExch(a.ctrl + CT_POLL, 0u);
             // the request's fields were written before the published count:
             // one system-scope acquire per claim orders the reads below
-            if (ticket >= 0) __atomic_thread_fence(__ATOMIC_ACQUIRE);
+            if (have) __atomic_thread_fence(__ATOMIC_ACQUIRE);
             s_ticket = ticket;
+            s_have = have ? 1 : 0;
             s_uerr = 0;
         }
         __syncthreads();
-        const int32_t ticket = s_ticket;
-        if (ticket < 0) break;
-        const Req *rp = a.ring + ((uint32_t)ticket & a.ring_mask);
+        const uint32_t ticket = s_ticket;
+        if (!s_have) break;
+        const Req *rp = a.ring + (ticket & a.ring_mask);
         Req rq;
         rq.type = sys_ld_i(&rp->type);
         rq.s = sys_ld_i(&rp->s);
@@ -225,6 +231,10 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                 __builtin_amdgcn_s_sleep(4);
             }
         }
+        // A failed update (a version wait that ran out, or a bad log entry)
+        // leaves the graph without that version: the error is sticky and
+        // every later request reports it instead of running.
+        if (lane == 0) s_uerr |= ld_u32(a.ctrl + CT_ERR);
         __syncthreads();
         // every wave acquires the applied version before reading graph data
         (void)__hip_atomic_load(a.ctrl + CT_APPLIED, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
@@ -235,10 +245,12 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             __syncthreads();
             status = s_uerr;
             if (lane == 0) {
+                if (status) atomicOr(a.ctrl + CT_ERR, status);
                 __threadfence();
-                // publish even after a failed wait so later requests cannot stall behind it
-                __hip_atomic_store(a.ctrl + CT_APPLIED, (uint32_t)rq.version, __ATOMIC_RELEASE,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                // published even after a failure (later requests then report the
+                // sticky error instead of stalling); atomicMax keeps it from
+                // moving backwards when an earlier version finishes late
+                atomicMax(a.ctrl + CT_APPLIED, (uint32_t)rq.version);
             }
         } else if (!status) {
             GraphView G = a.g;
@@ -254,13 +266,13 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             if (__ballot(C.edges >= (1u << 30))) slot_flush_stats<B>(a.stats, C, stale);  // before 32 bits wrap
         }
         if (lane == 0) {
-            Res *r = a.res + ((uint32_t)ticket & a.ring_mask);
+            Res *r = a.res + (ticket & a.ring_mask);
             __hip_atomic_store(&r->cost, cost, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&r->status, (int32_t)status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&r->version, rq.version, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&r->t0, tstart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&r->t1, dev_clock_100mhz(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            __hip_atomic_store(&r->seq, (uint32_t)ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(&r->seq, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
         __syncthreads();
     }
@@ -347,10 +359,11 @@ struct ddp_server {
     uint32_t submitted = 0, collected = 0;
     int32_t version = 0;
     int32_t zero_w_version = INT32_MAX;  // first version with an inserted weight-0 edge
+    // per ring entry (request k lives at k & (ring - 1)); counts wrap modulo 2^32
     std::vector<uint64_t> tags;
     std::vector<double> submit_ms;
     std::vector<int32_t> types;
-    std::vector<uint8_t> delivered;  // per ring entry: handed to the caller, not yet retired
+    std::vector<uint8_t> delivered;  // handed to the caller, not yet retired
     clk::time_point t_create;
     hipEvent_t done_ev = nullptr;
     bool running = false;
@@ -433,7 +446,7 @@ static int server_ensure(ddp_server *sv)
         sv->running = false;
     }
     // with every workgroup gone, a request without a result was never claimed
-    for (uint32_t k = sv->collected; k < sv->submitted; ++k)
+    for (uint32_t k = sv->collected; k != sv->submitted; ++k)
         if (__atomic_load_n(&sv->res_h[k & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) != k + 1) return server_launch(sv);
     return DDP_OK;
 }
@@ -533,6 +546,9 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     hal((void **)&sv->arena_h, (void **)&sv->arena_d, sv->arena_cap * 4);
     hal((void **)&sv->flags_h, (void **)&sv->flags_d, 64);
     sv->delivered.assign(sv->ring, 0);
+    sv->tags.assign(sv->ring, 0);
+    sv->types.assign(sv->ring, 0);
+    sv->submit_ms.assign(sv->ring, 0.0);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sv->done_ev, hipEventDisableTiming);
     if (e != hipSuccess) return fail(ddp_hip_err(ctx, e, "serving allocation", __FILE__, __LINE__));
     hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->dver, (int64_t)g->E, INT32_MAX);
@@ -559,12 +575,14 @@ extern "C" int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *bl
 
 static void publish(ddp_server *sv, const Req &r, uint64_t tag, int32_t type)
 {
-    sv->ring_h[sv->submitted & (sv->ring - 1)] = r;
-    Res *res = sv->res_h + (sv->submitted & (sv->ring - 1));
-    __atomic_store_n(&res->seq, 0u, __ATOMIC_RELAXED);
-    sv->tags.push_back(tag);
-    sv->types.push_back(type);
-    sv->submit_ms.push_back(ms_now(sv));
+    const uint32_t k = sv->submitted & (sv->ring - 1);
+    sv->ring_h[k] = r;
+    Res *res = sv->res_h + k;
+    // cleared to the ticket itself: never equal to ticket + 1, even when it wraps
+    __atomic_store_n(&res->seq, sv->submitted, __ATOMIC_RELAXED);
+    sv->tags[k] = tag;
+    sv->types[k] = type;
+    sv->submit_ms[k] = ms_now(sv);
     ++sv->submitted;
     __atomic_store_n(&sv->flags_h[0], sv->submitted, __ATOMIC_RELEASE);
 }
@@ -605,7 +623,7 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
             if (l.w == 0) sv->zero_w_version = std::min(sv->zero_w_version, sv->version + 1);
         }
     }
-    DDP_CHECK(sv->log_used + (int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_NOMEM,
+    DDP_CHECK(sv->log_used + (int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_CAPACITY,
               "inserted-edge log full (%d entries); raise max_inserts", sv->log_cap);
     // entry ids in arrival order, then grouped by source (stable)
     std::vector<std::array<int32_t, 4>> q(ins.size());
@@ -618,7 +636,7 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
     // retries) if the new payload would overlap one still in use.
     size_t pos = sv->arena_head;
     if (pos + need > sv->arena_cap) pos = 0;
-    while (!sv->arena_marks.empty() && std::get<0>(sv->arena_marks.front()) < sv->collected)
+    while (!sv->arena_marks.empty() && (int32_t)(std::get<0>(sv->arena_marks.front()) - sv->collected) < 0)
         sv->arena_marks.erase(sv->arena_marks.begin());
     for (const auto &m : sv->arena_marks)
         if (pos < std::get<2>(m) && std::get<1>(m) < pos + need)
@@ -647,22 +665,22 @@ extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t
     // Results are handed out as they finish, in any order; a ring entry is
     // retired (reusable) once it and every earlier one have been handed out.
     const uint32_t mask = sv->ring - 1;
-    for (uint32_t k = sv->collected; k < sv->submitted && *n < max; ++k) {
+    for (uint32_t k = sv->collected; k != sv->submitted && *n < max; ++k) {
         if (sv->delivered[k & mask]) continue;
         Res *r = sv->res_h + (k & mask);
         if (__atomic_load_n(&r->seq, __ATOMIC_ACQUIRE) != k + 1) continue;
         ddp_server_result_t &o = out[(*n)++];
-        o.tag = sv->tags[k];
-        o.type = sv->types[k];
+        o.tag = sv->tags[k & mask];
+        o.type = sv->types[k & mask];
         o.cost = r->cost;
         o.version = r->version;
         o.status = r->status;
-        o.submit_ms = sv->submit_ms[k];
+        o.submit_ms = sv->submit_ms[k & mask];
         o.done_ms = ms_now(sv);
         o.device_ms = (double)(r->t1 - r->t0) * 1e-5;
         sv->delivered[k & mask] = 1;
     }
-    while (sv->collected < sv->submitted && sv->delivered[sv->collected & mask]) {
+    while (sv->collected != sv->submitted && sv->delivered[sv->collected & mask]) {
         sv->delivered[sv->collected & mask] = 0;
         ++sv->collected;
     }
@@ -673,13 +691,13 @@ extern "C" int ddp_server_drain(ddp_server *sv, int32_t timeout_ms)
 {
     if (!sv) return DDP_ERR_ARG;
     const auto t0 = clk::now();
-    std::vector<ddp_server_result_t> sink(1024);
-    while (sv->collected < sv->submitted) {
+    while (sv->collected != sv->submitted) {
         // results are kept for the caller: only wait here, do not consume
         const uint32_t k = sv->collected;
         uint32_t done = k;
-        while (done < sv->submitted &&
-               __atomic_load_n(&sv->res_h[done & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) == done + 1)
+        while (done != sv->submitted &&
+               (sv->delivered[done & (sv->ring - 1)] ||
+                __atomic_load_n(&sv->res_h[done & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) == done + 1))
             ++done;
         if (done == sv->submitted) break;
         int rc = server_ensure(sv);
@@ -745,9 +763,11 @@ extern "C" int ddp_server_replay(ddp_server *sv, int32_t n, const double *arriva
         if (std::chrono::duration<double>(clk::now() - t0).count() > 600)
             return ddp_set_err(ctx, DDP_ERR_TIMEOUT, "replay: %d of %d requests unfinished", n - received, n);
     }
-    // times relative to the replay start
+    // times relative to the replay start; a request's submit time is its
+    // scheduled arrival, so latencies include any wait before it entered the
+    // ring (a full ring, a busy payload arena)
     for (int32_t i = 0; i < n; ++i) {
-        out[i].submit_ms -= start;
+        out[i].submit_ms = arrival_ms[i];
         out[i].done_ms -= start;
     }
     return DDP_OK;

@@ -58,7 +58,11 @@ int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t max
 void ddp_batch_destroy(ddp_batch *b);
 int ddp_batch_info(const ddp_batch *b, int32_t *slots, int32_t *block, int32_t *heap_cap,
                    int32_t *lds_cap);
-/* Copies the query arrays to HBM and fixes the schedule order. */
+/* Copies the query arrays to HBM and fixes the schedule order.  With
+ * DDP_H_ARRAY every query must have the target the array was computed for:
+ * DDP_ERR_ARG if ddp_bfs_hops(store_as_h) computed it for another one (an
+ * array given to ddp_graph_set_h carries no target; the caller keeps this
+ * rule). */
 int ddp_batch_upload(ddp_batch *b, int32_t nq, const int32_t *src, const int32_t *dst);
 /* Asynchronous: replays the captured hipGraph (counter reset + persistent
  * search kernel) on the context stream. */

@@ -37,6 +37,8 @@ extern "C" {
 #define DDP_ERR_TIMEOUT   7  /* device wait or iteration limit reached */
 #define DDP_ERR_STATE     8  /* call not valid in the handle's current state */
 #define DDP_ERR_NODEVICE  9  /* no usable GPU */
+#define DDP_ERR_CAPACITY 10  /* a fixed capacity is used up and will not free itself
+                                (unlike DDP_ERR_NOMEM, retrying does not help) */
 
 #define DDP_INF_COST 2147483647
 

@@ -60,7 +60,11 @@ typedef struct {
     int32_t type;           /* DDP_REQ_QUERY / DDP_REQ_UPDATE */
     int32_t cost;           /* queries: Cx of t, DDP_INF_COST if unreachable */
     int32_t version;        /* graph version the request saw (queries) or made (updates) */
-    int32_t status;         /* 0 ok; 1 heap_cap exceeded; 2 iteration limit; 4 bad id */
+    int32_t status;         /* 0 ok; 1 heap_cap exceeded; 2 iteration limit; 4 bad id;
+                               16 an update's wait for the previous version ran out;
+                               32 an update carried an invalid log entry.  16 and 32
+                               are sticky: once an update fails, every later request
+                               reports the same bits and is not run */
     double submit_ms;       /* host clock (ms since ddp_server_create) */
     double done_ms;         /* host clock when the result was collected */
     double device_ms;       /* device time from claim to completion */
@@ -69,7 +73,10 @@ typedef struct {
 int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *opts,
                       ddp_server **out);
 int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_t *version);
-/* Non-blocking; DDP_ERR_NOMEM if the ring is full (try again after polling). */
+/* Non-blocking; DDP_ERR_NOMEM if the ring (or, for updates, the payload arena)
+ * is full: try again after polling.  ddp_server_submit_update returns
+ * DDP_ERR_CAPACITY when the inserted-edge log cannot take the batch, which
+ * polling does not cure. */
 int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uint64_t tag);
 int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *lines, int32_t n, uint64_t tag);
 /* Collects up to max finished requests in completion order of submission. */
@@ -80,7 +87,11 @@ void ddp_server_destroy(ddp_server *sv);
 
 /* Open-loop replay: request i is submitted at arrival_ms[i] after the start
  * (queries: s[i], t[i]; updates: type[i] = DDP_REQ_UPDATE with lines
- * upd[upd_off[i] .. upd_off[i+1])), results are written to out[i]. */
+ * upd[upd_off[i] .. upd_off[i+1])), results are written to out[i], with
+ * submit_ms = arrival_ms[i] (so done_ms - submit_ms includes any wait before
+ * the request entered the ring) and done_ms relative to the start.  Only
+ * DDP_ERR_NOMEM is retried (after polling); any other failure, such as
+ * DDP_ERR_CAPACITY, is returned. */
 int ddp_server_replay(ddp_server *sv, int32_t n, const double *arrival_ms, const int32_t *type,
                       const int32_t *s, const int32_t *t, const int32_t *upd_off,
                       const ddp_update_t *upd, ddp_server_result_t *out);

@@ -91,3 +91,21 @@ def geometric_updates(hg, nb, per, seed, extra=1000):
             lines.append((1, int(u), int(v), max(1, euclid_ceil(p[u], p[v])) + int(rng.integers(0, extra))))
         out.append(lines)
     return out
+
+
+def rmat_updates(hg, nb, per, seed):
+    """nb update batches of `per` deletes of existing edges (sources drawn
+    uniformly among nodes with out-edges) and `per` random inserts with
+    w uniform in [1, 1000]: the C5 serving mix."""
+    rng = np.random.default_rng(seed)
+    o = full_offsets(hg.off, hg.E)
+    deg = np.diff(o)
+    out = []
+    for _ in range(nb):
+        u = rng.integers(0, hg.N, per)
+        u = u[deg[u] > 0]
+        j = o[u] + (rng.random(len(u)) * deg[u]).astype(np.int64)
+        dels = [(0, int(a), int(b), 0) for a, b in zip(u, hg.col[j])]
+        iu, iv, iw = rng.integers(0, hg.N, per), rng.integers(0, hg.N, per), rng.integers(1, 1001, per)
+        out.append(dels + [(1, int(a), int(b), int(c)) for a, b, c in zip(iu, iv, iw)])
+    return out

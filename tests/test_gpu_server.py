@@ -195,3 +195,96 @@ def test_server_config4_full_size(ctx):
     assert (ver[:mid] == 0).all() and (ver[mid:] == 1).all()
     assert np.array_equal(got[:mid], before)
     assert np.array_equal(got[mid + 1:], after)
+
+
+@pytest.mark.timeout(1200)
+def test_server_config5_served(ctx):
+    """C5's served workload on its own graph: R-MAT scale 23 (8,388,608
+    nodes, 134,217,728 edges, a/b/c/d = 0.57/0.19/0.19/0.05, seed 5),
+    w in [1,1000], h = 0.  Poisson arrivals of point queries between connected
+    pairs, interleaved with 3 update batches of 1,000 deletes + 1,000 inserts.
+    Every request succeeds and is bound to the version of its submission
+    position; on every version, sampled queries equal the oracle's Dijkstra on
+    the oracle-applied graph, and every served cost equals the batch kernel's
+    on that version's graph, whose paths are checked edge by edge."""
+    from helpers import path_cost, rmat_updates
+    hg = ddp_amd.HostGraph.rmat(23, 16 * (1 << 23), 0.57, 0.19, 0.19, 1, 1000, 5)
+    say("c5 served: graph", hg.N, hg.E)
+    src, dst = hg.hub_pairs(96, 5)
+    ub = rmat_updates(hg, 3, 1000, 9)
+    n = len(src) + len(ub)
+    pos = [24, 48, 72]  # request index of each update batch
+    types = np.zeros(n, np.int32)
+    types[pos] = 1
+    qidx = np.full(n, -1)
+    qidx[types == 0] = np.arange(len(src))
+    qs = np.where(qidx >= 0, src[np.maximum(qidx, 0)], 0).astype(np.int32)
+    qt = np.where(qidx >= 0, dst[np.maximum(qidx, 0)], 0).astype(np.int32)
+    ups = [None] * n
+    for k, p in enumerate(pos):
+        ups[p] = ub[k]
+    rng = np.random.default_rng(55)
+    arrival = np.cumsum(rng.exponential(1000.0 / 150.0, n))  # 150 requests/s offered
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=True)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO)
+    say("c5 served: replay", sv.info())
+    res = sv.replay(arrival, types, qs, qt, ups)
+    sv.close()
+    dg.close()
+    assert [r["tag"] for r in res] == list(range(n)) and all(r["status"] == 0 for r in res), \
+        [r for r in res if r["status"]][:3]
+    ver = np.cumsum(types)
+    assert [r["version"] for r in res] == ver.tolist()
+    got = np.array([r["cost"] for r in res])
+    lat = np.array([r["done_ms"] - r["submit_ms"] for r in res if r["type"] == 0])
+    say("c5 served: latency p50 / p99 ms", np.percentile(lat, 50), np.percentile(lat, 99))
+    cur = (hg.off, hg.col, hg.w)
+    for v in range(len(ub) + 1):
+        if v:
+            cur = pyoracle.apply_batch(*cur, ub[v - 1])[:3]
+        mine = np.flatnonzero((ver == v) & (types == 0))
+        # the batch kernel on this version's graph, with paths
+        gv = ddp_amd.HostGraph(*cur)
+        dgv = ddp_amd.DeviceGraph(ctx, gv, reorder=True)
+        costs, paths, st = ddp_amd.astar_batch(ctx, dgv, qs[mine], qt[mine], heur=ddp_amd.H_ZERO, path_cap=4096)
+        dgv.close()
+        assert st["errors"] == 0
+        assert np.array_equal(got[mine], costs), v
+        for k in range(len(mine)):
+            p = paths[k]
+            assert len(p) and p[0] == qs[mine[k]] and p[-1] == qt[mine[k]] and path_cost(*cur, p) == costs[k]
+        for i in mine[:2]:  # sampled against the oracle's Dijkstra
+            d, _ = pyoracle.dijkstra(*cur, int(qs[i]), int(qt[i]))
+            say("c5 served: version", v, "request", i, "cost", got[i], "oracle", d[qt[i]])
+            assert got[i] == d[qt[i]]
+
+
+def test_server_replay_log_full_returns_capacity(ctx):
+    """A full inserted-edge log is not a transient condition: replay returns
+    DDP_ERR_CAPACITY instead of polling forever, and the updates and queries
+    submitted before it still finish correctly."""
+    hg = ddp_amd.HostGraph.random(2000, 4, 1, 50, 61)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    src, dst = hg.reachable_pairs(8, 62)
+    ub = hg.gen_updates(4, 30, 10, 1, 50, 63).batches  # 30 inserts each; the log holds 64
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, max_inserts=64, slots=4)
+    n = 8
+    types = np.array([0, 1, 0, 1, 0, 1, 0, 0], np.int32)
+    ups = [None, ub[0], None, ub[1], None, ub[2], None, None]
+    with pytest.raises(ddp_amd.DDPError) as e:
+        sv.replay(np.zeros(n), types, src[:n], dst[:n], ups)
+    assert "DDP_ERR_CAPACITY" in str(e.value)
+    sv.drain()
+    sv.poll(64)
+    assert sv.info()["version"] == 2  # the third update was refused, the first two applied
+    g2 = pyoracle.apply_batch(*pyoracle.apply_batch(hg.off, hg.col, hg.w, ub[0])[:3], ub[1])[:3]
+    for k in range(4):
+        assert sv.submit_query(int(src[k]), int(dst[k]), 100 + k) == 0
+    sv.drain()
+    done = {r["tag"]: r for r in sv.poll(64)}
+    for k in range(4):
+        d, _ = pyoracle.dijkstra(*g2, int(src[k]))
+        r = done[100 + k]
+        assert r["version"] == 2 and r["status"] == 0
+        assert r["cost"] == (d[dst[k]] if d[dst[k]] >= 0 else ddp_amd.INF_COST)
+    sv.close()
