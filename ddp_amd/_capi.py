@@ -111,6 +111,7 @@ SIGNATURES = {
     "ddp_graph_set_coords": (c_int, [c_void_p, P_I32]),
     "ddp_graph_set_grid": (c_int, [c_void_p, c_int32]),
     "ddp_bfs_hops": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_int32, P_I32, P_I32]),
+    "ddp_bfs_hops_noalloc": (c_int, [c_void_p, c_void_p, c_int32, c_int32, c_int32, c_void_p, P_I32]),
     "ddp_astar": (c_int, [c_void_p, c_void_p, Heur, c_int32, c_int32, POINTER(AstarOpts), P_I32, P_I32,
                           P_I32, c_int32, P_I32, POINTER(AstarStats)]),
     # ddp_batch.h

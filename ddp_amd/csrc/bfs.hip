@@ -11,6 +11,7 @@
 // separated by a grid barrier; the frontier count of level L lives in one of
 // three rotating words so no extra barrier is needed to reset it.
 #include <algorithm>
+#include <mutex>
 
 #include "device_common.h"
 
@@ -116,32 +117,31 @@ __global__ void bfs_scale(const int32_t *dist, int32_t *out, int32_t N, int32_t 
     }
 }
 
-}  // namespace
-
-extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t reverse, int32_t scale,
-                            int32_t store_as_h, int32_t *h_out, int32_t *levels_out)
+// bfs_scale into the caller's numbering: out[iperm[i]] = hops(i) * scale.
+__global__ void bfs_scale_perm(const int32_t *dist, const int32_t *iperm, int32_t *out, int32_t N, int32_t scale)
 {
-    if (!ctx || !g) return ddp_set_err(ctx, DDP_ERR_ARG, "ddp_bfs_hops: null argument");
-    DDP_CHECK(t >= 0 && t < g->N, ctx, DDP_ERR_ARG, "ddp_bfs_hops: t=%d outside [0,%d)", t, g->N);
-    DDP_CHECK(!reverse || g->roff, ctx, DDP_ERR_STATE,
-              "ddp_bfs_hops: reverse=1 needs a graph uploaded with DDP_GRAPH_REVERSE");
-    DDP_CHECK(scale >= 1, ctx, DDP_ERR_ARG, "ddp_bfs_hops: scale must be >= 1");
-    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const int32_t d = dist[i];
+        out[iperm[i]] = d < 0 ? -1 : d * scale;
+    }
+}
+
+constexpr int kBfsBlock = 256;
+
+// Runs the BFS from device node t into dist (N entries) with front (2N
+// entries) and ctrl (BC_WORDS words) as scratch; waits for it and returns the
+// level count.  Allocates nothing.
+int bfs_run(ddp_ctx *ctx, const ddp_graph *g, int32_t t, int32_t reverse, int32_t *dist, int32_t *front,
+            uint32_t *ctrl, int32_t *levels)
+{
+    constexpr int B = kBfsBlock;
     const int32_t N = g->N;
-    t = ddp_to_dev(g, t);
-    void *p;
-    int rc;
-    if ((rc = ddp_ws(ctx, WS_BFS_FRONT, (size_t)N * 12 + 256, &p))) return rc;
-    int32_t *dist = (int32_t *)p;
-    int32_t *front = dist + N;
-    if ((rc = ddp_ws(ctx, WS_BFS_CTRL, BC_WORDS * 4, &p))) return rc;
-    uint32_t *ctrl = (uint32_t *)p;
-    constexpr int B = 256;
     int per_cu = 0;
     DDP_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)bfs_kernel<B>, B, 0));
     DDP_CHECK(per_cu > 0, ctx, DDP_ERR_HIP, "bfs kernel does not fit on a CU");
     const int32_t need = (N + B - 1) / B;
     const int32_t grid = std::max(1, std::min(need, std::min(per_cu, 2) * ctx->prop.multiProcessorCount));
+    // the control words are cleared by a kernel, not a memset node
     hipLaunchKernelGGL(bfs_init, dim3(1024), dim3(256), 0, ctx->stream, dist, N, front, ctrl, t);
     hipLaunchKernelGGL(bfs_seed, dim3(1), dim3(64), 0, ctx->stream, dist, front, ctrl, t);
     BfsArgs a{reverse ? g->roff : g->off, reverse ? g->rcol : g->col, N, dist, front, ctrl,
@@ -159,7 +159,72 @@ extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t rever
     DDP_HIP(ctx, hipMemcpyAsync(res, ctrl + BC_LEVELS, 8, hipMemcpyDeviceToHost, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     DDP_CHECK(res[1] == 0, ctx, DDP_ERR_TIMEOUT, "bfs: grid barrier wait timed out");
-    if (levels_out) *levels_out = (int32_t)res[0];
+    *levels = (int32_t)res[0];
+    return DDP_OK;
+}
+
+int bfs_check(ddp_ctx *ctx, const ddp_graph *g, int32_t t, int32_t reverse, int32_t scale, const char *who)
+{
+    if (!ctx || !g) return ddp_set_err(ctx, DDP_ERR_ARG, "%s: null argument", who);
+    DDP_CHECK(t >= 0 && t < g->N, ctx, DDP_ERR_ARG, "%s: t=%d outside [0,%d)", who, t, g->N);
+    DDP_CHECK(!reverse || g->roff, ctx, DDP_ERR_STATE, "%s: reverse=1 needs a graph uploaded with DDP_GRAPH_REVERSE",
+              who);
+    DDP_CHECK(scale >= 1, ctx, DDP_ERR_ARG, "%s: scale must be >= 1", who);
+    return DDP_OK;
+}
+
+// ------------------------------------------------- allocation-free variant
+//
+// Device memory of ddp_bfs_hops_noalloc: the distance array, the two frontier
+// buffers and the control words of one BFS over at most kBfsScratchNodes
+// nodes, 12 bytes per node (24 MiB + 256 bytes).  Every GPU the library runs
+// on holds its own copy until the process ends.  In host code the name is not
+// a device address: kernels get scratch_dev(device), the address
+// hipGetSymbolAddress returns for that device.
+constexpr int32_t kBfsScratchNodes = 1 << 21;
+static __device__ __align__(256) int32_t scratch[3 * (size_t)kBfsScratchNodes + BC_WORDS];
+
+constexpr int kMaxDevices = 64;
+static std::mutex s_scratch_mu;
+static int32_t *s_scratch_ptr[kMaxDevices];
+
+// The address of this device's copy of scratch, looked up once per device
+// with that device current.
+static int scratch_dev(ddp_ctx *ctx, int32_t **out)
+{
+    DDP_CHECK(ctx->device >= 0 && ctx->device < kMaxDevices, ctx, DDP_ERR_ARG, "device %d out of range",
+              ctx->device);
+    std::lock_guard<std::mutex> lock(s_scratch_mu);
+    if (!s_scratch_ptr[ctx->device]) {
+        void *p = nullptr;
+        DDP_HIP(ctx, hipSetDevice(ctx->device));
+        DDP_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        DDP_CHECK(p != nullptr, ctx, DDP_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        s_scratch_ptr[ctx->device] = (int32_t *)p;
+    }
+    *out = s_scratch_ptr[ctx->device];
+    return DDP_OK;
+}
+
+}  // namespace
+
+extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t reverse, int32_t scale,
+                            int32_t store_as_h, int32_t *h_out, int32_t *levels_out)
+{
+    int rc = bfs_check(ctx, g, t, reverse, scale, "ddp_bfs_hops");
+    if (rc) return rc;
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    const int32_t N = g->N;
+    t = ddp_to_dev(g, t);
+    void *p;
+    if ((rc = ddp_ws(ctx, WS_BFS_FRONT, (size_t)N * 12 + 256, &p))) return rc;
+    int32_t *dist = (int32_t *)p;
+    int32_t *front = dist + N;
+    if ((rc = ddp_ws(ctx, WS_BFS_CTRL, BC_WORDS * 4, &p))) return rc;
+    uint32_t *ctrl = (uint32_t *)p;
+    int32_t levels = 0;
+    if ((rc = bfs_run(ctx, g, t, reverse, dist, front, ctrl, &levels))) return rc;
+    if (levels_out) *levels_out = levels;
     int32_t *scaled = dist;
     if (scale != 1 || store_as_h) {
         if (store_as_h) {
@@ -179,6 +244,34 @@ extern "C" int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t rever
         DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         for (int32_t i = 0; i < N; ++i) h_out[g->iperm[i]] = tmp[i];
     }
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return DDP_OK;
+}
+
+extern "C" int ddp_bfs_hops_noalloc(ddp_ctx *ctx, const ddp_graph *g, int32_t t, int32_t reverse, int32_t scale,
+                                    int32_t *h_out_dev, int32_t *levels_out)
+{
+    int rc = bfs_check(ctx, g, t, reverse, scale, "ddp_bfs_hops_noalloc");
+    if (rc) return rc;
+    DDP_CHECK(h_out_dev, ctx, DDP_ERR_ARG, "ddp_bfs_hops_noalloc: null output");
+    DDP_CHECK(g->N <= kBfsScratchNodes, ctx, DDP_ERR_CAPACITY,
+              "ddp_bfs_hops_noalloc: N = %d exceeds the %d nodes its static device memory holds", g->N,
+              kBfsScratchNodes);
+    DDP_HIP(ctx, hipSetDevice(ctx->device));
+    int32_t *base = nullptr;
+    if ((rc = scratch_dev(ctx, &base))) return rc;
+    const int32_t N = g->N;
+    int32_t *dist = base, *front = base + N;  // dist N, front 2N, ctrl BC_WORDS
+    uint32_t *ctrl = (uint32_t *)(base + 3 * (size_t)N);
+    int32_t levels = 0;
+    if ((rc = bfs_run(ctx, g, ddp_to_dev(g, t), reverse, dist, front, ctrl, &levels))) return rc;
+    if (levels_out) *levels_out = levels;
+    if (g->d_iperm)
+        hipLaunchKernelGGL(bfs_scale_perm, dim3(1024), dim3(256), 0, ctx->stream, dist, (const int32_t *)g->d_iperm,
+                           h_out_dev, N, scale);
+    else
+        hipLaunchKernelGGL(bfs_scale, dim3(1024), dim3(256), 0, ctx->stream, dist, h_out_dev, N, scale);
+    DDP_HIP(ctx, hipGetLastError());
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return DDP_OK;
 }

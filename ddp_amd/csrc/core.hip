@@ -165,7 +165,7 @@ extern "C" void ddp_graph_destroy(ddp_graph *g)
 {
     if (!g) return;
     hipSetDevice(g->device);  // hipFree waits for work that uses the memory
-    void *ptrs[] = {g->off, g->col, g->w, g->roff, g->rcol, g->reid, g->rw, g->h, g->xy};
+    void *ptrs[] = {g->off, g->col, g->w, g->roff, g->rcol, g->reid, g->rw, g->h, g->xy, g->d_iperm};
     for (void *p : ptrs)
         if (p) hipFree(p);
     delete g;
@@ -325,7 +325,8 @@ extern "C" int ddp_graph_upload_ex(ddp_ctx *ctx, int32_t N, int32_t E, const int
     g->iperm.swap(iperm);
     int rc;
     if ((rc = upload(ctx, &g->off, o.data(), (size_t)N + 1)) || (rc = upload(ctx, &g->col, col, (size_t)E)) ||
-        (rc = upload(ctx, &g->w, w, (size_t)E))) {
+        (rc = upload(ctx, &g->w, w, (size_t)E)) ||
+        (!g->iperm.empty() && (rc = upload(ctx, &g->d_iperm, g->iperm.data(), (size_t)N)))) {
         ddp_graph_destroy(g);
         return rc;
     }

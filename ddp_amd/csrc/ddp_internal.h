@@ -43,6 +43,7 @@ struct ddp_graph {
     // node relabelling of DDP_GRAPH_REORDER: perm[caller id] = device id,
     // iperm[device id] = caller id; both empty when ids are kept
     std::vector<int32_t> perm, iperm;
+    int32_t *d_iperm = nullptr;  // device copy of iperm (relabelled graphs only)
 };
 
 // Caller node id -> device node id and back (identity without relabelling).

@@ -143,6 +143,21 @@ int ddp_bfs_hops(ddp_ctx *ctx, ddp_graph *g, int32_t t, int32_t reverse,
                  int32_t scale, int32_t store_as_h, int32_t *h_out,
                  int32_t *levels_out);
 
+/* No reference counterpart.  ddp_bfs_hops for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): it calls no hipMalloc/hipFree.  Its
+ * distance array, frontier buffers and control words live in a static device
+ * array of 12 bytes per node for at most 2,097,152 nodes (24 MiB, held by
+ * every GPU the library runs on until the process ends); a larger graph gets
+ * DDP_ERR_CAPACITY.  h_out_dev: N int32 in DEVICE memory the caller owns,
+ * written in the caller's node numbering (hops x scale, -1 = not reached).
+ * No store_as_h.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (several host threads or contexts) race; each call
+ * clears what it needs before use and waits for its work before returning. */
+int ddp_bfs_hops_noalloc(ddp_ctx *ctx, const ddp_graph *g, int32_t t,
+                         int32_t reverse, int32_t scale, int32_t *h_out_dev,
+                         int32_t *levels_out);
+
 /* ------------------------------------------------------ single-query GA* */
 
 typedef struct {
