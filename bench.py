@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "exact-cost A* queries/sec + p50/p99 latency, batched vs one-at-a-time GA*"
 HBM_PEAK_GBS = 8000.0
+ENGINE_NAMES = {0: "piles", 1: "heaps", 2: "piles", 3: "bidi"}
 
 
 def log(*a):
@@ -191,14 +192,17 @@ def run_c4(args):
     src, dst = hg.geo_pairs(args.queries, 100, 100000, 4)
     log(f"workload ready in {time.time() - t0:.1f}s: N={hg.N} E={hg.E}")
     ctx = ddp_amd.Context(0)
-    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=not args.no_reorder)
-    block, engine = args.block, args.engine
+    # transposed rows too: the bidirectional engine walks them backward from t
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True, reorder=not args.no_reorder)
+    block, engine, heap_cap, lds_cap = args.block, args.engine, 0, 0
     if block == 0:
         from ddp_amd import autotune
         tuned = autotune.tune_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, sample=len(src))
         block, engine = tuned["block"], engine or tuned["engine"]
+        heap_cap, lds_cap = tuned["heap_cap"], tuned["lds_cap"]
         log(f"tuned: {tuned}")
-    b = ddp_amd.Batch(ctx, dg, len(src), heur=ddp_amd.H_EUCLID, block=block, path_cap=4096, engine=engine)
+    b = ddp_amd.Batch(ctx, dg, len(src), heur=ddp_amd.H_EUCLID, block=block, path_cap=4096, engine=engine,
+                      heap_cap=heap_cap, lds_cap=lds_cap)
     b.upload(src, dst)
     for _ in range(args.warmup):
         b.launch()
@@ -216,9 +220,25 @@ def run_c4(args):
     if stats["errors"]:
         raise SystemExit(f"{stats['errors']} batched queries reported errors")
     costs, lat = out["cost"], out["latency_ms"]
+    bad = [i for i in range(len(src)) if costs[i] != ddp_amd.INF_COST and
+           (len(out["paths"][i]) == 0 or out["paths"][i][0] != src[i] or out["paths"][i][-1] != dst[i])]
+    if bad:
+        raise SystemExit(f"{len(bad)} batched paths do not run from s to t (first: query {bad[0]})")
     info = b.info()
     b.close()
     log(f"batched: {args.steps} steps in {elapsed:.3f}s")
+    # every engine on the same queries: identical costs, their kernel times side by side
+    engines = {}
+    for name, eng in () if args.no_engines else (("piles", ddp_amd.ENGINE_PILES), ("bidi", ddp_amd.ENGINE_BIDI),
+                                                  ("heaps", ddp_amd.ENGINE_HEAPS)):
+        blk = 512 if eng == ddp_amd.ENGINE_HEAPS else 1024
+        c_e, _, s_e = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_EUCLID, block=blk, engine=eng)
+        if not np.array_equal(c_e, costs):
+            raise SystemExit(f"engine {name} disagrees with the timed batch")
+        engines[name] = {"block": blk, "kernel_ms": round(s_e["kernel_ms"], 3),
+                         "queries_per_s": round(len(src) / (s_e["kernel_ms"] * 1e-3), 1),
+                         "expansions": s_e["expansions"], "edges": s_e["edges"]}
+    log(f"engines: {engines}")
 
     # the same queries, one at a time through the single-query GA* kernel
     rng = np.random.default_rng(1)
@@ -242,7 +262,9 @@ def run_c4(args):
                       "per step")
     res["config"].update({"N": hg.N, "E": hg.E, "queries": len(src), "block": info["block"],
                           "slots": info["slots"], "path_cap": 4096, "relabel": not args.no_reorder,
-                          "engine": "heaps" if engine == ddp_amd.ENGINE_HEAPS else "piles"})
+                          "engine": ENGINE_NAMES.get(engine, "piles"), "heap_cap": info["heap_cap"],
+                          "lds_cap": info["lds_cap"]})
+    res["engines"] = engines
     if len(sample):
         res["one_at_a_time"] = {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
                                 "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096}
@@ -254,6 +276,8 @@ def run_c4(args):
         "roofline": measured_traffic(roofline(algorithmic_bytes_geo(stats), kmean, "batch_kernel", copy_bandwidth(ctx)),
                                      workload="c4", block=info["block"], relabel=not args.no_reorder,
                                      queries=len(src), engine=res["config"]["engine"]),
+        "p50_p99_note": "p50_ms / p99_ms: completion times inside one batched launch of all queries (submitted at "
+                        "t = 0); request latencies under load are the served leg's",
     })
     if not args.no_serve:
         ups = geometric_updates(hg, args.serve_requests // 10, 100, 8)
@@ -294,8 +318,10 @@ def run_c1(args):
     h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
     r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup)
     other = {}
-    for name, mode in (("workgroup_mode", ddp_amd.MODE_WORKGROUP), ("pile_mode", ddp_amd.MODE_PILES)):
-        ro, wall_o, kms_o = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup, mode=mode)
+    for name, mode, heur in (("workgroup_mode", ddp_amd.MODE_WORKGROUP, ddp_amd.H_ARRAY),
+                             ("pile_mode", ddp_amd.MODE_PILES, ddp_amd.H_ARRAY),
+                             ("bidi_mode_h0", ddp_amd.MODE_BIDI, ddp_amd.H_ZERO)):
+        ro, wall_o, kms_o = time_single(ctx, dg, s, t, heur, 1024, args.steps, args.warmup, mode=mode)
         if ro.cost != r.cost:
             raise SystemExit(f"C1: {name} {ro.cost} != grid mode {r.cost}")
         so = ro.stats
@@ -334,12 +360,13 @@ def run_c2(args):
     hg = ddp_amd.HostGraph.grid_maze(W, W, 300, 2)
     s, t = 0, W * W - 1
     ctx = ddp_amd.Context(0)
-    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)  # transposed rows for the bidirectional mode
     dg.set_grid(W)
     sweep = {}
     cost = None
     for name, K, mode in (("512-workgroup", 512, ddp_amd.MODE_WORKGROUP), ("1024-workgroup", 1024,
                           ddp_amd.MODE_WORKGROUP), ("1024-piles", 1024, ddp_amd.MODE_PILES),
+                          ("512-bidi", 512, ddp_amd.MODE_BIDI), ("1024-bidi", 1024, ddp_amd.MODE_BIDI),
                           ("512-grid", 512, ddp_amd.MODE_GRID),
                           ("2048-grid", 2048, ddp_amd.MODE_GRID), ("8192-grid", 8192, ddp_amd.MODE_GRID)):
         r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_MANHATTAN, K, args.steps, args.warmup, mode=mode)
@@ -470,13 +497,23 @@ def run_c5(args):
     ups = rmat_updates(hg, max(1, args.c5_requests // 15), 1000, 9)
     ctx = ddp_amd.Context(0)
     t0 = time.time()
-    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=not args.no_reorder)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True, reorder=not args.no_reorder)
     log(f"uploaded in {time.time() - t0:.1f}s")
-    # batched kernel on all queries (unmodified graph): reference costs for the served leg
-    cost_b, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, heap_cap=args.c5_heap_cap)
-    if bst["errors"]:
-        raise SystemExit(f"C5 batch: {bst['errors']} queries failed")
-    log(f"C5 batch: {bst}")
+    # batched kernel on all queries (unmodified graph), one-way piles and bidirectional
+    batched = {}
+    cost_b = None
+    for name, eng in (("piles", ddp_amd.ENGINE_PILES), ("bidi", ddp_amd.ENGINE_BIDI)):
+        c, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, heap_cap=args.c5_heap_cap, engine=eng)
+        if bst["errors"]:
+            raise SystemExit(f"C5 batch ({name}): {bst['errors']} queries failed")
+        if cost_b is not None and not np.array_equal(c, cost_b):
+            raise SystemExit(f"C5 batch: {name} disagrees with piles")
+        cost_b = c
+        batched[name] = {"queries_per_s": round(len(src) / (bst["kernel_ms"] * 1e-3), 2),
+                         "kernel_ms": round(bst["kernel_ms"], 2), "edges": bst["edges"],
+                         "expansions": bst["expansions"], "block": bst["block"], "slots": bst["slots"],
+                         "hbm_algorithmic_bytes": 16 * bst["expansions"] + 16 * bst["edges"] + 8 * bst["improvements"]}
+        log(f"C5 batch {name}: {batched[name]}")
     lat1 = []
     for i in range(min(args.single_sample, 8)):
         q0 = time.perf_counter()
@@ -485,30 +522,58 @@ def run_c5(args):
         lat1.append((time.perf_counter() - q0) * 1e3)
         if r.cost != cost_b[i]:
             raise SystemExit(f"C5 query {i}: single {r.cost} batch {cost_b[i]}")
-    sweep = []
-    for qps in args.c5_qps:
-        o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_requests, ups, 11,
+    single_qps = 1000.0 / float(np.mean(lat1)) if lat1 else None
+    # served: closed loop (every request at once) for the capacity, then open
+    # loop at fractions of it for the latency curve
+    o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, 0.0, args.c5_requests, ups, 11,
+                                       heap_cap=args.c5_heap_cap)
+    closed = served_summary(o_, ty, qi, el, info, 0.0, cost_b)
+    log(f"C5 served closed loop {closed}")
+    cap = closed["queries_per_s"]
+    offered = args.c5_qps or [round(f * cap, 1) for f in (0.25, 0.5, 0.75, 0.9)]
+    sweep = [closed]
+    for qps in offered:
+        o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_sweep_requests, ups, 12,
                                            heap_cap=args.c5_heap_cap)
         sweep.append(served_summary(o_, ty, qi, el, info, qps, cost_b))
         log(f"C5 served {sweep[-1]}")
-    closed = [x for x in sweep if x["offered_qps"] <= 0]
-    cap = max(x["queries_per_s"] for x in (closed or sweep))
     under = [x for x in sweep if 0 < x["offered_qps"] < cap] or sweep
     lat_at = max(under, key=lambda x: x["offered_qps"])
+    best_batched = max(v["queries_per_s"] for v in batched.values())
     res = base_record(args, cap, "queries/s", None,
                       f"C5 online serving: R-MAT scale {scale} ({hg.N:,} nodes, {hg.E:,} edges, a/b/c/d = "
-                      f"0.57/0.19/0.19/0.05, seed 5), w in [1,1000], h = 0, {args.c5_queries} reachable pairs, "
+                      f"0.57/0.19/0.19/0.05, seed 5), w in [1,1000], h = 0, {args.c5_queries} connected pairs, "
                       f"95% queries, 5% update batches of 1,000 deletes + 1,000 inserts; value = closed-loop "
                       f"served capacity")
-    res["config"].update({"N": hg.N, "E": hg.E, "relabel": not args.no_reorder, "block": bst["block"],
-                          "slots": bst["slots"]})
+    res["config"].update({"N": hg.N, "E": hg.E, "relabel": not args.no_reorder, "slots": info["slots"],
+                          "serve_block": info["block"]})
     res.update({"p50_ms": lat_at["p50_ms"], "p99_ms": lat_at["p99_ms"], "latency_at_qps": lat_at["offered_qps"],
-                "sweep": sweep, "batched_queries_per_s": round(len(src) / (bst["kernel_ms"] * 1e-3), 2),
-                "one_at_a_time": ({"queries_per_s": round(1000.0 / float(np.mean(lat1)), 3),
-                                   "p50_ms": pct(lat1, 50), "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": 8192}
-                                  if lat1 else None)})
+                "sweep": sweep, "batched": batched, "batched_queries_per_s": best_batched,
+                "one_at_a_time": ({"queries_per_s": round(single_qps, 3), "p50_ms": pct(lat1, 50),
+                                   "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": 8192} if lat1 else None),
+                "batched_over_one_at_a_time": round(best_batched / single_qps, 2) if single_qps else None,
+                "served_over_one_at_a_time": round(cap / single_qps, 2) if single_qps else None})
     ctx.close()
     return res
+
+
+def summary_c3(r):
+    """The C3 line reduced to what the headline line carries."""
+    keep = ("value", "unit", "repair_ms_mean", "recompute_ms_mean", "repair_device_ms_mean",
+            "recompute_device_ms_mean", "speedup_vs_recompute")
+    out = {k: r[k] for k in keep}
+    out["config"] = r["config"]
+    out["batches"] = len(r["batches"])
+    out["checked"] = "repair == recompute on every batch; oracle Dijkstra on the merged graph every 4th and the last"
+    return out
+
+
+def summary_c5(r):
+    keep = ("value", "unit", "p50_ms", "p99_ms", "latency_at_qps", "sweep", "batched", "batched_queries_per_s",
+            "one_at_a_time", "batched_over_one_at_a_time", "served_over_one_at_a_time")
+    out = {k: r[k] for k in keep}
+    out["config"] = r["config"]
+    return out
 
 
 def main():
@@ -526,6 +591,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1),
                     help="host threads of the batched CPU baseline (the GPU box's share is 16)")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-engines", action="store_true", help="c4: skip the side-by-side run of every engine")
     ap.add_argument("--no-reorder", action="store_true",
                     help="keep the input node numbering on the device (default: Hilbert/BFS relabelling)")
     ap.add_argument("--serve-qps", type=float, default=5000.0,
@@ -540,14 +606,22 @@ def main():
     ap.add_argument("--c5-queries", type=int, default=256)
     ap.add_argument("--c5-requests", type=int, default=1024)
     ap.add_argument("--c5-heap-cap", type=int, default=0, help="queue entries per lane (0 = library default)")
-    ap.add_argument("--c5-qps", type=float, nargs="+", default=[0.0, 40.0],
-                    help="offered loads of the served leg (0 = closed loop, all requests at once)")
+    ap.add_argument("--c5-qps", type=float, nargs="+", default=None,
+                    help="offered loads of the open-loop served points (default: 25/50/75/90%% of the "
+                         "closed-loop capacity)")
+    ap.add_argument("--c5-sweep-requests", type=int, default=256, help="requests per open-loop point")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="c4 only: skip the C3 and C5 legs that the default line carries as extra keys")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); every box has a single MI355X "
               f"and no multi-GPU path is built", file=sys.stderr)
         sys.exit(2)
     res = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4, "c5": run_c5}[args.workload](args)
+    if args.workload == "c4" and not args.no_extra:
+        # the other served/dynamic configurations ride along as extra keys
+        res["c3"] = summary_c3(run_c3(args))
+        res["c5"] = summary_c5(run_c5(args))
     print(json.dumps(res), flush=True)
 
 

@@ -13,15 +13,15 @@ from ctypes import byref, c_int32, c_uint32, c_void_p
 import numpy as np
 
 from . import _capi
-from ._capi import (DDPError, ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES, GRAPH_REORDER, GRAPH_REVERSE, H_ARRAY, H_EUCLID,
+from ._capi import (DDPError, ENGINE_AUTO, ENGINE_BIDI, ENGINE_HEAPS, ENGINE_PILES, GRAPH_REORDER, GRAPH_REVERSE, H_ARRAY, H_EUCLID,
                     H_MANHATTAN, H_ZERO, INF_COST, MODE_AUTO,
-                    MODE_GRID, MODE_PILES, MODE_WORKGROUP, AstarOpts,
+                    MODE_BIDI, MODE_GRID, MODE_PILES, MODE_WORKGROUP, AstarOpts,
                     AstarStats, BatchOpts, BatchStats, DeviceInfo, Heur, Update, as_i32, as_u32, ptr_f64, ptr_i32,
                     ptr_u32)
 
 __all__ = [
     "DDPError", "INF_COST", "H_ZERO", "H_ARRAY", "H_MANHATTAN", "H_EUCLID", "MODE_AUTO", "MODE_GRID",
-    "MODE_WORKGROUP", "MODE_PILES", "ENGINE_AUTO", "ENGINE_HEAPS", "ENGINE_PILES", "Context", "HostGraph",
+    "MODE_WORKGROUP", "MODE_PILES", "MODE_BIDI", "ENGINE_AUTO", "ENGINE_HEAPS", "ENGINE_PILES", "ENGINE_BIDI", "Context", "HostGraph",
     "DeviceGraph", "AStarResult", "astar", "Batch", "astar_batch", "Updates", "GPU_Graph", "GPU_A_Star",
     "DAStar", "Server", "alternative_paths",
 ]
@@ -485,9 +485,11 @@ class DAStar:
 
     REPAIR, RECOMPUTE = 0, 1
 
-    def __init__(self, ctx, hg, s, t, h=None, K=0, block=0, heap_cap=0, lds_cap=0):
+    def __init__(self, ctx, hg, s, t, h=None, K=0, block=0, heap_cap=0, lds_cap=0, mode=MODE_AUTO):
+        """mode: MODE_PILES runs every search (initial, resumed, recomputed) on
+        one workgroup's pile engine; anything else the GA* grid kernel."""
         self.ctx, self.N, self.s, self.t = ctx, hg.N, s, t
-        o = AstarOpts(K, block, heap_cap, lds_cap, 0)
+        o = AstarOpts(K, block, heap_cap, lds_cap, 0, mode)
         hh = None if h is None else as_i32(h)
         handle = c_void_p()
         cost = c_int32()
@@ -504,6 +506,14 @@ class DAStar:
         _check(_capi.lib().ddp_dastar_apply(self._h, arr, len(lines), mode, byref(cost), byref(st)), self.ctx.handle)
         self.cost = cost.value
         return cost.value, {f: getattr(st, f) for f, _ in _capi.DAStats._fields_}
+
+    def compact(self):
+        """Fold the inserted-edge log into the CSR rows (ddp_dastar_compact)."""
+        _check(_capi.lib().ddp_dastar_compact(self._h), self.ctx.handle)
+
+    def set_compaction(self, log_entries):
+        """Compact after any batch that leaves more than log_entries in the log (0: never)."""
+        _check(_capi.lib().ddp_dastar_set_compaction(self._h, log_entries), self.ctx.handle)
 
     def path(self):
         buf = np.zeros(self.N, np.int32)
@@ -591,6 +601,10 @@ class Server:
 
     def drain(self, timeout_ms=600000):
         _check(_capi.lib().ddp_server_drain(self._h, timeout_ms), self.ctx.handle)
+
+    def compact(self):
+        """Fold the inserted-edge log into CSR rows (ddp_server_compact)."""
+        _check(_capi.lib().ddp_server_compact(self._h), self.ctx.handle)
 
     def replay(self, arrival_ms, types, src, dst, updates):
         """Open-loop replay; updates: list aligned with requests (None for

@@ -22,8 +22,8 @@ ERR_NAMES = {
 INF_COST = 2147483647
 
 H_ZERO, H_ARRAY, H_MANHATTAN, H_EUCLID = 0, 1, 2, 3
-MODE_AUTO, MODE_GRID, MODE_WORKGROUP, MODE_PILES = 0, 1, 2, 3  # DDP_ASTAR_*
-ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES = 0, 1, 2  # DDP_BATCH_*
+MODE_AUTO, MODE_GRID, MODE_WORKGROUP, MODE_PILES, MODE_BIDI = 0, 1, 2, 3, 4  # DDP_ASTAR_*
+ENGINE_AUTO, ENGINE_HEAPS, ENGINE_PILES, ENGINE_BIDI = 0, 1, 2, 3  # DDP_BATCH_*
 GRAPH_REVERSE = 1
 GRAPH_REORDER = 2
 
@@ -75,7 +75,8 @@ class DAStats(ctypes.Structure):
     _fields_ = [("inserts", c_int32), ("deletes", c_int32), ("deletes_effective", c_int32), ("roots", c_int32),
                 ("invalidated", c_int32), ("repulled", c_int32), ("pushes", c_int32), ("iterations", c_int32),
                 ("expansions", c_int64), ("edges", c_int64), ("improvements", c_int64), ("host_ms", c_float),
-                ("repair_ms", c_float), ("search_ms", c_float), ("total_ms", c_float)]
+                ("repair_ms", c_float), ("search_ms", c_float), ("total_ms", c_float), ("compacted", c_int32),
+                ("compact_ms", c_float)]
 
 
 class ServerOpts(ctypes.Structure):
@@ -130,6 +131,8 @@ SIGNATURES = {
     "ddp_dastar_apply": (c_int, [c_void_p, POINTER(Update), c_int32, c_int32, P_I32, POINTER(DAStats)]),
     "ddp_dastar_path": (c_int, [c_void_p, P_I32, c_int32, P_I32]),
     "ddp_dastar_num_edges": (c_int, [c_void_p, P_I32]),
+    "ddp_dastar_compact": (c_int, [c_void_p]),
+    "ddp_dastar_set_compaction": (c_int, [c_void_p, c_int64]),
     "ddp_dastar_export": (c_int, [c_void_p, P_I32, P_I32, P_U32]),
     "ddp_dastar_close": (None, [c_void_p]),
     # ddp_server.h
@@ -139,6 +142,7 @@ SIGNATURES = {
     "ddp_server_submit_update": (c_int, [c_void_p, POINTER(Update), c_int32, c_uint64]),
     "ddp_server_poll": (c_int, [c_void_p, POINTER(ServerResult), c_int32, P_I32]),
     "ddp_server_drain": (c_int, [c_void_p, c_int32]),
+    "ddp_server_compact": (c_int, [c_void_p]),
     "ddp_server_destroy": (None, [c_void_p]),
     "ddp_server_replay": (c_int, [c_void_p, c_int32, P_F64, P_I32, P_I32, P_I32, P_I32, POINTER(Update),
                                   POINTER(ServerResult)]),

@@ -20,7 +20,8 @@ def tune_batch(ctx, dg, src, dst, heur, scale=1, sample=1024):
     q = c_float()
     _check(_capi.lib().ddp_tune_batch(ctx.handle, dg.handle, Heur(heur, scale), src.shape[0], ptr_i32(src),
                                       ptr_i32(dst), sample, byref(o), byref(q)), ctx)
-    return {"block": o.block, "engine": o.engine, "queries_per_s": q.value}
+    return {"block": o.block, "engine": o.engine, "heap_cap": o.heap_cap, "lds_cap": o.lds_cap,
+            "queries_per_s": q.value}
 
 
 def pick_batch_block(ctx, dg, src, dst, heur, scale=1, sample=1024):
@@ -33,4 +34,5 @@ def tune_astar(ctx, dg, src, dst, heur, scale=1, sample=16):
     ms = c_float()
     _check(_capi.lib().ddp_tune_astar(ctx.handle, dg.handle, Heur(heur, scale), src.shape[0], ptr_i32(src),
                                       ptr_i32(dst), sample, byref(o), byref(ms)), ctx)
-    return {"K": o.K, "block": o.block, "mode": o.mode, "mean_ms": ms.value}
+    return {"K": o.K, "block": o.block, "mode": o.mode, "heap_cap": o.heap_cap, "lds_cap": o.lds_cap,
+            "mean_ms": ms.value}

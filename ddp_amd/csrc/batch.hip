@@ -16,20 +16,23 @@
 #include <cstring>
 #include <numeric>
 
+#include <type_traits>
+
 #include "../../include/ddp_batch.h"
-#include "slot_search.h"
+#include "bidi_search.h"
 
 namespace {
 
 struct BatchArgs {
     GraphView g;
+    GraphView gr;        // transposed rows (bidirectional engine)
     HeurView h;
     const int32_t *qs, *qt, *order;
     int32_t nq;
     uint32_t *next;
-    uint64_t *gp;        // slots * N
-    int32_t *touched;    // slots * N
-    uint64_t *home;      // slots * cap * B
+    uint64_t *gp;        // slots * N (bidirectional: slots * 2N, forward then backward)
+    int32_t *touched;    // as gp
+    uint64_t *home;      // slots * cap * B (bidirectional: twice that, one half per direction)
     int32_t *cost, *plen, *path;
     int32_t path_cap;
     uint32_t *qerr;
@@ -40,22 +43,23 @@ struct BatchArgs {
     uint32_t lring;               // pile engine: entries of the LDS near ring (dynamic LDS)
 };
 
-enum { ENG_HEAPS = 0, ENG_PILES = 1 };
+enum { ENG_HEAPS = 0, ENG_PILES = 1, ENG_BIDI = 2 };
 
 template <int B, int HK, int ENG>
 __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
 {
-    extern __shared__ uint64_t s_heap[];  // lcap * B (heap engine)
+    extern __shared__ uint64_t s_heap[];  // lcap * B (heap engine), near rings (pile engines)
     constexpr int SV = (ENG == ENG_PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
-    __shared__ SlotLds<B, SV> S;
+    __shared__ std::conditional_t<ENG == ENG_BIDI, BidiLds<B>, SlotLds<B, SV>> S;
     __shared__ int32_t s_q;
 
     const int lane = threadIdx.x;
     const int64_t slot = blockIdx.x;
     const int32_t N = a.N;
-    uint64_t *gp = a.gp + slot * N;
-    int32_t *touched = a.touched + slot * N;
-    uint64_t *home = a.home + slot * (int64_t)a.cap * B;  // this slot's queue memory
+    constexpr int DIRS = ENG == ENG_BIDI ? 2 : 1;
+    uint64_t *gp = a.gp + slot * DIRS * N;
+    int32_t *touched = a.touched + slot * DIRS * N;
+    uint64_t *home = a.home + slot * DIRS * (int64_t)a.cap * B;  // this slot's queue memory
     const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
     const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
     ExpandCounters C;
@@ -68,19 +72,40 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
         const int32_t q = a.order ? a.order[qi] : qi;
         const int32_t s = a.qs[q], t = a.qt[q];
         if (lane == 0) a.qtime[2 * (int64_t)q] = dev_clock_100mhz();
-        if constexpr (ENG == ENG_PILES)
-            slot_search_piles<B, HK>(a.g, a.h, gp, touched, P, N, s, t, a.maxit, a.delta0, S, C, stale);
-        else
-            slot_search<B, HK>(a.g, a.h, gp, touched, H, s_heap, N, s, t, a.maxit, S, C, stale);
-        if (a.path_cap > 0) {
-            slot_path(gp, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
-            if (lane == 0) a.plen[q] = S.plen;
-        }
-        slot_reset(gp, touched, S);
-        if (lane == 0) {
-            a.cost[q] = (int32_t)S.F;
-            a.qerr[q] = S.err;
-            a.qtime[2 * (int64_t)q + 1] = dev_clock_100mhz();
+        if constexpr (ENG == ENG_BIDI) {
+            // one half of the queue memory and of the LDS near ring per direction
+            const uint64_t *bh = home + (int64_t)a.cap * B;
+            const PilePtrs PB{(uint64_t *)bh, (uint64_t *)bh + a.ncap, (uint64_t *)bh + a.ncap + a.fcap, a.ncap, a.fcap,
+                              s_heap + a.lring / 2, a.lring / 2};
+            const PilePtrs PF{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring / 2};
+            const BidiSide D{{a.g.off, a.gr.off}, {a.g.col, a.gr.col}, {a.g.w, a.gr.w}, {gp, gp + N},
+                             {touched, touched + N}, {&PF, &PB}};
+            slot_search_bidi<B, HK>(D, a.h, a.g.relax_min != 0, N, s, t, a.maxit, a.delta0, S, C, stale);
+            if (a.path_cap > 0) {
+                slot_path_bidi(gp, gp + N, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
+                if (lane == 0) a.plen[q] = S.plen;
+            }
+            slot_reset_bidi(gp, gp + N, touched, touched + N, S);
+            if (lane == 0) {
+                a.cost[q] = (int32_t)(uint32_t)(S.meet >> 32);
+                a.qerr[q] = S.err;
+                a.qtime[2 * (int64_t)q + 1] = dev_clock_100mhz();
+            }
+        } else {
+            if constexpr (ENG == ENG_PILES)
+                slot_search_piles<B, HK>(a.g, a.h, gp, touched, P, N, s, t, a.maxit, a.delta0, S, C, stale);
+            else
+                slot_search<B, HK>(a.g, a.h, gp, touched, H, s_heap, N, s, t, a.maxit, S, C, stale);
+            if (a.path_cap > 0) {
+                slot_path(gp, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);
+                if (lane == 0) a.plen[q] = S.plen;
+            }
+            slot_reset(gp, touched, S);
+            if (lane == 0) {
+                a.cost[q] = (int32_t)S.F;
+                a.qerr[q] = S.err;
+                a.qtime[2 * (int64_t)q + 1] = dev_clock_100mhz();
+            }
         }
         if (__ballot(C.edges >= (1u << 30))) slot_flush_stats<B>(a.stats, C, stale);  // before 32 bits wrap
     }
@@ -108,7 +133,9 @@ BatchFn pick_b(int hk)
 {
     switch (hk) {
     case DDP_H_ZERO: return batch_kernel<B, DDP_H_ZERO, ENG>;
-    case DDP_H_ARRAY: return batch_kernel<B, DDP_H_ARRAY, ENG>;
+    case DDP_H_ARRAY:
+        if constexpr (ENG == ENG_BIDI) return nullptr;  // the potential needs h toward s as well as t
+        else return batch_kernel<B, DDP_H_ARRAY, ENG>;
     case DDP_H_MANHATTAN: return batch_kernel<B, DDP_H_MANHATTAN, ENG>;
     default: return batch_kernel<B, DDP_H_EUCLID, ENG>;
     }
@@ -117,6 +144,14 @@ BatchFn pick_b(int hk)
 template <int ENG>
 BatchFn pick_be(int B, int hk)
 {
+    if constexpr (ENG == ENG_BIDI) {  // two directions need half a block each: 256 lanes and up
+        switch (B) {
+        case 256: return pick_b<256, ENG>(hk);
+        case 512: return pick_b<512, ENG>(hk);
+        case 1024: return pick_b<1024, ENG>(hk);
+        default: return nullptr;
+        }
+    }
     switch (B) {
     case 64: return pick_b<64, ENG>(hk);
     case 128: return pick_b<128, ENG>(hk);
@@ -128,6 +163,7 @@ BatchFn pick_be(int B, int hk)
 
 BatchFn pick_batch(int B, int hk, int eng)
 {
+    if (eng == ENG_BIDI) return pick_be<ENG_BIDI>(B, hk);
     return eng == ENG_PILES ? pick_be<ENG_PILES>(B, hk) : pick_be<ENG_HEAPS>(B, hk);
 }
 
@@ -185,25 +221,34 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     int rc = ddp_make_heur(g, h, -1, &hv);
     if (rc) return rc;
     const int32_t engine = o ? o->engine : DDP_BATCH_AUTO;
-    DDP_CHECK(engine >= DDP_BATCH_AUTO && engine <= DDP_BATCH_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
-    const int eng = engine == DDP_BATCH_HEAPS ? ENG_HEAPS : ENG_PILES;  // AUTO: piles (3x on the C4 batch)
-    int32_t B = (o && o->block) ? o->block : (eng == ENG_PILES ? 1024 : 256);
+    DDP_CHECK(engine >= DDP_BATCH_AUTO && engine <= DDP_BATCH_BIDI, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
+    const int eng = engine == DDP_BATCH_HEAPS ? ENG_HEAPS : engine == DDP_BATCH_BIDI ? ENG_BIDI : ENG_PILES;
+    const bool piles = eng != ENG_HEAPS;  // both pile engines
+    int32_t B = (o && o->block) ? o->block : (piles ? 1024 : 256);
     DDP_CHECK(B == 64 || B == 128 || B == 256 || B == 512 || B == 1024, ctx, DDP_ERR_ARG,
               "block must be 64, 128, 256, 512 or 1024 (got %d)", B);
+    if (eng == ENG_BIDI) {
+        DDP_CHECK(hv.kind != DDP_H_ARRAY, ctx, DDP_ERR_ARG,
+                  "the bidirectional engine needs a closed-form heuristic (DDP_H_ZERO, _MANHATTAN or _EUCLID)");
+        DDP_CHECK(g->roff, ctx, DDP_ERR_STATE, "the bidirectional engine needs a graph uploaded with DDP_GRAPH_REVERSE");
+        DDP_CHECK(B >= 256, ctx, DDP_ERR_ARG, "the bidirectional engine runs on blocks of 256, 512 or 1024 lanes");
+    }
+    const int DIRS = eng == ENG_BIDI ? 2 : 1;
     // deeper LDS heaps keep sift-downs out of HBM (about 10% on the C4 batch at
     // block 512, where occupancy is set by waves rather than LDS); the pile
     // engine keeps its queues in HBM and needs no dynamic LDS
     int32_t lcap = (o && o->lds_cap) ? o->lds_cap : std::max(4, std::min(32, 8192 / B));
-    if (eng == ENG_PILES) lcap = 0;
+    if (piles) lcap = 0;
     DDP_CHECK((int64_t)lcap * B * 8 <= 64 * 1024, ctx, DDP_ERR_ARG, "lds_cap %d x block %d exceeds 64 KiB", lcap, B);
     int32_t cap = (o && o->heap_cap) ? o->heap_cap : std::max(lcap, 2048);
-    if (eng == ENG_PILES && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot
+    if (piles && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot and direction
         cap = (int32_t)std::min<int64_t>(std::max<int64_t>(cap, (3LL * g->N + B - 1) / B), INT32_MAX / 2);
     cap = std::max(cap, std::max(lcap, 16));
     // pile engine: the dynamic LDS holds the near ring (8 entries per lane,
     // 64 KiB at 1,024 lanes, which the one-slot-per-CU occupancy leaves free)
-    const uint32_t lring = eng == ENG_PILES ? 8u * (uint32_t)B : 0u;
-    const size_t lds = eng == ENG_PILES ? (size_t)lring * 8 : (size_t)lcap * B * 8;
+    const uint32_t lring = piles ? 8u * (uint32_t)B : 0u;  // bidirectional: half per direction
+    const size_t lds = piles ? (size_t)lring * 8 : (size_t)lcap * B * 8;
+    DDP_CHECK(pick_batch(B, hv.kind, eng), ctx, DDP_ERR_ARG, "no batch kernel for block %d and this engine", B);
     int per_cu = 0;
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)pick_batch(B, hv.kind, eng),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -215,7 +260,7 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     // admission by free HBM: keep 10% headroom
     size_t fr = 0, tot = 0;
     DDP_HIP(ctx, hipMemGetInfo(&fr, &tot));
-    const size_t per_slot = (size_t)g->N * 12 + (size_t)cap * B * 8;
+    const size_t per_slot = DIRS * ((size_t)g->N * 12 + (size_t)cap * B * 8);
     const size_t fixed = (size_t)max_queries * (7 * 4 + 16 + (size_t)std::max(0, o ? o->path_cap : 0) * 4) + 4096;
     const size_t budget = fr > fixed ? (size_t)((fr - fixed) * 0.9) : 0;
     slots = std::min<int64_t>(slots, (int64_t)(budget / per_slot));
@@ -263,16 +308,16 @@ extern "C" int ddp_batch_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, 
     al((void **)&b->path, M * (size_t)b->path_cap * 4);
     al((void **)&b->next, 256);
     al((void **)&b->stats, 64);
-    al((void **)&b->gp, (size_t)slots * g->N * 8);
-    al((void **)&b->touched, (size_t)slots * g->N * 4);
-    al((void **)&b->home, (size_t)slots * cap * B * 8);
+    al((void **)&b->gp, (size_t)slots * DIRS * g->N * 8);
+    al((void **)&b->touched, (size_t)slots * DIRS * g->N * 4);
+    al((void **)&b->home, (size_t)slots * DIRS * cap * B * 8);
     if (e != hipSuccess) {
         batch_free(b);
         delete b;
         return ddp_hip_err(ctx, e, "batch allocation", __FILE__, __LINE__);
     }
     // one-time clear of every slot's (f, parent) array
-    hipLaunchKernelGGL(batch_fill_u64, dim3(4096), dim3(256), 0, ctx->stream, b->gp, (int64_t)slots * g->N,
+    hipLaunchKernelGGL(batch_fill_u64, dim3(4096), dim3(256), 0, ctx->stream, b->gp, (int64_t)slots * DIRS * g->N,
                        (uint64_t)DDP_GP_INF);
     e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
@@ -363,6 +408,12 @@ static int batch_capture(ddp_batch *b)
     BatchArgs a{};
     a.g = ddp_view(b->g);
     a.g.relax_min = b->g->minw >= 1;
+    if (b->g->roff) {  // transposed rows (the bidirectional engine walks them backward from t)
+        a.gr = a.g;
+        a.gr.off = b->g->roff;
+        a.gr.col = b->g->rcol;
+        a.gr.w = b->g->rw;
+    }
     a.h = b->hv;
     a.qs = b->qs;
     a.qt = b->qt;

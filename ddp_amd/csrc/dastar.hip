@@ -31,8 +31,9 @@
 #include <vector>
 
 #include "../../include/ddp_dastar.h"
+#include "compact.h"
 #include "gastar.h"
-#include "search_expand.h"
+#include "slot_search.h"
 
 namespace {
 
@@ -345,6 +346,59 @@ __global__ void relax_inserts(const int32_t *ins, int32_t n, const int32_t *h, u
     P.push(has, v, cand);
 }
 
+// The search behind DA* in pile mode: one workgroup runs the pile engine of
+// the batched slots (slot_search.h) resumed from the retained open set -- the
+// dormant pile of the previous search plus this batch's repair pushes -- with
+// F(t) read from the target's word.  Its rounds synchronise through LDS
+// barriers only, where the grid mode pays a device-wide barrier per
+// iteration; a repair's few-hop propagation is bound by that per-round
+// latency.  Entries that cannot beat F(t) now are kept (dormant) for later
+// batches, which may raise F(t) again.
+constexpr int kPileBlock = 1024;
+
+struct DaPileArgs {
+    GraphView g;
+    HeurView h;
+    uint64_t *gp;
+    uint64_t *open, *alt, *dorm;  // open set in, far-pile partner, dormant pile out (cap entries each)
+    uint64_t *near;               // ncap entries
+    uint32_t cap, ncap, n_open, lring, delta0;
+    const uint64_t *push;         // repair pushes of this batch
+    const uint32_t *push_n;
+    uint32_t pcap;
+    int32_t N, t, maxit;
+    uint32_t *res;                // iterations, F(t), error flags, dormant entries
+    unsigned long long *stats;
+};
+
+__global__ void __launch_bounds__(kPileBlock) dastar_pile_kernel(DaPileArgs a)
+{
+    extern __shared__ uint64_t s_ring[];
+    __shared__ SlotLds<kPileBlock, kPileBlock> S;
+    const int lane = threadIdx.x;
+    // the repair pushes join the open set at the end of its buffer
+    const uint32_t np0 = __hip_atomic_load(a.push_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t np = np0 < a.pcap ? np0 : a.pcap;
+    const bool fits = (uint64_t)a.n_open + np <= a.cap;
+    if (fits)
+        for (uint32_t i = lane; i < np; i += kPileBlock) a.open[a.n_open + i] = a.push[i];
+    block_sync_global();
+    const uint32_t F0 = (uint32_t)(ld_gp(a.gp + a.t) >> 32);
+    const PilePtrs P{a.near, a.open, a.alt, a.ncap, a.cap, s_ring, a.lring};
+    const PileResume Z{a.dorm, a.cap, fits ? a.n_open + np : 0u, F0};
+    ExpandCounters C;
+    uint32_t stale = 0;
+    slot_search_piles<kPileBlock, DDP_H_ARRAY, 1, true>(a.g, a.h, a.gp, nullptr, P, a.N, -1, a.t, a.maxit, a.delta0, S,
+                                                        C, stale, &Z);
+    if (lane == 0) {
+        a.res[0] = (uint32_t)S.iters;
+        a.res[1] = S.F;
+        a.res[2] = S.err | (fits && np0 <= a.pcap ? 0u : (uint32_t)QERR_HEAP);
+        a.res[3] = S.dorm_n;
+    }
+    slot_flush_stats<kPileBlock>(a.stats, C, stale);
+}
+
 __global__ void map_h(const int32_t *in, int32_t *out, int32_t N)
 {
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
@@ -381,6 +435,8 @@ struct ddp_dastar {
     // ... and the log of inserted edges
     InsLog L{};
     int32_t log_cap = 0, log_used = 0;
+    int64_t compact_at = 0;     // log entries that trigger a compaction after a batch (0 = never)
+    int32_t compactions = 0;
     int32_t *d_h = nullptr;
     int32_t *d_mark = nullptr, *d_front = nullptr, *d_marked = nullptr;
     uint32_t *d_rctrl = nullptr;
@@ -393,6 +449,15 @@ struct ddp_dastar {
     HeurView hv{};
     int32_t rr = 0;
     uint32_t F = DDP_INF_F;
+    // pile mode (DDP_ASTAR_PILES): three rotating buffers (open set in,
+    // far-pile partner, dormant pile out), the HBM near ring and result words
+    bool piles = false;
+    uint64_t *pbuf[3] = {nullptr, nullptr, nullptr};
+    uint64_t *pnear = nullptr;
+    uint32_t *pres = nullptr;
+    uint32_t pcap_open = 0, pncap = 0;
+    int32_t open_at = 0;   // which buffer holds the open set
+    uint32_t open_n = 0;   // its entries
 };
 
 static void log_free(InsLog &L)
@@ -410,7 +475,8 @@ static void dastar_free(ddp_dastar *d)
     ddp_ga_free(&d->S);
     log_free(d->L);
     void *ptrs[] = {d->L.fhead, d->L.rhead, d->L.ftail, d->L.rtail, d->d_reid, d->d_h,
-                    d->d_mark, d->d_front, d->d_marked, d->d_rctrl, d->d_work};
+                    d->d_mark, d->d_front, d->d_marked, d->d_rctrl, d->d_work,
+                    d->pbuf[0], d->pbuf[1], d->pbuf[2], d->pnear, d->pres};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->g) ddp_graph_destroy(d->g);
@@ -499,6 +565,102 @@ static int log_reserve(ddp_dastar *d, int32_t need)
     return DDP_OK;
 }
 
+// Folds the log into the base rows (merge_graph / mergeDiff,
+// DA_star/DA_dyn_insert_del_nocycle.cu:940-980, 2236-2284): the base rows
+// become the merged rows of the current version (tombstones dropped, live
+// inserts appended in insert order), the transposed rows are rebuilt from
+// them, and the log is emptied.  Nodes keep their ids, so the search state
+// (g values, parents, queues) carries over unchanged.
+static int dastar_compact(ddp_dastar *d)
+{
+    ddp_ctx *ctx = d->ctx;
+    const MergeSrc m{d->g->off, d->g->col, d->g->w, d->N, nullptr, INT32_MAX - 1, d->L.fhead, d->L.fnext,
+                     d->L.v, d->L.w, d->L.del};
+    MergedCsr f, r;
+    int rc = ddp_merge_rows(ctx, m, &f);
+    if (rc) return rc;
+    if ((rc = ddp_transpose_rows(ctx, f, true, &r))) {
+        ddp_merged_free(&f);
+        return rc;
+    }
+    void *old[] = {d->g->off, d->g->col, d->g->w, d->g->roff, d->g->rcol, d->g->rw, d->d_reid};
+    for (void *p : old)
+        if (p) hipFree(p);
+    d->g->off = f.off;
+    d->g->col = f.col;
+    d->g->w = f.w;
+    d->g->E = f.E;
+    d->g->roff = r.off;
+    d->g->rcol = r.col;
+    d->g->rw = r.w;
+    d->g->RE = r.E;
+    d->d_reid = r.eid;
+    d->E = f.E;
+    const int32_t N = d->N;
+    hipLaunchKernelGGL(fill_u64, dim3(1024), dim3(256), 0, ctx->stream, d->L.fhead, (int64_t)N, ~0ull);
+    hipLaunchKernelGGL(fill_u64, dim3(1024), dim3(256), 0, ctx->stream, d->L.rhead, (int64_t)N, ~0ull);
+    hipLaunchKernelGGL(fill_i32, dim3(1024), dim3(256), 0, ctx->stream, d->L.ftail, (int64_t)N, -1);
+    hipLaunchKernelGGL(fill_i32, dim3(1024), dim3(256), 0, ctx->stream, d->L.rtail, (int64_t)N, -1);
+    DDP_HIP(ctx, hipGetLastError());
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    d->log_used = 0;
+    d->live_base = f.E;
+    d->live_log = 0;
+    d->compactions += 1;
+    return DDP_OK;
+}
+
+// One pile-mode search over the open set d->pbuf[open_at] (open_n entries)
+// plus the repair pushes counted in control slot 2; the dormant pile it leaves
+// becomes the next open set.
+static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
+{
+    ddp_ctx *ctx = d->ctx;
+    DaPileArgs a{};
+    a.g = fwd_view(d);
+    a.g.relax_min = d->g->minw >= 1;
+    a.h = d->hv;
+    a.gp = d->S.gp;
+    a.open = d->pbuf[d->open_at];
+    a.alt = d->pbuf[(d->open_at + 1) % 3];
+    a.dorm = d->pbuf[(d->open_at + 2) % 3];
+    a.near = d->pnear;
+    a.cap = d->pcap_open;
+    a.ncap = d->pncap;
+    a.n_open = d->open_n;
+    a.lring = 8u * kPileBlock;
+    a.delta0 = std::max<uint32_t>(1, d->g->meanw);
+    a.push = d->S.pbuf + d->P.pcap;
+    a.push_n = d->S.ctrl + 2 * GA_SLOT_WORDS + GA_S_PUSHCNT;
+    a.pcap = (uint32_t)d->P.pcap;
+    a.N = d->N;
+    a.t = d->t;
+    a.maxit = d->P.maxit;
+    a.res = d->pres;
+    a.stats = d->S.stats;
+    const size_t lds = (size_t)a.lring * 8;
+    DDP_HIP(ctx, hipFuncSetAttribute((const void *)dastar_pile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds));
+    DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(dastar_pile_kernel, dim3(1), dim3(kPileBlock), lds, ctx->stream, a);
+    DDP_HIP(ctx, hipGetLastError());
+    DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    uint32_t res[4];
+    DDP_HIP(ctx, hipMemcpyAsync(res, d->pres, 16, hipMemcpyDeviceToHost, ctx->stream));
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *kms = ms;
+    *iters = (int32_t)res[0];
+    DDP_CHECK(!(res[2] & QERR_ITERS), ctx, DDP_ERR_TIMEOUT, "DA*: iteration limit %d reached", d->P.maxit);
+    DDP_CHECK(!(res[2] & QERR_HEAP), ctx, DDP_ERR_OVERFLOW,
+              "DA*: the pile-mode open set outgrew its %u entries", d->pcap_open);
+    d->F = res[1];
+    d->open_at = (d->open_at + 2) % 3;
+    d->open_n = res[3];
+    return DDP_OK;
+}
+
 static int fresh_search(ddp_dastar *d, int32_t *iters, float *kms)
 {
     ddp_ctx *ctx = d->ctx;
@@ -514,6 +676,16 @@ static int fresh_search(ddp_dastar *d, int32_t *iters, float *kms)
         DDP_HIP(ctx, hipMemcpyAsync(d->S.gp + d->s, &word, 8, hipMemcpyHostToDevice, ctx->stream));
         DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return DDP_OK;
+    }
+    if (d->piles) {
+        // the open set is s alone; no repair pushes
+        const uint64_t word = ((uint64_t)(uint32_t)hs << 32) | 0xFFFFFFFFull;
+        const uint64_t key = ((uint64_t)(uint32_t)hs << 32) | (uint32_t)d->s;
+        DDP_HIP(ctx, hipMemcpyAsync(d->S.gp + d->s, &word, 8, hipMemcpyHostToDevice, ctx->stream));
+        DDP_HIP(ctx, hipMemcpyAsync(d->pbuf[d->open_at], &key, 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = ddp_ga_prepare_resume(ctx, d->S))) return rc;
+        d->open_n = 1;
+        return pile_run(d, iters, kms);
     }
     if ((rc = ddp_ga_seed(ctx, d->P, d->S, d->s, (uint32_t)hs))) return rc;
     return ddp_ga_run(ctx, fwd_view(d), d->hv, d->P, d->S, d->t, DDP_INF_F, 0, &d->F, &d->rr, iters, kms);
@@ -548,6 +720,7 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
     for (int32_t i = 0; i < N; ++i) roff[i + 1] += roff[i];
     const int32_t RE = roff[N];
     d->live_base = RE;
+    d->compact_at = std::max<int64_t>(65536, RE / 4);
     rcol.assign(RE, 0);
     rw.assign(RE, 0);
     reid.assign(RE, 0);
@@ -616,6 +789,24 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
         dastar_free(d);
         delete d;
         return rc;
+    }
+    d->piles = opts && opts->mode == DDP_ASTAR_PILES;
+    if (d->piles) {
+        // room for three open sets of an exhaustive search each; the near ring
+        // a power of two of about a quarter of that
+        d->pcap_open = (uint32_t)std::min<int64_t>(std::max<int64_t>(1 << 20, 3LL * N), 1LL << 31);
+        uint32_t nc = 1;
+        while (nc * 2 <= std::max<uint32_t>(d->pcap_open / 4, 1u << 16)) nc *= 2;
+        d->pncap = nc;
+        hipError_t e2 = hipSuccess;
+        for (int k = 0; k < 3 && e2 == hipSuccess; ++k) e2 = hipMalloc((void **)&d->pbuf[k], (size_t)d->pcap_open * 8);
+        if (e2 == hipSuccess) e2 = hipMalloc((void **)&d->pnear, (size_t)nc * 8);
+        if (e2 == hipSuccess) e2 = hipMalloc((void **)&d->pres, 64);
+        if (e2 != hipSuccess) {
+            dastar_free(d);
+            delete d;
+            return ddp_hip_err(ctx, e2, "DA* pile state", __FILE__, __LINE__);
+        }
     }
     int32_t iters = 0;
     float kms = 0;
@@ -757,6 +948,8 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
         // ---- 6. resume GA* from the retained queues plus the repair pushes
         if (d->s == d->t) {
             d->F = (uint32_t)(tword >> 32);
+        } else if (d->piles) {
+            if ((rc = pile_run(d, &iters, &kms))) return rc;
         } else if ((rc = ddp_ga_run(ctx, F, d->hv, d->P, d->S, d->t, (uint32_t)(tword >> 32), d->rr, &d->F, &d->rr,
                                     &iters, &kms))) {
             return rc;
@@ -769,9 +962,31 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     S.expansions = (int64_t)cnt[0];
     S.edges = (int64_t)cnt[1];
     S.improvements = (int64_t)cnt[2];
+    // list walks grow with the log: fold it into the rows once it is long
+    if (d->compact_at > 0 && d->log_used >= d->compact_at) {
+        const auto tc = clk::now();
+        if ((rc = dastar_compact(d))) return rc;
+        S.compacted = 1;
+        S.compact_ms = ms_since(tc);
+    }
     S.total_ms = ms_since(t0);
     *cost = (int32_t)d->F;
     if (st) *st = S;
+    return DDP_OK;
+}
+
+extern "C" int ddp_dastar_compact(ddp_dastar *d)
+{
+    if (!d) return DDP_ERR_ARG;
+    DDP_HIP(d->ctx, hipSetDevice(d->ctx->device));
+    return dastar_compact(d);
+}
+
+extern "C" int ddp_dastar_set_compaction(ddp_dastar *d, int64_t log_entries)
+{
+    if (!d) return DDP_ERR_ARG;
+    DDP_CHECK(log_entries >= 0, d->ctx, DDP_ERR_ARG, "compaction threshold must be >= 0");
+    d->compact_at = log_entries;
     return DDP_OK;
 }
 

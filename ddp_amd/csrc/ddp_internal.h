@@ -120,6 +120,7 @@ struct HeurView {
     const int32_t *__restrict__ xy;  // DDP_H_EUCLID
     int32_t width;                   // DDP_H_MANHATTAN
     int32_t tx, ty;                  // target coordinates (closed forms)
+    int32_t sx, sy;                  // source coordinates (bidirectional search)
 };
 
 int ddp_make_heur(const ddp_graph *g, ddp_heur_t h, int32_t t, HeurView *out);

@@ -11,7 +11,7 @@
 enum : int {
     GA_SLOT_WORDS = 8,
     GA_S_PUSHCNT = 0, GA_S_TOPMIN = 1, GA_S_PUSHMIN = 2, GA_S_TMIN = 3, GA_S_ERR = 4,
-    GA_RES_IT = 32, GA_RES_RR = 33, GA_RES_F = 34, GA_RES_ERR = 35,
+    GA_RES_IT = 32, GA_RES_RR = 33, GA_RES_F = 34, GA_RES_ERR = 35, GA_RES_PLEN = 36,
     GA_BAR = 64,
     GA_CTRL_WORDS = 96,
 };
@@ -20,15 +20,16 @@ struct GaGridPlan {
     int32_t K, B, nblocks, lcap, cap, pcap, maxit;
     bool wg = false;     // one workgroup (ddp_astar only): lane heaps, or piles when `piles`
     bool piles = false;
+    bool bidi = false;   // piles from both ends: two (f, parent) arrays, touched logs and pile sets
     uint32_t ncap = 0, fcap = 0, delta0 = 1;  // pile mode: near ring, far buffers, first window
 };
 
 // Device buffers of one GA* search state.
 struct GaGridState {
-    uint64_t *gp = nullptr;      // N packed (f << 32 | parent)
-    uint64_t *home = nullptr;    // cap * K heap entries
+    uint64_t *gp = nullptr;      // N packed (f << 32 | parent); bidirectional: 2N, forward then backward
+    uint64_t *home = nullptr;    // cap * K heap entries (bidirectional: twice)
     int32_t *hsize = nullptr;    // K
-    int32_t *touched = nullptr;  // N, workgroup mode only
+    int32_t *touched = nullptr;  // N, workgroup mode only (bidirectional: 2N)
     uint64_t *pbuf = nullptr;    // 2 * pcap push entries
     uint32_t *ctrl = nullptr;    // GA_CTRL_WORDS
     unsigned long long *stats = nullptr;  // 8 counters after ctrl
@@ -53,9 +54,13 @@ int ddp_ga_prepare_resume(ddp_ctx *ctx, const GaGridState &S);
 int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
                const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
                int32_t *rr_out, int32_t *iters_out, float *kernel_ms);
-// Workgroup mode: one search from s to completion (no seed needed).
+// Workgroup mode: one search from s to completion (no seed needed).  The
+// bidirectional plan also needs the transposed rows (gr) and walks the path on
+// the device into dpath[0..dpath_cap) (length to *plen: 0 if unreachable,
+// -len if dpath_cap is too small) when dpath is given.
 int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
                   const GaGridState &S, int32_t N, int32_t s, int32_t t, uint32_t *F_out, int32_t *iters_out,
-                  float *kernel_ms);
+                  float *kernel_ms, const GraphView *gr = nullptr, int32_t *dpath = nullptr, int32_t dpath_cap = 0,
+                  int32_t *plen = nullptr);
 // Reads the four search counters (expanded, edges, improved, stale).
 int ddp_ga_stats(ddp_ctx *ctx, const GaGridState &S, unsigned long long out[4]);

@@ -26,12 +26,14 @@
 #include <chrono>
 #include <cstring>
 
+#include <type_traits>
+
+#include "bidi_search.h"
 #include "gastar.h"
-#include "slot_search.h"
 
 namespace {
 
-enum : uint32_t { ERR_HEAP = 1, ERR_PBUF = 2, ERR_ITERS = 4, ERR_BARRIER = 8 };
+enum : uint32_t { ERR_HEAP = 1, ERR_PBUF = 2, ERR_ITERS = 4, ERR_BARRIER = 8, ERR_PATH = 16 };
 
 struct GridArgs {
     GraphView g;
@@ -276,10 +278,13 @@ KernelFn pick(int B, int hk)
 // path) are bound by that per-iteration latency, not by bandwidth.
 struct WgArgs {
     GraphView g;
+    GraphView gr;        // transposed rows (bidirectional mode)
     HeurView h;
-    uint64_t *gp;
-    uint64_t *home;      // heap spill (entry i of queue q at home[i*B + q]), or the piles
-    int32_t *touched;    // N: nodes whose word left INF
+    uint64_t *gp;        // N (bidirectional: 2N, forward then backward)
+    uint64_t *home;      // heap spill (entry i of queue q at home[i*B + q]), or the piles (two sets: bidirectional)
+    int32_t *touched;    // N: nodes whose word left INF (bidirectional: 2N)
+    int32_t *path;       // bidirectional mode: the path s..t is walked on the device into path[0..path_cap)
+    int32_t path_cap;
     uint32_t *ctrl;
     unsigned long long *stats;
     int32_t N, s, t, lcap, cap, maxit;
@@ -287,58 +292,94 @@ struct WgArgs {
     uint32_t lring;               // pile mode: entries of the LDS near ring (dynamic LDS)
 };
 
-// PILES = false: lane heaps (slot_search); true: near/far piles (slot_search_piles)
-template <int B, int HK, bool PILES>
+enum { WM_HEAPS = 0, WM_PILES = 1, WM_BIDI = 2 };
+
+// WM_HEAPS: lane heaps (slot_search); WM_PILES: near/far piles
+// (slot_search_piles); WM_BIDI: piles from both ends (slot_search_bidi)
+template <int B, int HK, int WM>
 __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
 {
-    extern __shared__ uint64_t s_heap[];  // lcap * B (heaps)
-    constexpr int SV = (PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
-    __shared__ SlotLds<B, SV> S;
+    extern __shared__ uint64_t s_heap[];  // lcap * B (heaps), near rings (piles)
+    constexpr int SV = (WM == WM_PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
+    __shared__ std::conditional_t<WM == WM_BIDI, BidiLds<B>, SlotLds<B, SV>> S;
     const int lane = threadIdx.x;
     const LaneHeap H{s_heap + lane, a.home + lane, B, a.lcap, (int64_t)B, a.cap};
     ExpandCounters C;
     uint32_t stale = 0;
-    if constexpr (PILES) {
-        const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
-        slot_search_piles<B, HK>(a.g, a.h, a.gp, a.touched, P, a.N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
+    if constexpr (WM == WM_BIDI) {
+        const int32_t N = a.N;
+        uint64_t *bh = a.home + (int64_t)a.cap * B;
+        const PilePtrs PF{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring / 2};
+        const PilePtrs PB{bh, bh + a.ncap, bh + a.ncap + a.fcap, a.ncap, a.fcap, s_heap + a.lring / 2, a.lring / 2};
+        const BidiSide D{{a.g.off, a.gr.off}, {a.g.col, a.gr.col}, {a.g.w, a.gr.w}, {a.gp, a.gp + N},
+                         {a.touched, a.touched + N}, {&PF, &PB}};
+        slot_search_bidi<B, HK>(D, a.h, a.g.relax_min != 0, N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
+        if (a.path) slot_path_bidi(a.gp, a.gp + N, N, a.s, a.t, a.path, a.path_cap, S);
+        if (lane == 0) {
+            const uint32_t e = S.err;
+            a.ctrl[GA_RES_IT] = (uint32_t)S.iters;
+            a.ctrl[GA_RES_F] = (uint32_t)(S.meet >> 32);
+            a.ctrl[GA_RES_ERR] = ((e & QERR_HEAP) ? ERR_HEAP : 0u) | ((e & QERR_ITERS) ? ERR_ITERS : 0u) |
+                                 ((e & QERR_PATH) ? ERR_PATH : 0u);
+            a.ctrl[GA_RES_PLEN] = (uint32_t)(a.path ? S.plen : 0);
+        }
     } else {
-        slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
-    }
-    if (lane == 0) {
-        const uint32_t e = S.err;
-        a.ctrl[GA_RES_IT] = (uint32_t)S.iters;
-        a.ctrl[GA_RES_F] = S.F;
-        a.ctrl[GA_RES_ERR] = ((e & QERR_HEAP) ? ERR_HEAP : 0u) | ((e & QERR_ITERS) ? ERR_ITERS : 0u);
+        if constexpr (WM == WM_PILES) {
+            const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
+            slot_search_piles<B, HK>(a.g, a.h, a.gp, a.touched, P, a.N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
+        } else {
+            slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
+        }
+        if (lane == 0) {
+            const uint32_t e = S.err;
+            a.ctrl[GA_RES_IT] = (uint32_t)S.iters;
+            a.ctrl[GA_RES_F] = S.F;
+            a.ctrl[GA_RES_ERR] = ((e & QERR_HEAP) ? ERR_HEAP : 0u) | ((e & QERR_ITERS) ? ERR_ITERS : 0u);
+        }
     }
     slot_flush_stats<B>(a.stats, C, stale);
 }
 
 using WgFn = void (*)(WgArgs);
 
-template <int B, bool PILES>
+template <int B, int WM>
 WgFn pick_wg_b(int hk)
 {
     switch (hk) {
-    case DDP_H_ZERO: return ga_wg_kernel<B, DDP_H_ZERO, PILES>;
-    case DDP_H_ARRAY: return ga_wg_kernel<B, DDP_H_ARRAY, PILES>;
-    case DDP_H_MANHATTAN: return ga_wg_kernel<B, DDP_H_MANHATTAN, PILES>;
-    default: return ga_wg_kernel<B, DDP_H_EUCLID, PILES>;
+    case DDP_H_ZERO: return ga_wg_kernel<B, DDP_H_ZERO, WM>;
+    case DDP_H_ARRAY:
+        if constexpr (WM == WM_BIDI) return nullptr;  // the potential needs h toward s as well as t
+        else return ga_wg_kernel<B, DDP_H_ARRAY, WM>;
+    case DDP_H_MANHATTAN: return ga_wg_kernel<B, DDP_H_MANHATTAN, WM>;
+    default: return ga_wg_kernel<B, DDP_H_EUCLID, WM>;
     }
 }
 
-template <bool PILES>
+template <int WM>
 WgFn pick_wg_p(int B, int hk)
 {
+    if constexpr (WM == WM_BIDI) {
+        switch (B) {
+        case 256: return pick_wg_b<256, WM>(hk);
+        case 512: return pick_wg_b<512, WM>(hk);
+        case 1024: return pick_wg_b<1024, WM>(hk);
+        default: return nullptr;
+        }
+    }
     switch (B) {
-    case 64: return pick_wg_b<64, PILES>(hk);
-    case 128: return pick_wg_b<128, PILES>(hk);
-    case 256: return pick_wg_b<256, PILES>(hk);
-    case 512: return pick_wg_b<512, PILES>(hk);
-    default: return pick_wg_b<1024, PILES>(hk);
+    case 64: return pick_wg_b<64, WM>(hk);
+    case 128: return pick_wg_b<128, WM>(hk);
+    case 256: return pick_wg_b<256, WM>(hk);
+    case 512: return pick_wg_b<512, WM>(hk);
+    default: return pick_wg_b<1024, WM>(hk);
     }
 }
 
-WgFn pick_wg(int B, int hk, bool piles) { return piles ? pick_wg_p<true>(B, hk) : pick_wg_p<false>(B, hk); }
+WgFn pick_wg(int B, int hk, int wm)
+{
+    return wm == WM_BIDI ? pick_wg_p<WM_BIDI>(B, hk) : wm == WM_PILES ? pick_wg_p<WM_PILES>(B, hk)
+                                                                       : pick_wg_p<WM_HEAPS>(B, hk);
+}
 
 }  // namespace
 
@@ -349,12 +390,12 @@ int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaG
     int32_t K = (o && o->K) ? o->K : 1024;
     DDP_CHECK(K >= 1, ctx, DDP_ERR_ARG, "K must be >= 1 (got %d)", K);
     const int32_t mode = (o && allow_wg) ? o->mode : DDP_ASTAR_GRID;
-    DDP_CHECK(mode >= DDP_ASTAR_AUTO && mode <= DDP_ASTAR_PILES, ctx, DDP_ERR_ARG, "unknown mode %d", mode);
-    DDP_CHECK(!((mode == DDP_ASTAR_WORKGROUP || mode == DDP_ASTAR_PILES) && K > 1024), ctx, DDP_ERR_ARG,
-              "one-workgroup modes hold at most 1024 lanes (K=%d)", K);
-    if (mode == DDP_ASTAR_WORKGROUP || mode == DDP_ASTAR_PILES) {
-        const bool piles = mode == DDP_ASTAR_PILES;
-        int32_t WB = 64;
+    DDP_CHECK(mode >= DDP_ASTAR_AUTO && mode <= DDP_ASTAR_BIDI, ctx, DDP_ERR_ARG, "unknown mode %d", mode);
+    const bool one_wg = mode == DDP_ASTAR_WORKGROUP || mode == DDP_ASTAR_PILES || mode == DDP_ASTAR_BIDI;
+    DDP_CHECK(!(one_wg && K > 1024), ctx, DDP_ERR_ARG, "one-workgroup modes hold at most 1024 lanes (K=%d)", K);
+    if (one_wg) {
+        const bool piles = mode == DDP_ASTAR_PILES || mode == DDP_ASTAR_BIDI;
+        int32_t WB = mode == DDP_ASTAR_BIDI ? 256 : 64;  // two directions: half a block each
         while (WB < K) WB *= 2;
         DDP_CHECK(B == 0 || B == WB, ctx, DDP_ERR_ARG,
                   "one-workgroup mode runs K=%d on a block of %d lanes (got block %d)", K, WB, B);
@@ -375,6 +416,7 @@ int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaG
             uint64_t nc = 1;
             while (nc * 2 <= total / 4) nc *= 2;
             p->piles = true;
+            p->bidi = mode == DDP_ASTAR_BIDI;
             p->ncap = (uint32_t)nc;
             p->fcap = (uint32_t)std::min<uint64_t>((total - nc) / 2, 0xFFFFFFFFull);
             p->delta0 = std::max<uint32_t>(1, g->meanw);
@@ -414,9 +456,10 @@ int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridSt
 {
     void *p;
     int rc;
-    if ((rc = ddp_ws(ctx, WS_ASTAR_GP, (size_t)g->N * 8, &p))) return rc;
+    const size_t dirs = P.bidi ? 2 : 1;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_GP, dirs * g->N * 8, &p))) return rc;
     S->gp = (uint64_t *)p;
-    if ((rc = ddp_ws(ctx, WS_ASTAR_HEAP, (size_t)P.cap * P.K * 8, &p))) return rc;
+    if ((rc = ddp_ws(ctx, WS_ASTAR_HEAP, dirs * P.cap * P.K * 8, &p))) return rc;
     S->home = (uint64_t *)p;
     if ((rc = ddp_ws(ctx, WS_ASTAR_HSIZE, (size_t)P.K * 4, &p))) return rc;
     S->hsize = (int32_t *)p;
@@ -427,7 +470,7 @@ int ddp_ga_alloc(ddp_ctx *ctx, const ddp_graph *g, const GaGridPlan &P, GaGridSt
     S->stats = (unsigned long long *)((char *)p + GA_CTRL_WORDS * 4);
     S->touched = nullptr;
     if (P.wg) {
-        if ((rc = ddp_ws(ctx, WS_ASTAR_TOUCH, (size_t)g->N * 4, &p))) return rc;
+        if ((rc = ddp_ws(ctx, WS_ASTAR_TOUCH, dirs * g->N * 4, &p))) return rc;
         S->touched = (int32_t *)p;
     }
     S->owned = false;
@@ -466,7 +509,7 @@ void ddp_ga_free(GaGridState *S)
 
 int ddp_ga_reset(ddp_ctx *ctx, int32_t N, const GaGridPlan &P, const GaGridState &S)
 {
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.gp, (int64_t)N,
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.gp, (int64_t)N * (P.bidi ? 2 : 1),
                        (uint64_t)DDP_GP_INF);
     hipLaunchKernelGGL(fill_i32_kernel, dim3(std::max(1, std::min(1024, P.K / 256 + 1))), dim3(256), 0,
                        ctx->stream, S.hsize, (int64_t)P.K, 0);
@@ -566,11 +609,15 @@ int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGr
 
 int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
                   const GaGridState &S, int32_t N, int32_t s, int32_t t, uint32_t *F_out, int32_t *iters_out,
-                  float *kernel_ms)
+                  float *kernel_ms, const GraphView *gr, int32_t *dpath, int32_t dpath_cap, int32_t *plen)
 {
     DDP_CHECK(P.wg && S.touched && P.B == P.K && P.nblocks == 1, ctx, DDP_ERR_STATE, "GA*: not a workgroup plan");
+    DDP_CHECK(!P.bidi || gr, ctx, DDP_ERR_STATE, "GA*: the bidirectional plan needs transposed rows");
     WgArgs a{};
     a.g = gv;
+    if (gr) a.gr = *gr;
+    a.path = dpath;
+    a.path_cap = dpath_cap;
     a.h = hv;
     a.gp = S.gp;
     a.home = S.home;  // cap * K entries
@@ -588,24 +635,27 @@ int ddp_ga_run_wg(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const G
     a.delta0 = P.delta0;
     a.lring = P.piles ? 8u * (uint32_t)P.B : 0u;  // LDS near ring
     const size_t lds = P.piles ? (size_t)a.lring * 8 : (size_t)P.lcap * P.B * 8;
-    WgFn fn = pick_wg(P.B, hv.kind, P.piles);
+    WgFn fn = pick_wg(P.B, hv.kind, P.bidi ? WM_BIDI : P.piles ? WM_PILES : WM_HEAPS);
+    DDP_CHECK(fn, ctx, DDP_ERR_ARG, "no workgroup kernel for block %d with this heuristic and mode", P.B);
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     hipLaunchKernelGGL(fn, dim3(1), dim3(P.B), lds, ctx->stream, a);
     DDP_HIP(ctx, hipGetLastError());
     DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    uint32_t res[4];
-    DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + GA_RES_IT, 16, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t res[5];
+    DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + GA_RES_IT, 20, hipMemcpyDeviceToHost, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     if (kernel_ms) *kernel_ms = ms;
     if (iters_out) *iters_out = (int32_t)res[0];
     if (F_out) *F_out = res[2];
+    if (plen) *plen = (int32_t)res[4];
     const uint32_t err = res[3];
     DDP_CHECK(!(err & ERR_ITERS), ctx, DDP_ERR_TIMEOUT, "GA*: iteration limit %d reached", P.maxit);
     DDP_CHECK(!(err & ERR_HEAP), ctx, DDP_ERR_OVERFLOW,
               "GA*: a queue exceeded heap_cap=%d entries; raise heap_cap", P.cap);
+    DDP_CHECK(!(err & ERR_PATH), ctx, DDP_ERR_STATE, "GA*: the parent chains through the meeting node are broken");
     return DDP_OK;
 }
 
@@ -627,8 +677,14 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
     if (rc) return rc;
     GaGridPlan P;
     if ((rc = ddp_ga_plan(ctx, g, opts, &P, 0, true))) return rc;
+    if (P.bidi) {
+        DDP_CHECK(g->roff, ctx, DDP_ERR_STATE, "DDP_ASTAR_BIDI needs a graph uploaded with DDP_GRAPH_REVERSE");
+        DDP_CHECK(hv.kind != DDP_H_ARRAY, ctx, DDP_ERR_ARG,
+                  "DDP_ASTAR_BIDI needs a closed-form heuristic (DDP_H_ZERO, _MANHATTAN or _EUCLID)");
+    }
     GaGridState S;
     if ((rc = ddp_ga_alloc(ctx, g, P, &S))) return rc;
+    std::vector<int32_t> bpath;  // bidirectional mode: the path walked on the device (device ids)
     int32_t hs = 0;
     if ((rc = ddp_heur_at(g, hv, s, &hs))) return rc;
     if (stats) {
@@ -654,7 +710,30 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
         uint32_t F = 0;
         int32_t iters = 0;
         float kms = 0;
-        if (P.wg) {
+        if (P.wg && P.bidi) {
+            GraphView gv = ddp_view(g);
+            gv.relax_min = g->minw >= 1;
+            GraphView gr = gv;
+            gr.off = g->roff;
+            gr.col = g->rcol;
+            gr.w = g->rw;
+            // the path is walked on the device; a parent array needs all of it
+            const int32_t dcap = parent_out ? g->N : (path_out ? std::max(path_cap, 1) : 0);
+            int32_t *dpath = nullptr, plen = 0;
+            void *p;
+            if (dcap > 0) {
+                if ((rc = ddp_ws(ctx, WS_PATH, ((size_t)dcap + 64) * 4, &p))) return rc;
+                dpath = (int32_t *)p;
+            }
+            rc = ddp_ga_run_wg(ctx, gv, hv, P, S, g->N, s, t, &F, &iters, &kms, &gr, dpath, dcap, &plen);
+            if (!rc && plen != 0) {
+                bpath.resize(plen > 0 ? plen : 0);
+                if (plen > 0)
+                    DDP_HIP(ctx, hipMemcpy(bpath.data(), dpath, (size_t)plen * 4, hipMemcpyDeviceToHost));
+                else
+                    bpath.assign(1, plen);  // too long for dcap: only the (negative) length is known
+            }
+        } else if (P.wg) {
             GraphView gv = ddp_view(g);
             gv.relax_min = g->minw >= 1;
             rc = ddp_ga_run_wg(ctx, gv, hv, P, S, g->N, s, t, &F, &iters, &kms);
@@ -674,7 +753,18 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
             stats->iterations = iters;
             stats->kernel_ms = kms;
         }
-        if (path_out) {
+        if (path_out && P.bidi) {
+            if (F == DDP_INF_F || bpath.empty()) {
+                *path_len = 0;
+            } else if (bpath.size() == 1 && bpath[0] < 0) {
+                *path_len = bpath[0];
+            } else if ((int64_t)bpath.size() > path_cap) {
+                *path_len = -(int32_t)bpath.size();
+            } else {
+                std::memcpy(path_out, bpath.data(), bpath.size() * 4);
+                *path_len = (int32_t)bpath.size();
+            }
+        } else if (path_out) {
             if (F == DDP_INF_F) *path_len = 0;
             else if ((rc = ddp_extract_path(ctx, S.gp, g->N, s, t, path_out, path_cap, path_len))) return rc;
         } else if (path_len) {
@@ -693,6 +783,11 @@ extern "C" int ddp_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t
         hipLaunchKernelGGL(unpack_parent_kernel, dim3(1024), dim3(256), 0, ctx->stream, S.gp, dpar, g->N);
         DDP_HIP(ctx, hipMemcpyAsync(parent_out, dpar, (size_t)g->N * 4, hipMemcpyDeviceToHost, ctx->stream));
         DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // bidirectional mode: the forward search's parents, and along the path
+        // from the meeting node to t the path's own predecessors, so that the
+        // parent chain from t leads back to s
+        if (P.bidi && !(bpath.size() == 1 && bpath[0] < 0))
+            for (size_t i = 1; i < bpath.size(); ++i) parent_out[bpath[i]] = bpath[i - 1];
         if (!g->iperm.empty()) {  // device ids and order -> caller ids and order
             std::vector<int32_t> tmp(parent_out, parent_out + g->N);
             for (int32_t i = 0; i < g->N; ++i) parent_out[g->iperm[i]] = ddp_to_user(g, tmp[i]);

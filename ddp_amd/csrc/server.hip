@@ -26,10 +26,12 @@
 #include <chrono>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ddp_server.h"
-#include "slot_search.h"
+#include "bidi_search.h"
+#include "compact.h"
 
 namespace {
 
@@ -39,6 +41,7 @@ struct Req {       // pinned, written by the host
     int32_t version;
     int32_t del_off, del_n;  // updates: (u,v) pairs in the arena
     int32_t ins_off, ins_n;  // updates: (u,v,w,entry) quads, sorted by u
+    int32_t rins_off;        // bidirectional engine: (v,entry) pairs, sorted by v (ins_n of them)
     int32_t relax_min;       // queries: every weight at this version is >= 1
 };
 
@@ -58,6 +61,7 @@ enum : int { CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_ERR = 
 
 struct ServeArgs {
     GraphView g;           // base rows + versioned view fields (ver set per query)
+    GraphView gr;          // bidirectional engine: transposed base rows
     HeurView h;
     int32_t *dver;         // E, writable
     uint64_t *lhead;       // N, writable
@@ -66,6 +70,13 @@ struct ServeArgs {
     uint32_t *lw;
     int32_t *ldel;
     int32_t *ltail;        // N, last entry of each list (updaters only)
+    // bidirectional engine: delete version of each transposed entry, its
+    // forward edge id, the source of each log entry and per-target lists
+    int32_t *rdver;
+    const int32_t *reid;
+    int32_t *lu;
+    uint64_t *rlhead, *rlnext;
+    int32_t *rltail;
     int32_t log_cap;
     const Req *ring;       // pinned
     Res *res;              // pinned
@@ -84,14 +95,14 @@ struct ServeArgs {
     uint64_t idle_ticks, wait_ticks;
 };
 
-enum { SENG_HEAPS = 0, SENG_PILES = 1 };
+enum { SENG_HEAPS = 0, SENG_PILES = 1, SENG_BIDI = 2 };
 
 __device__ __forceinline__ int32_t sys_ld_i(const int32_t *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-template <int B>
+template <int B, bool BIDI>
 __device__ void apply_update(const ServeArgs &a, const Req &rq, uint32_t *s_err)
 {
     const int32_t V = rq.version;
@@ -102,7 +113,15 @@ __device__ void apply_update(const ServeArgs &a, const Req &rq, uint32_t *s_err)
         if (u < 0 || u >= a.N || v < 0 || v >= a.N) continue;
         bool done = false;
         for (int32_t j = a.g.off[u]; j < a.g.off[u + 1] && !done; ++j)
-            if (a.g.col[j] == v && atomicCAS(a.dver + j, INT32_MAX, V) == INT32_MAX) done = true;
+            if (a.g.col[j] == v && atomicCAS(a.dver + j, INT32_MAX, V) == INT32_MAX) {
+                done = true;
+                if constexpr (BIDI)  // the same edge in v's transposed row
+                    for (int32_t r = a.gr.off[v]; r < a.gr.off[v + 1]; ++r)
+                        if (a.reid[r] == j) {
+                            a.rdver[r] = V;
+                            break;
+                        }
+            }
         uint64_t link = __hip_atomic_load(a.lhead + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         while (!done && (uint32_t)link < (uint32_t)V) {
             const int32_t x = (int32_t)(link >> 32);
@@ -130,6 +149,10 @@ __device__ void apply_update(const ServeArgs &a, const Req &rq, uint32_t *s_err)
             a.lw[x] = w;
             a.ldel[x] = INT32_MAX;
             a.lnext[x] = ~0ull;
+            if constexpr (BIDI) {
+                a.lu[x] = u;
+                a.rlnext[x] = ~0ull;
+            }
             const uint64_t link = ((uint64_t)(uint32_t)x << 32) | (uint32_t)V;
             const int32_t last = a.ltail[u];
             if (last < 0) a.lhead[u] = link;
@@ -138,6 +161,28 @@ __device__ void apply_update(const ServeArgs &a, const Req &rq, uint32_t *s_err)
         }
     }
     block_sync_global();
+    if constexpr (BIDI) {
+        // the same entries on per-target lists, one lane per target (pairs
+        // sorted by target, entry order kept within a target)
+        for (int32_t i = lane; i < rq.ins_n; i += B) {
+            const int32_t *q = a.arena + rq.rins_off + 2 * i;
+            const int32_t v = sys_ld_i(q);
+            if (i > 0 && sys_ld_i(q - 2) == v) continue;
+            if (v < 0 || v >= a.N) continue;  // the forward pass flagged it
+            for (int32_t k = i; k < rq.ins_n; ++k) {
+                const int32_t *e = a.arena + rq.rins_off + 2 * k;
+                if (sys_ld_i(e) != v) break;
+                const int32_t x = sys_ld_i(e + 1);
+                if (x < 0 || x >= a.log_cap) continue;
+                const uint64_t link = ((uint64_t)(uint32_t)x << 32) | (uint32_t)V;
+                const int32_t last = a.rltail[v];
+                if (last < 0) a.rlhead[v] = link;
+                else a.rlnext[last] = link;
+                a.rltail[v] = x;
+            }
+        }
+        block_sync_global();
+    }
 }
 
 template <int B, int HK, int ENG>
@@ -145,16 +190,17 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
 {
     extern __shared__ uint64_t s_heap[];
     constexpr int SV = (ENG == SENG_PILES) ? B * pile_entries_per_lane<B>() : B;  // expansion width
-    __shared__ SlotLds<B, SV> S;
+    __shared__ std::conditional_t<ENG == SENG_BIDI, BidiLds<B>, SlotLds<B, SV>> S;
     __shared__ uint32_t s_ticket;
     __shared__ int32_t s_have;
     __shared__ uint32_t s_uerr;
     const int lane = threadIdx.x;
     const int64_t slot = blockIdx.x;
     const int32_t N = a.N;
-    uint64_t *gp = a.gp + slot * N;
-    int32_t *touched = a.touched + slot * N;
-    uint64_t *home = a.home + slot * (int64_t)a.cap * B;  // this slot's queue memory
+    constexpr int DIRS = ENG == SENG_BIDI ? 2 : 1;
+    uint64_t *gp = a.gp + slot * DIRS * N;
+    int32_t *touched = a.touched + slot * DIRS * N;
+    uint64_t *home = a.home + slot * DIRS * (int64_t)a.cap * B;  // this slot's queue memory
     const LaneHeap H{s_heap + lane, home + lane, B, a.lcap, (int64_t)B, a.cap};
     const PilePtrs P{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
     ExpandCounters C;
@@ -218,6 +264,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         rq.del_n = sys_ld_i(&rp->del_n);
         rq.ins_off = sys_ld_i(&rp->ins_off);
         rq.ins_n = sys_ld_i(&rp->ins_n);
+        rq.rins_off = sys_ld_i(&rp->rins_off);
         rq.relax_min = sys_ld_i(&rp->relax_min);
         const uint64_t tstart = dev_clock_100mhz();
         // ---- wait for the graph version this request needs (bounded)
@@ -241,7 +288,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         int32_t cost = DDP_INF_COST;
         uint32_t status = s_uerr;
         if (rq.type == DDP_REQ_UPDATE) {
-            if (!status) apply_update<B>(a, rq, &s_uerr);
+            if (!status) apply_update<B, ENG == SENG_BIDI>(a, rq, &s_uerr);
             __syncthreads();
             status = s_uerr;
             if (lane == 0) {
@@ -256,13 +303,29 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             GraphView G = a.g;
             G.ver = rq.version;
             G.relax_min = a.g.relax_min && rq.relax_min;
-            if constexpr (ENG == SENG_PILES)
-                slot_search_piles<B, HK>(G, a.h, gp, touched, P, N, rq.s, rq.t, a.maxit, a.delta0, S, C, stale);
-            else
-                slot_search<B, HK>(G, a.h, gp, touched, H, s_heap, N, rq.s, rq.t, a.maxit, S, C, stale);
-            cost = (int32_t)S.F;
-            status = S.err;
-            slot_reset(gp, touched, S);
+            if constexpr (ENG == SENG_BIDI) {
+                uint64_t *bh = home + (int64_t)a.cap * B;
+                const PilePtrs PF{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring / 2};
+                const PilePtrs PB{bh, bh + a.ncap, bh + a.ncap + a.fcap, a.ncap, a.fcap, s_heap + a.lring / 2,
+                                  a.lring / 2};
+                const BidiSide D{{a.g.off, a.gr.off},  {a.g.col, a.gr.col}, {a.g.w, a.gr.w},     {gp, gp + N},
+                                 {touched, touched + N}, {&PF, &PB},          {a.dver, a.rdver}, rq.version,
+                                 {a.lhead, a.rlhead},  {a.lnext, a.rlnext}, {a.lv, a.lu},       a.lw,
+                                 a.ldel};
+                slot_search_bidi<B, HK, true>(D, a.h, G.relax_min != 0, N, rq.s, rq.t, a.maxit, a.delta0, S, C,
+                                              stale);
+                cost = (int32_t)(uint32_t)(S.meet >> 32);
+                status = S.err;
+                slot_reset_bidi(gp, gp + N, touched, touched + N, S);
+            } else {
+                if constexpr (ENG == SENG_PILES)
+                    slot_search_piles<B, HK>(G, a.h, gp, touched, P, N, rq.s, rq.t, a.maxit, a.delta0, S, C, stale);
+                else
+                    slot_search<B, HK>(G, a.h, gp, touched, H, s_heap, N, rq.s, rq.t, a.maxit, S, C, stale);
+                cost = (int32_t)S.F;
+                status = S.err;
+                slot_reset(gp, touched, S);
+            }
             if (__ballot(C.edges >= (1u << 30))) slot_flush_stats<B>(a.stats, C, stale);  // before 32 bits wrap
         }
         if (lane == 0) {
@@ -298,7 +361,7 @@ ServeFn pick_s(int hk)
 {
     switch (hk) {
     case DDP_H_ZERO: return serve_kernel<B, DDP_H_ZERO, ENG>;
-    case DDP_H_ARRAY: return serve_kernel<B, DDP_H_ARRAY, ENG>;
+    case DDP_H_ARRAY: return nullptr;  // refused at creation: one target per h array
     case DDP_H_MANHATTAN: return serve_kernel<B, DDP_H_MANHATTAN, ENG>;
     default: return serve_kernel<B, DDP_H_EUCLID, ENG>;
     }
@@ -307,6 +370,14 @@ ServeFn pick_s(int hk)
 template <int ENG>
 ServeFn pick_se(int B, int hk)
 {
+    if constexpr (ENG == SENG_BIDI) {  // two directions need half a block each: 256 lanes and up
+        switch (B) {
+        case 256: return pick_s<256, ENG>(hk);
+        case 512: return pick_s<512, ENG>(hk);
+        case 1024: return pick_s<1024, ENG>(hk);
+        default: return nullptr;
+        }
+    }
     switch (B) {
     case 64: return pick_s<64, ENG>(hk);
     case 128: return pick_s<128, ENG>(hk);
@@ -318,6 +389,7 @@ ServeFn pick_se(int B, int hk)
 
 ServeFn pick_serve(int B, int hk, int eng)
 {
+    if (eng == SENG_BIDI) return pick_se<SENG_BIDI>(B, hk);
     return eng == SENG_PILES ? pick_se<SENG_PILES>(B, hk) : pick_se<SENG_HEAPS>(B, hk);
 }
 
@@ -339,8 +411,15 @@ struct ddp_server {
     int32_t idle_ms = 2000;
     // host view of the graph (for nothing but sizes)
     int32_t N = 0, E = 0;
+    // base rows after a compaction (owned; until the first one the caller's graph is read)
+    MergedCsr own{}, own_r{};  // forward, transposed (bidirectional engine; eid = forward edge id)
+    int32_t RE = 0;            // transposed entries
+    bool base_relax_min = false;  // every base weight is >= 1
+    int32_t compactions = 0;
     // device
     int32_t *dver = nullptr, *lv = nullptr, *ldel = nullptr, *ltail = nullptr, *touched = nullptr;
+    int32_t *rdver = nullptr, *lu = nullptr, *rltail = nullptr;  // bidirectional engine
+    uint64_t *rlhead = nullptr, *rlnext = nullptr;
     uint32_t *lw = nullptr, *ctrl = nullptr;
     uint64_t *lhead = nullptr, *lnext = nullptr, *gp = nullptr, *home = nullptr;
     unsigned long long *stats = nullptr;
@@ -377,9 +456,11 @@ static double ms_now(const ddp_server *sv)
 static void server_free(ddp_server *sv)
 {
     void *dev[] = {sv->dver, sv->lv, sv->ldel, sv->ltail, sv->touched, sv->lw, sv->ctrl, sv->lhead,
-                   sv->lnext, sv->gp, sv->home, sv->stats};
+                   sv->lnext, sv->gp, sv->home, sv->stats, sv->rdver, sv->lu, sv->rltail, sv->rlhead, sv->rlnext};
     for (void *p : dev)
         if (p) hipFree(p);
+    ddp_merged_free(&sv->own);
+    ddp_merged_free(&sv->own_r);
     void *host[] = {sv->ring_h, sv->res_h, sv->arena_h, sv->flags_h};
     for (void *p : host)
         if (p) hipHostFree(p);
@@ -391,7 +472,24 @@ static int server_launch(ddp_server *sv)
     ddp_ctx *ctx = sv->ctx;
     ServeArgs a{};
     a.g = ddp_view(sv->g);
-    a.g.relax_min = sv->g->minw >= 1;
+    if (sv->own.off) {  // compacted base rows
+        a.g.off = sv->own.off;
+        a.g.col = sv->own.col;
+        a.g.w = sv->own.w;
+    }
+    a.g.relax_min = sv->base_relax_min;
+    if (sv->eng == SENG_BIDI) {
+        a.gr = a.g;
+        a.gr.off = sv->own_r.off ? sv->own_r.off : sv->g->roff;
+        a.gr.col = sv->own_r.off ? sv->own_r.col : sv->g->rcol;
+        a.gr.w = sv->own_r.off ? sv->own_r.w : sv->g->rw;
+        a.reid = sv->own_r.off ? sv->own_r.eid : sv->g->reid;
+        a.rdver = sv->rdver;
+        a.lu = sv->lu;
+        a.rlhead = sv->rlhead;
+        a.rlnext = sv->rlnext;
+        a.rltail = sv->rltail;
+    }
     a.g.dver = sv->dver;
     a.g.lhead = sv->lhead;
     a.g.lnext = sv->lnext;
@@ -462,6 +560,10 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     if (rc) return rc;
     DDP_CHECK(hv.kind != DDP_H_ARRAY, ctx, DDP_ERR_ARG,
               "serving needs a per-query heuristic: DDP_H_ZERO, DDP_H_MANHATTAN or DDP_H_EUCLID");
+    const int32_t engine = o ? o->engine : DDP_SERVER_AUTO;
+    DDP_CHECK(engine >= DDP_SERVER_AUTO && engine <= DDP_SERVER_BIDI, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
+    DDP_CHECK(engine != DDP_SERVER_BIDI || (g->roff && g->reid), ctx, DDP_ERR_STATE,
+              "the bidirectional engine needs a graph uploaded with DDP_GRAPH_REVERSE");
     ddp_server *sv = new ddp_server();
     sv->ctx = ctx;
     sv->device = ctx->device;
@@ -469,14 +571,16 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     sv->hv = hv;
     sv->N = g->N;
     sv->E = g->E;
-    const int32_t engine = o ? o->engine : DDP_SERVER_AUTO;
-    DDP_CHECK(engine >= DDP_SERVER_AUTO && engine <= DDP_SERVER_PILES, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
-    sv->eng = engine == DDP_SERVER_HEAPS ? SENG_HEAPS : SENG_PILES;
+    sv->base_relax_min = g->minw >= 1;
+    sv->eng = engine == DDP_SERVER_HEAPS ? SENG_HEAPS : engine == DDP_SERVER_BIDI ? SENG_BIDI : SENG_PILES;
+    const bool piles = sv->eng != SENG_HEAPS;
+    const int DIRS = sv->eng == SENG_BIDI ? 2 : 1;
+    sv->RE = g->RE;
     // 1024-lane pile slots served the C4 mix with the best capacity and tail latency
-    sv->B = (o && o->block) ? o->block : (sv->eng == SENG_PILES ? 1024 : 256);
-    sv->lcap = sv->eng == SENG_PILES ? 0 : std::max(4, std::min(32, 4096 / std::max(64, sv->B)));
+    sv->B = (o && o->block) ? o->block : (piles ? 1024 : 256);
+    sv->lcap = piles ? 0 : std::max(4, std::min(32, 4096 / std::max(64, sv->B)));
     sv->cap = (o && o->heap_cap) ? o->heap_cap : 2048;
-    if (sv->eng == SENG_PILES && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot
+    if (piles && !(o && o->heap_cap))  // room for an exhaustive search: 3N entries per slot and direction
         sv->cap = (int32_t)std::min<int64_t>(std::max<int64_t>(sv->cap, (3LL * g->N + sv->B - 1) / sv->B),
                                              INT32_MAX / 2);
     sv->cap = std::max(sv->cap, 16);
@@ -502,9 +606,10 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
         (sv->ring & (sv->ring - 1)) || sv->ring < 64)
         return fail(ddp_set_err(ctx, DDP_ERR_ARG, "block must be 64..1024 and ring a power of two >= 64"));
     const void *fn = (const void *)pick_serve(sv->B, hv.kind, sv->eng);
+    if (!fn) return fail(ddp_set_err(ctx, DDP_ERR_ARG, "no serving kernel for block %d and this engine", sv->B));
     int per_cu = 0;
-    sv->lring = sv->eng == SENG_PILES ? 8u * (uint32_t)sv->B : 0u;  // LDS near ring, as for batches
-    sv->lds = sv->eng == SENG_PILES ? (size_t)sv->lring * 8 : (size_t)sv->lcap * sv->B * 8;
+    sv->lring = piles ? 8u * (uint32_t)sv->B : 0u;  // LDS near ring, as for batches (half per direction: bidi)
+    sv->lds = piles ? (size_t)sv->lring * 8 : (size_t)sv->lcap * sv->B * 8;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sv->lds);
     if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, sv->B, sv->lds);
     if (e != hipSuccess || per_cu < 1) return fail(ddp_set_err(ctx, DDP_ERR_HIP, "serve kernel occupancy query failed"));
@@ -513,8 +618,9 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     // admission by free HBM: graph-version state first, then as many slots as fit (10% headroom)
     size_t fr = 0, tot = 0;
     e = hipMemGetInfo(&fr, &tot);
-    const size_t fixed = (size_t)g->E * 4 + (size_t)g->N * 12 + (size_t)sv->log_cap * 24 + (1 << 20);
-    const size_t per_slot = (size_t)g->N * 12 + (size_t)sv->cap * sv->B * 8;
+    const size_t fixed = (size_t)g->E * 4 + (size_t)g->N * 12 + (size_t)sv->log_cap * 24 + (1 << 20) +
+                         (DIRS == 2 ? (size_t)g->RE * 4 + (size_t)g->N * 12 + (size_t)sv->log_cap * 12 : 0);
+    const size_t per_slot = DIRS * ((size_t)g->N * 12 + (size_t)sv->cap * sv->B * 8);
     const size_t budget = fr > fixed ? (size_t)((fr - fixed) * 0.9) : 0;
     slots = std::min<int64_t>(slots, (int64_t)(budget / per_slot));
     if (e != hipSuccess || slots < 1)
@@ -532,9 +638,16 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     al((void **)&sv->ldel, (size_t)sv->log_cap * 4);
     al((void **)&sv->ctrl, CT_WORDS * 4);
     al((void **)&sv->stats, 64);
-    al((void **)&sv->gp, (size_t)slots * g->N * 8);
-    al((void **)&sv->touched, (size_t)slots * g->N * 4);
-    al((void **)&sv->home, (size_t)slots * sv->cap * sv->B * 8);
+    al((void **)&sv->gp, (size_t)slots * DIRS * g->N * 8);
+    al((void **)&sv->touched, (size_t)slots * DIRS * g->N * 4);
+    al((void **)&sv->home, (size_t)slots * DIRS * sv->cap * sv->B * 8);
+    if (DIRS == 2) {
+        al((void **)&sv->rdver, (size_t)g->RE * 4);
+        al((void **)&sv->lu, (size_t)sv->log_cap * 4);
+        al((void **)&sv->rlhead, (size_t)g->N * 8);
+        al((void **)&sv->rlnext, (size_t)sv->log_cap * 8);
+        al((void **)&sv->rltail, (size_t)g->N * 4);
+    }
     sv->arena_cap = (size_t)1 << 22;
     auto hal = [&](void **hp, void **dp, size_t bytes) {
         if (e == hipSuccess) e = hipHostMalloc(hp, bytes, hipHostMallocCoherent | hipHostMallocMapped);
@@ -554,8 +667,13 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->dver, (int64_t)g->E, INT32_MAX);
     hipLaunchKernelGGL(serve_fill_u64, dim3(1024), dim3(256), 0, ctx->stream, sv->lhead, (int64_t)g->N, ~0ull);
     hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->ltail, (int64_t)g->N, -1);
-    hipLaunchKernelGGL(serve_fill_u64, dim3(4096), dim3(256), 0, ctx->stream, sv->gp, (int64_t)slots * g->N,
+    hipLaunchKernelGGL(serve_fill_u64, dim3(4096), dim3(256), 0, ctx->stream, sv->gp, (int64_t)slots * DIRS * g->N,
                        (uint64_t)DDP_GP_INF);
+    if (DIRS == 2) {
+        hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->rdver, (int64_t)g->RE, INT32_MAX);
+        hipLaunchKernelGGL(serve_fill_u64, dim3(1024), dim3(256), 0, ctx->stream, sv->rlhead, (int64_t)g->N, ~0ull);
+        hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->rltail, (int64_t)g->N, -1);
+    }
     if (e == hipSuccess) e = hipMemsetAsync(sv->ctrl, 0, CT_WORDS * 4, ctx->stream);
     if (e == hipSuccess) e = hipMemsetAsync(sv->stats, 0, 64, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -603,6 +721,86 @@ extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uin
     return server_ensure(sv);
 }
 
+// Folds the inserted-edge log into the base rows at the current version
+// (merge_graph / mergeDiff, DA_star/DA_dyn_insert_del_nocycle.cu:940-980,
+// 2236-2284): every submitted request is finished first and the kernel is
+// stopped, since queries read the log and the delete versions; the merged rows
+// of the current version V become the base rows (their delete versions INF)
+// and the log starts empty.  A query at version >= V reads the same graph
+// through the new rows, and later updates number their versions on from V.
+static int server_compact(ddp_server *sv)
+{
+    ddp_ctx *ctx = sv->ctx;
+    int rc = ddp_server_drain(sv, 600000);
+    if (rc) return rc;
+    if (sv->running) {  // idle slots exit on the stop flag
+        __atomic_store_n(&sv->flags_h[1], 1u, __ATOMIC_RELEASE);
+        DDP_HIP(ctx, hipEventSynchronize(sv->done_ev));
+        sv->running = false;
+    }
+    __atomic_store_n(&sv->flags_h[1], 0u, __ATOMIC_RELEASE);
+    const uint32_t zero = 0;
+    DDP_HIP(ctx, hipMemcpy(sv->ctrl + CT_STOP, &zero, 4, hipMemcpyHostToDevice));
+    const GraphView base = [&] {
+        GraphView v = ddp_view(sv->g);
+        if (sv->own.off) {
+            v.off = sv->own.off;
+            v.col = sv->own.col;
+            v.w = sv->own.w;
+        }
+        return v;
+    }();
+    const MergeSrc m{base.off, base.col, base.w, sv->N, sv->dver, sv->version, sv->lhead, sv->lnext,
+                     sv->lv, sv->lw, sv->ldel};
+    MergedCsr f, fr;
+    if ((rc = ddp_merge_rows(ctx, m, &f))) return rc;
+    if (sv->eng == SENG_BIDI && (rc = ddp_transpose_rows(ctx, f, true, &fr))) {
+        ddp_merged_free(&f);
+        return rc;
+    }
+    int32_t *dver = nullptr, *rdver = nullptr;
+    hipError_t e = hipMalloc((void **)&dver, (size_t)std::max(f.E, 1) * 4);
+    if (e == hipSuccess && sv->eng == SENG_BIDI) e = hipMalloc((void **)&rdver, (size_t)std::max(fr.E, 1) * 4);
+    if (e != hipSuccess) {
+        if (dver) hipFree(dver);
+        ddp_merged_free(&f);
+        ddp_merged_free(&fr);
+        return ddp_hip_err(ctx, e, "compaction: delete versions", __FILE__, __LINE__);
+    }
+    hipFree(sv->dver);
+    sv->dver = dver;
+    ddp_merged_free(&sv->own);
+    sv->own = f;
+    sv->E = f.E;
+    if (sv->eng == SENG_BIDI) {
+        hipFree(sv->rdver);
+        sv->rdver = rdver;
+        ddp_merged_free(&sv->own_r);
+        sv->own_r = fr;
+        sv->RE = fr.E;
+        hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->rdver, (int64_t)fr.E, INT32_MAX);
+        hipLaunchKernelGGL(serve_fill_u64, dim3(1024), dim3(256), 0, ctx->stream, sv->rlhead, (int64_t)sv->N, ~0ull);
+        hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->rltail, (int64_t)sv->N, -1);
+    }
+    // a zero-weight insert at or before this version is now a base edge
+    sv->base_relax_min = sv->base_relax_min && sv->zero_w_version > sv->version;
+    hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->dver, (int64_t)f.E, INT32_MAX);
+    hipLaunchKernelGGL(serve_fill_u64, dim3(1024), dim3(256), 0, ctx->stream, sv->lhead, (int64_t)sv->N, ~0ull);
+    hipLaunchKernelGGL(serve_fill_i32, dim3(1024), dim3(256), 0, ctx->stream, sv->ltail, (int64_t)sv->N, -1);
+    DDP_HIP(ctx, hipGetLastError());
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    sv->log_used = 0;
+    sv->compactions += 1;
+    return DDP_OK;
+}
+
+extern "C" int ddp_server_compact(ddp_server *sv)
+{
+    if (!sv) return DDP_ERR_ARG;
+    DDP_HIP(sv->ctx, hipSetDevice(sv->ctx->device));
+    return server_compact(sv);
+}
+
 extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *lines, int32_t n, uint64_t tag)
 {
     if (!sv || (n && !lines)) return DDP_ERR_ARG;
@@ -623,13 +821,26 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
             if (l.w == 0) sv->zero_w_version = std::min(sv->zero_w_version, sv->version + 1);
         }
     }
-    DDP_CHECK(sv->log_used + (int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_CAPACITY,
-              "inserted-edge log full (%d entries); raise max_inserts", sv->log_cap);
+    DDP_CHECK((int64_t)ins.size() <= sv->log_cap, ctx, DDP_ERR_CAPACITY,
+              "update batch of %zu inserts exceeds the inserted-edge log (%d entries); raise max_inserts",
+              ins.size(), sv->log_cap);
+    if (sv->log_used + (int64_t)ins.size() > sv->log_cap) {
+        // the log is full: fold it into the rows (waits for every request in flight)
+        int rc = server_compact(sv);
+        if (rc) return rc;
+    }
     // entry ids in arrival order, then grouped by source (stable)
     std::vector<std::array<int32_t, 4>> q(ins.size());
     for (size_t k = 0; k < ins.size(); ++k) q[k] = {ins[k][0], ins[k][1], ins[k][2], sv->log_used + (int32_t)k};
     std::stable_sort(q.begin(), q.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
-    const size_t need = dels.size() + 4 * q.size();
+    // bidirectional engine: the same entries grouped by target (stable in entry order)
+    std::vector<std::array<int32_t, 2>> rq;
+    if (sv->eng == SENG_BIDI) {
+        rq.resize(ins.size());
+        for (size_t k = 0; k < ins.size(); ++k) rq[k] = {ins[k][1], sv->log_used + (int32_t)k};
+        std::stable_sort(rq.begin(), rq.end(), [](const auto &x, const auto &y) { return x[0] < y[0]; });
+    }
+    const size_t need = dels.size() + 4 * q.size() + 2 * rq.size();
     DDP_CHECK(need < sv->arena_cap / 2, ctx, DDP_ERR_ARG, "update batch too large for the payload arena");
     // The arena is a ring: a payload is free again once its update has been
     // collected (the device is done with it).  Refuse (caller polls and
@@ -643,6 +854,8 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
             return ddp_set_err(ctx, DDP_ERR_NOMEM, "update payload arena busy; poll and retry");
     std::memcpy(sv->arena_h + pos, dels.data(), dels.size() * 4);
     for (size_t k = 0; k < q.size(); ++k) std::memcpy(sv->arena_h + pos + dels.size() + 4 * k, q[k].data(), 16);
+    const size_t roff = pos + dels.size() + 4 * q.size();
+    for (size_t k = 0; k < rq.size(); ++k) std::memcpy(sv->arena_h + roff + 2 * k, rq[k].data(), 8);
     Req r{};
     r.type = DDP_REQ_UPDATE;
     r.version = sv->version + 1;
@@ -650,6 +863,7 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
     r.del_n = (int32_t)(dels.size() / 2);
     r.ins_off = (int32_t)(pos + dels.size());
     r.ins_n = (int32_t)q.size();
+    r.rins_off = (int32_t)roff;
     sv->arena_head = pos + need;
     sv->arena_marks.emplace_back(sv->submitted, pos, pos + need);
     sv->log_used += (int32_t)q.size();

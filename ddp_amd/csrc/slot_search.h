@@ -29,6 +29,7 @@ struct SlotLds {
     uint32_t phase_n;               // near entries taken since the last refill
     uint32_t ovf;                   // a push found both piles full and was dropped
     uint32_t no_early;              // the far pile holds nothing near the window: no early refill
+    uint32_t dorm_n;                // resumed searches: entries set aside in the dormant pile
 };
 
 template <int B>
@@ -271,18 +272,46 @@ __device__ __forceinline__ bool pile_place(const PilePtrs &P, uint32_t thr, bool
     return dropped;
 }
 
-template <int B, int V>
+// A resumed search (DA*): the open set arrives as the first n_in entries of
+// the far pile, the target's current f is F0, and entries that cannot beat
+// F(t) now are kept in a dormant pile instead of being dropped -- a later
+// batch may raise F(t) again.  (f, parent) words persist between searches,
+// so nothing is logged for a reset.
+struct PileResume {
+    uint64_t *dorm;  // dcap entries
+    uint32_t dcap;
+    uint32_t n_in;
+    uint32_t F0;
+};
+
+// Wave-collective: appends the selected keys to the dormant pile.
+template <int R, class SL>
+__device__ __forceinline__ void dorm_append(const PileResume &Z, SL &S, const bool (&sel)[R], const uint64_t (&key)[R])
+{
+    uint32_t pos[R];
+    wave_append<R>(sel, &S.dorm_n, pos);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (!sel[r]) continue;
+        if (pos[r] < Z.dcap) Z.dorm[pos[r]] = key[r];
+        else atomicOr(&S.err, (uint32_t)QERR_HEAP);
+    }
+}
+
+template <int B, int V, bool RESUME = false>
 struct PilePush {
     const PilePtrs *P;
     uint64_t *far;
     SlotLds<B, V> *S;
     int32_t *touched;
+    const PileResume *Z = nullptr;
     uint32_t err = 0;
 
     __device__ void target(uint32_t f) { atomicMin(&S->F, f); }
     __device__ void touch(int32_t c, uint64_t prev)
     {
-        if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(&S->ntouch, 1)] = c;
+        if constexpr (!RESUME)
+            if ((uint32_t)(prev >> 32) == DDP_INF_F) touched[atomicAdd(&S->ntouch, 1)] = c;
     }
     // near candidates: the LDS ring, then the HBM ring, then the far pile
     __device__ void operator()(bool has, uint64_t key)
@@ -294,6 +323,7 @@ struct PilePush {
     template <int R>
     __device__ void touch_all(const bool (&first)[R], const int32_t (&c)[R])
     {
+        if constexpr (RESUME) return;
         uint32_t pos[R];
         wave_append<R>(first, (uint32_t *)&S->ntouch, pos);
 #pragma unroll
@@ -312,6 +342,12 @@ struct PilePush {
         bool put[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) put[r] = has[r] && (uint32_t)(key[r] >> 32) < F;
+        if constexpr (RESUME) {
+            bool dz[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) dz[r] = has[r] && !put[r];
+            dorm_append<R>(*Z, *S, dz, key);
+        }
         const bool lds_ok = P->lcap && S->near_tail == S->near_head;
         if (pile_place<R>(*P, S->thr, lds_ok, S->lring_head, S->near_head, &S->lring_tail, &S->near_tail, far,
                           &S->far_n, put, key))
@@ -354,16 +390,25 @@ constexpr int pile_entries_per_lane()
 // 256 (three or fewer entries) refilled so often that it was slower.
 constexpr uint32_t kThinRing = 64;
 
-template <int B, int HK, int W = pile_entries_per_lane<B>()>
+//
+// RESUME (a DA* search resumed after a batch): the search starts from the
+// open set of PileResume Z -- its first Z.n_in entries already in P.far0 --
+// with F(t) = Z.F0 instead of from s, and an entry that cannot beat F(t) goes
+// to Z's dormant pile instead of being dropped (see PileResume).  On return
+// the dormant pile holds every open entry left: the search's open set for the
+// next resume.  A far-pile rebuild re-queues every node with a finite word.
+template <int B, int HK, int W = pile_entries_per_lane<B>(), bool RESUME = false>
 __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched,
                                   const PilePtrs &P, int32_t N, int32_t s, int32_t t, int32_t maxit,
-                                  uint32_t delta0, SlotLds<B, B * W> &S, ExpandCounters &C, uint32_t &stale)
+                                  uint32_t delta0, SlotLds<B, B * W> &S, ExpandCounters &C, uint32_t &stale,
+                                  const PileResume *Z = nullptr)
 {
     constexpr uint32_t V = (uint32_t)B * W;
     const int lane = threadIdx.x;
     const uint32_t nmask = P.ncap - 1, lmask = P.lcap ? P.lcap - 1 : 0;
     if (lane == 0) {
         S.F = DDP_INF_F;
+        S.dorm_n = 0;
         S.ntouch = 0;
         S.err = 0;
         S.iters = 0;
@@ -378,7 +423,14 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
     }
     __syncthreads();
     bool run = s >= 0 && s < N && t >= 0 && t < N;
-    if (run) {
+    if constexpr (RESUME) {
+        run = t >= 0 && t < N && Z->n_in > 0;
+        set_target<HK>(hv, t);
+        if (lane == 0) {
+            S.F = Z->F0;
+            S.far_n = Z->n_in;  // the open set; the first refill sets the window
+        }
+    } else if (run) {
         set_target<HK>(hv, t);
         const int32_t hs = heur_eval<HK>(hv, s);
         if (HK == DDP_H_ARRAY && hs < 0) {
@@ -424,8 +476,9 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
         if (early && from_lds) block_sync_global();  // the far pile is read below
         if (avail == 0 || early) {
             if (S.ovf) {
-                // ---- rebuild the far pile from the touched log (entries were dropped)
-                const int32_t nt = S.ntouch;
+                // ---- rebuild the far pile from the touched log (entries were dropped);
+                // a resumed search, which logs nothing, scans every node
+                const int32_t nt = RESUME ? N : S.ntouch;
                 if (lane == 0) S.far_n2 = 0;
                 __syncthreads();
                 for (int32_t i0 = 0; i0 < nt; i0 += B) {  // uniform trip count: the reservations are wave-wide
@@ -433,8 +486,13 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
                     int32_t v = -1;
                     uint32_t f = 0xFFFFFFFFu;
                     if (i < nt) {
-                        v = __hip_atomic_load(touched + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        v = RESUME ? i : __hip_atomic_load(touched + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         f = (uint32_t)(ld_gp(gp + v) >> 32);
+                    }
+                    if constexpr (RESUME) {  // finite but hopeless now: dormant
+                        const bool dz[1] = {f != DDP_INF_F && f >= F && v != t};
+                        const uint64_t dk[1] = {((uint64_t)f << 32) | (uint32_t)v};
+                        dorm_append<1>(*Z, S, dz, dk);
                     }
                     const uint32_t idx = far_reserve(f < F, &S.far_n2);  // f < F also excludes t
                     if (idx == 0xFFFFFFFFu) continue;
@@ -494,6 +552,14 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             }
             const uint32_t fmin = block_min_u32<B>(mymin, S.red);
             if (fmin >= F) {  // nothing in the far pile can improve t (also: far pile empty)
+                if constexpr (RESUME) {  // kept for later batches
+                    for (uint32_t i0 = 0; i0 < nf; i0 += B) {
+                        const uint32_t i = i0 + lane;
+                        uint64_t k[1] = {i < nf ? far[i] : 0};
+                        const bool dz[1] = {i < nf && (uint32_t)(k[0] >> 32) >= F && (int32_t)(uint32_t)k[0] != t};
+                        dorm_append<1>(*Z, S, dz, k);
+                    }
+                }
                 if (!early) break;
                 if (lane == 0) S.far_n = 0;  // F only falls: those entries are hopeless for good
                 __syncthreads();
@@ -531,6 +597,14 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
                 load_chunk(base, k2, l2);
                 (void)pile_place<KC>(P, thr, lds_ok, lh, hh, &S.lring_tail, &S.near_tail, far2, &S.far_n2, l2,
                                      k2);
+                if constexpr (RESUME) {  // hopeless now, kept for later batches
+                    bool dz[KC];
+#pragma unroll
+                    for (int j = 0; j < KC; ++j)
+                        dz[j] = base + (uint32_t)j * B + lane < nf && (uint32_t)(k2[j] >> 32) >= F &&
+                                (int32_t)(uint32_t)k2[j] != t;
+                    dorm_append<KC>(*Z, S, dz, k2);
+                }
             }
             block_sync_global();
             if (lane == 0) S.far_n = S.far_n2;
@@ -582,8 +656,8 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             S.near_head = hh;
             S.phase_n += n;
         }
-        PilePush<B, B * W> push{&P, far, &S, touched};
-        expand_rows<B, HK, W, PilePush<B, B * W>, false>(G, hv, gp, t, u, rs, S.L, C, push);
+        PilePush<B, B * W, RESUME> push{&P, far, &S, touched, Z};
+        expand_rows<B, HK, W, PilePush<B, B * W, RESUME>, false>(G, hv, gp, t, u, rs, S.L, C, push);
         if (push.err) atomicOr(&S.err, push.err);
         __syncthreads();  // the round's pushes and errors are visible to the next round
         if (S.err) break;

@@ -84,58 +84,92 @@ void sample_queries(int32_t nq, const int32_t *src, const int32_t *dst, int32_t 
     }
 }
 
+// Best of three timed launches of a batch with options o on the sample (the
+// first launch warms up); -1 if the options do not fit or a query fails.
+// Costs of the last launch go to costs (n entries) when given.
+float time_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_batch_opts_t &o,
+                 const std::vector<int32_t> &s, const std::vector<int32_t> &t, std::vector<int32_t> *costs)
+{
+    const int32_t n = (int32_t)s.size();
+    ddp_batch *b = nullptr;
+    if (ddp_batch_create(ctx, g, h, n, &o, &b) != DDP_OK) return -1.f;
+    float ms = 1e30f;
+    bool ok = ddp_batch_upload(b, n, s.data(), t.data()) == DDP_OK;
+    for (int rep = 0; ok && rep < 3; ++rep) {
+        ddp_batch_stats_t st;
+        ok = ddp_batch_launch(b) == DDP_OK && ddp_batch_wait(b, &st) == DDP_OK && st.errors == 0;
+        if (ok && rep > 0) ms = std::min(ms, st.kernel_ms);
+    }
+    if (ok && costs) {
+        costs->assign(n, 0);
+        ok = ddp_batch_download(b, costs->data(), nullptr, nullptr, nullptr, nullptr) == DDP_OK;
+    }
+    ddp_batch_destroy(b);
+    return ok ? n / (ms * 1e-3f) : -1.f;
+}
+
 }  // namespace
 
 extern "C" int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
                               const int32_t *dst, int32_t sample, ddp_batch_opts_t *best, float *qps)
 {
     if (!ctx || !g || !best || nq <= 0 || !src || !dst) return DDP_ERR_ARG;
-    const std::string k = key(ctx, g, h, "batch-engine", std::min(nq, sample));
+    // the key names every option the tuner varies, so entries of older tuners never match
+    const std::string k = key(ctx, g, h, "batch:block,engine,heap_cap,lds_cap", std::min(nq, sample));
     std::vector<long> cached;
-    if (cache_get(k, cached) && cached.size() >= 3) {
+    if (cache_get(k, cached) && cached.size() >= 5) {
         std::memset(best, 0, sizeof *best);
         best->block = (int32_t)cached[0];
         best->engine = (int32_t)cached[1];
-        if (qps) *qps = (float)cached[2];
+        best->heap_cap = (int32_t)cached[2];
+        best->lds_cap = (int32_t)cached[3];
+        if (qps) *qps = (float)cached[4];
         return DDP_OK;
     }
     std::vector<int32_t> s, t;
     sample_queries(nq, src, dst, sample, s, t);
-    const int32_t n = (int32_t)s.size();
+    int32_t N = 0;
+    ddp_graph_info(g, &N, nullptr, nullptr, nullptr);
+    // (block, engine, heap_cap, lds_cap); 0 = the library's default.  Heap
+    // depth: the heap engine's LDS levels per lane and its HBM entries per
+    // lane; the pile engines' queue memory per lane (their default holds an
+    // exhaustive search, 3N entries per slot and direction; half of it can
+    // drop pushes into far-pile rebuilds on long searches).
+    struct Cand {
+        int32_t B, eng, cap, lcap;
+    };
+    std::vector<Cand> cand = {{256, DDP_BATCH_PILES, 0, 0},  {512, DDP_BATCH_PILES, 0, 0},
+                              {1024, DDP_BATCH_PILES, 0, 0}, {1024, DDP_BATCH_PILES, std::max(2048, (3 * N / 2) / 1024), 0},
+                              {512, DDP_BATCH_HEAPS, 0, 8},  {512, DDP_BATCH_HEAPS, 0, 16},
+                              {512, DDP_BATCH_HEAPS, 4096, 16}};
+    const bool closed_form = h.kind != DDP_H_ARRAY;
+    if (closed_form) {  // needs transposed rows; create refuses the candidate without them
+        cand.push_back({512, DDP_BATCH_BIDI, 0, 0});
+        cand.push_back({1024, DDP_BATCH_BIDI, 0, 0});
+    }
     float best_qps = -1.f;
-    int32_t best_block = 256, best_engine = DDP_BATCH_PILES;
-    const int32_t cand[][2] = {{256, DDP_BATCH_PILES}, {512, DDP_BATCH_PILES}, {1024, DDP_BATCH_PILES},
-                               {512, DDP_BATCH_HEAPS}};
-    for (const auto &c : cand) {
-        const int32_t B = c[0];
+    Cand win{256, DDP_BATCH_PILES, 0, 0};
+    for (const Cand &c : cand) {
         ddp_batch_opts_t o;
         std::memset(&o, 0, sizeof o);
-        o.block = B;
-        o.engine = c[1];
-        ddp_batch *b = nullptr;
-        if (ddp_batch_create(ctx, g, h, n, &o, &b) != DDP_OK) continue;
-        float ms = 1e30f;
-        bool ok = ddp_batch_upload(b, n, s.data(), t.data()) == DDP_OK;
-        for (int rep = 0; ok && rep < 3; ++rep) {
-            ddp_batch_stats_t st;
-            ok = ddp_batch_launch(b) == DDP_OK && ddp_batch_wait(b, &st) == DDP_OK && st.errors == 0;
-            if (ok && rep > 0) ms = std::min(ms, st.kernel_ms);  // first launch warms up
-        }
-        ddp_batch_destroy(b);
-        if (!ok) continue;
-        const float q = n / (ms * 1e-3f);
+        o.block = c.B;
+        o.engine = c.eng;
+        o.heap_cap = c.cap;
+        o.lds_cap = c.lcap;
+        const float q = time_batch(ctx, g, h, o, s, t, nullptr);
         if (q > best_qps) {
             best_qps = q;
-            best_block = B;
-            best_engine = c[1];
+            win = c;
         }
     }
     if (best_qps < 0) return DDP_ERR_STATE;
     std::memset(best, 0, sizeof *best);
-    best->block = best_block;
-    best->engine = best_engine;
+    best->block = win.B;
+    best->engine = win.eng;
+    best->heap_cap = win.cap;
+    best->lds_cap = win.lcap;
     if (qps) *qps = best_qps;
-    cache_put(k, {best_block, best_engine, (long)best_qps});
+    cache_put(k, {win.B, win.eng, win.cap, win.lcap, (long)best_qps});
     return DDP_OK;
 }
 
@@ -143,14 +177,16 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
                               const int32_t *dst, int32_t sample, ddp_astar_opts_t *best, float *mean_ms)
 {
     if (!ctx || !g || !best || nq <= 0 || !src || !dst) return DDP_ERR_ARG;
-    const std::string k = key(ctx, g, h, "astar-mode", std::min(nq, sample));
+    const std::string k = key(ctx, g, h, "astar:K,block,mode,heap_cap,lds_cap", std::min(nq, sample));
     std::vector<long> cached;
-    if (cache_get(k, cached) && cached.size() >= 4) {
+    if (cache_get(k, cached) && cached.size() >= 6) {
         std::memset(best, 0, sizeof *best);
         best->K = (int32_t)cached[0];
         best->block = (int32_t)cached[1];
         best->mode = (int32_t)cached[2];
-        if (mean_ms) *mean_ms = cached[3] * 1e-3f;
+        best->heap_cap = (int32_t)cached[3];
+        best->lds_cap = (int32_t)cached[4];
+        if (mean_ms) *mean_ms = cached[5] * 1e-3f;
         return DDP_OK;
     }
     std::vector<int32_t> s, t;
@@ -167,29 +203,45 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     for (int32_t K : {512, 1024}) cands.push_back({K, 0, DDP_ASTAR_PILES});
     for (int32_t K : {1024, 2048, 4096, 8192})
         for (int32_t B : {64, 256}) cands.push_back({K, B, DDP_ASTAR_GRID});
+    auto time_opts = [&](const ddp_astar_opts_t &o) {
+        double total = 0;
+        for (size_t i = 0; i < s.size(); ++i) {
+            int32_t cost = 0;
+            ddp_astar_stats_t st;
+            if (ddp_astar(ctx, g, h, s[i], t[i], &o, &cost, nullptr, nullptr, 0, nullptr, &st) != DDP_OK) return 1e300;
+            total += st.total_ms;
+        }
+        return total;
+    };
     for (const Cand &c : cands) {
-            ddp_astar_opts_t o;
-            std::memset(&o, 0, sizeof o);
-            o.K = c.K;
-            o.block = c.B;
-            o.mode = c.mode;
-            double total = 0;
-            bool ok = true;
-            for (size_t i = 0; i < s.size() && ok; ++i) {
-                int32_t cost = 0;
-                ddp_astar_stats_t st;
-                ok = ddp_astar(ctx, g, h, s[i], t[i], &o, &cost, nullptr, nullptr, 0, nullptr, &st) == DDP_OK;
-                total += st.total_ms;
-            }
-            if (ok && total < best_ms) {
+        ddp_astar_opts_t o;
+        std::memset(&o, 0, sizeof o);
+        o.K = c.K;
+        o.block = c.B;
+        o.mode = c.mode;
+        const double total = time_opts(o);
+        if (total < best_ms) {
+            best_ms = total;
+            win = o;
+        }
+    }
+    if (best_ms >= 1e300) return DDP_ERR_STATE;
+    // Heap depth of the winning shape: LDS levels per queue (grid and
+    // workgroup modes; the rest spills to HBM) and HBM entries per queue.
+    if (win.mode != DDP_ASTAR_PILES) {
+        for (int32_t lc : {8, 16, 32}) {
+            ddp_astar_opts_t o = win;
+            o.lds_cap = lc;
+            const double total = time_opts(o);
+            if (total < best_ms) {
                 best_ms = total;
                 win = o;
             }
         }
-    if (best_ms >= 1e300) return DDP_ERR_STATE;
+    }
     *best = win;
     const double mean = best_ms / s.size();
     if (mean_ms) *mean_ms = (float)mean;
-    cache_put(k, {win.K, win.block, win.mode, (long)(mean * 1e3)});
+    cache_put(k, {win.K, win.block, win.mode, win.heap_cap, win.lds_cap, (long)(mean * 1e3)});
     return DDP_OK;
 }

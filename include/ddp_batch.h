@@ -39,6 +39,14 @@ typedef struct {
 #define DDP_BATCH_AUTO  0
 #define DDP_BATCH_HEAPS 1
 #define DDP_BATCH_PILES 2
+/* BIDI: the pile engine run from both ends at once in one workgroup, a
+ * forward search from s over the rows and a backward one from t over the
+ * transposed rows, meeting in the middle (about half the rounds per query).
+ * Needs a graph uploaded with DDP_GRAPH_REVERSE, a closed-form heuristic
+ * (DDP_H_ZERO, _MANHATTAN or _EUCLID; the keys use the average potential
+ * (h_t - h_s) / 2) and a block of 256, 512 or 1024 lanes; a slot then holds
+ * two (f, parent) arrays and twice the queue memory.  Same exact costs. */
+#define DDP_BATCH_BIDI  3
 
 typedef struct {
     int64_t expansions, edges, improvements, stale_pops;

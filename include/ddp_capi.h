@@ -166,7 +166,7 @@ typedef struct {
     int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = auto */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
     int32_t max_iters;  /* iteration limit; 0 = auto (bounded) */
-    int32_t mode;       /* DDP_ASTAR_AUTO / _GRID / _WORKGROUP / _PILES */
+    int32_t mode;       /* DDP_ASTAR_AUTO / _GRID / _WORKGROUP / _PILES / _BIDI */
     int32_t reserved[2];
 } ddp_astar_opts_t;
 
@@ -181,6 +181,14 @@ typedef struct {
 #define DDP_ASTAR_GRID      1
 #define DDP_ASTAR_WORKGROUP 2
 #define DDP_ASTAR_PILES     3
+/* BIDI: the pile search of PILES run from both ends at once in one workgroup
+ * of K (256..1024) lanes -- forward from s over the rows, backward from t
+ * over the transposed rows, keys g +- (h_t - h_s) / 2 -- until no open entry
+ * on either side can shorten the best meeting.  Needs DDP_GRAPH_REVERSE and a
+ * closed-form heuristic (not DDP_H_ARRAY).  parent_out then holds the forward
+ * search's parents, with the path's own predecessors from the meeting node
+ * on to t. */
+#define DDP_ASTAR_BIDI      4
 
 typedef struct {
     int64_t expansions;     /* nodes popped and expanded */

@@ -46,6 +46,8 @@ typedef struct {
     float repair_ms;            /* device invalidation, re-pull, insert relax */
     float search_ms;            /* resumed (or recomputed) GA* kernel time */
     float total_ms;
+    int32_t compacted;          /* 1 if the batch ended with a log compaction */
+    float compact_ms;           /* its time (included in total_ms) */
 } ddp_dastar_stats_t;
 
 /* Uploads the graph (forward + transposed rows), runs the initial GA* from s
@@ -59,6 +61,19 @@ int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_t n, int32_
                      int32_t *cost, ddp_dastar_stats_t *stats);
 /* Current path s..t (*len = 0 if unreachable, -len if cap is too small). */
 int ddp_dastar_path(ddp_dastar *d, int32_t *path, int32_t cap, int32_t *len);
+/* Folds the inserted-edge log back into the CSR rows on the device, as the
+ * reference's merge_graph / mergeDiff rebuild merged rows after every batch
+ * (DA_star/DA_dyn_insert_del_nocycle.cu:940-980, 2015-2102, 2236-2284;
+ * DA_star/GA_merge.cu:1042-1089): each row becomes its surviving base edges in
+ * row order followed by its live inserted edges in insert order, deleted
+ * edges are dropped, the transposed rows are rebuilt and the log is emptied.
+ * The graph version and the search state are unchanged, so costs, paths and
+ * ddp_dastar_export read the same before and after.  ddp_dastar_apply also
+ * compacts by itself after a batch that leaves more than the threshold of
+ * ddp_dastar_set_compaction in the log (default max(65536, E/4) entries;
+ * 0 turns that off). */
+int ddp_dastar_compact(ddp_dastar *d);
+int ddp_dastar_set_compaction(ddp_dastar *d, int64_t log_entries);
 /* Live edge count of the current graph version. */
 int ddp_dastar_num_edges(const ddp_dastar *d, int32_t *E);
 /* Current graph in merged order (surviving base edges, then inserts, per

@@ -40,7 +40,8 @@ typedef struct {
     int32_t heap_cap;       /* queue entries per lane (a slot holds heap_cap * block);
                                0 = 2048 (heaps) or max(2048, 3N / block) (piles) */
     int32_t ring;           /* request ring entries (power of two); 0 = 65536 */
-    int32_t max_inserts;    /* capacity of the inserted-edge log; 0 = 1<<20 */
+    int32_t max_inserts;    /* capacity of the inserted-edge log; 0 = 1<<20.  An update
+                               that does not fit triggers ddp_server_compact first */
     int32_t idle_ms;        /* the kernel exits after this long without work; 0 = 2000 */
     int32_t engine;         /* DDP_SERVER_AUTO / _HEAPS / _PILES (as DDP_BATCH_*) */
     int32_t reserved;
@@ -51,6 +52,11 @@ typedef struct {
 #define DDP_SERVER_AUTO  0
 #define DDP_SERVER_HEAPS 1
 #define DDP_SERVER_PILES 2
+/* BIDI: each query searched from both ends at once (DDP_BATCH_BIDI) on the
+ * versioned graph: transposed rows carry their own delete versions and
+ * inserted edges are listed by target too.  Needs DDP_GRAPH_REVERSE and a
+ * block of 256, 512 or 1024 lanes; a slot holds two (f, parent) arrays. */
+#define DDP_SERVER_BIDI  3
 
 #define DDP_REQ_QUERY 0
 #define DDP_REQ_UPDATE 1
@@ -74,11 +80,22 @@ int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_
                       ddp_server **out);
 int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_t *version);
 /* Non-blocking; DDP_ERR_NOMEM if the ring (or, for updates, the payload arena)
- * is full: try again after polling.  ddp_server_submit_update returns
- * DDP_ERR_CAPACITY when the inserted-edge log cannot take the batch, which
+ * is full: try again after polling.  An update whose inserts do not fit in
+ * the rest of the inserted-edge log first compacts it (ddp_server_compact,
+ * which blocks); DDP_ERR_CAPACITY if the batch alone exceeds the log, which
  * polling does not cure. */
 int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uint64_t tag);
 int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *lines, int32_t n, uint64_t tag);
+/* Folds the inserted-edge log back into CSR rows at the current version, as
+ * the reference's merge_graph / mergeDiff rebuild merged rows
+ * (DA_star/DA_dyn_insert_del_nocycle.cu:940-980, 2236-2284;
+ * DA_star/GA_merge.cu:1042-1089): each row becomes its live base edges in row
+ * order followed by its live inserted edges in insert order, deleted edges
+ * are dropped and the log is emptied, so later deletes still take the first
+ * live instance of the merged row.  Waits until every submitted request has
+ * finished and stops the kernel first (queries read the log); results stay
+ * available to ddp_server_poll.  Requests submitted afterwards relaunch it. */
+int ddp_server_compact(ddp_server *sv);
 /* Collects up to max finished requests in completion order of submission. */
 int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t max, int32_t *n);
 /* Waits until every submitted request has finished (bounded by timeout_ms). */

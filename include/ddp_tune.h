@@ -16,16 +16,22 @@
 extern "C" {
 #endif
 
-/* Times the batched kernel (pile engine at blocks 256/512/1024, heap engine
- * at 512) on up to `sample` of the given queries (spread over the schedule
- * order) and writes the fastest options (block, engine) to *best; *qps gets
- * its measured queries/s. */
+/* Times the batched kernel on up to `sample` of the given queries (spread
+ * over the schedule order): the pile engine at blocks 256/512/1024 and, at
+ * 1024, with half its default queue memory; the heap engine at block 512 with
+ * 8 or 16 LDS levels per lane and the default or 4096 entries per lane (heap
+ * depth); the bidirectional engine at 512/1024 when the heuristic is a closed
+ * form and the graph has transposed rows.  Writes the fastest options (block,
+ * engine, heap_cap, lds_cap; 0 = default) to *best; *qps gets its measured
+ * queries/s.  Cached under a key naming every option it varies. */
 int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
                    const int32_t *dst, int32_t sample, ddp_batch_opts_t *best, float *qps);
 
 /* Times single-query GA* in workgroup mode (K 256..1024), pile mode (K 512,
  * 1024) and grid mode (K 1024..8192, block 64/256) on up to `sample` of the
- * queries; writes the fastest options (K, block, mode) and its mean ms. */
+ * queries, then the winner with 8, 16 and 32 LDS heap levels per queue (grid
+ * and workgroup modes); writes the fastest options (K, block, mode, heap_cap,
+ * lds_cap) and its mean ms. */
 int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, const int32_t *src,
                    const int32_t *dst, int32_t sample, ddp_astar_opts_t *best, float *mean_ms);
 

@@ -300,3 +300,77 @@ def test_dastar_bad_mode_changes_nothing(ctx):
     g = da.graph()
     assert np.array_equal(g.col, cur[1])
     da.close()
+
+
+def test_dastar_compaction(ctx):
+    """Log compaction (merge_graph / mergeDiff, DA_star/DA_dyn_insert_del_nocycle.cu:940-980,
+    2236-2284): with a small threshold the log is folded into the rows every
+    few batches, and an explicit compaction runs in between.  Every batch:
+    repair == recompute == the oracle's Dijkstra on the oracle-applied graph,
+    the path is real, and the exported graph equals the oracle's edge for edge
+    (GA_merge.cu's merged rows) whether or not the batch compacted."""
+    hg = ddp_amd.HostGraph.random(5000, 5, 1, 80, 81)
+    src, dst = hg.reachable_pairs(1, 82)
+    s, t = int(src[0]), int(dst[0])
+    upd = hg.gen_updates(24, 120, 120, 1, 80, 83).batches
+    da = ddp_amd.DAStar(ctx, hg, s, t, K=1024)
+    rc = ddp_amd.DAStar(ctx, hg, s, t, K=1024)
+    da.set_compaction(300)  # every third batch or so
+    cur = (hg.off, hg.col, hg.w)
+    compacted = 0
+    for bi, lines in enumerate(upd):
+        cost, st = da.apply(lines, mode=ddp_amd.DAStar.REPAIR)
+        cost_rc, _ = rc.apply(lines, mode=ddp_amd.DAStar.RECOMPUTE)
+        compacted += st["compacted"]
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
+        d, _ = pyoracle.dijkstra(*cur, s, t)
+        want = int(d[t]) if d[t] >= 0 else ddp_amd.INF_COST
+        say("compaction batch", bi, cost, want, "compacted" if st["compacted"] else "")
+        assert cost == cost_rc == want, bi
+        if want != ddp_amd.INF_COST:
+            p = da.path()
+            assert p[0] == s and p[-1] == t and path_cost(*cur, p) == want
+        g = da.graph()
+        assert np.array_equal(g.off, cur[0]) and np.array_equal(g.col, cur[1]) and np.array_equal(g.w, cur[2]), bi
+        if bi == 10:
+            da.compact()  # explicit, mid-run
+            g = da.graph()
+            assert np.array_equal(g.col, cur[1]) and np.array_equal(g.w, cur[2])
+    assert compacted >= 4
+    da.close()
+    rc.close()
+
+
+@pytest.mark.parametrize("seed", [101, 102])
+def test_dastar_pile_mode(ctx, seed):
+    """DA* with every search on one workgroup's pile engine (DDP_ASTAR_PILES):
+    the open set left by each search -- entries that could not beat f(t) kept
+    aside -- is resumed after the next batch's repair.  Every batch: the same
+    cost as the grid-mode session and the oracle's Dijkstra, a real path, and
+    recompute in pile mode agrees; a compaction in the middle changes nothing."""
+    hg = ddp_amd.HostGraph.random(20000, 6, 1, 100, seed)
+    src, dst = hg.reachable_pairs(1, seed + 1)
+    s, t = int(src[0]), int(dst[0])
+    upd = hg.gen_updates(12, 400, 400, 1, 100, seed + 2).batches
+    pl = ddp_amd.DAStar(ctx, hg, s, t, K=1024, mode=ddp_amd.MODE_PILES)
+    gr = ddp_amd.DAStar(ctx, hg, s, t, K=1024)
+    rc = ddp_amd.DAStar(ctx, hg, s, t, K=1024, mode=ddp_amd.MODE_PILES)
+    assert pl.cost == gr.cost == rc.cost
+    cur = (hg.off, hg.col, hg.w)
+    for bi, lines in enumerate(upd):
+        c_pl, st = pl.apply(lines, mode=ddp_amd.DAStar.REPAIR)
+        c_gr, _ = gr.apply(lines, mode=ddp_amd.DAStar.REPAIR)
+        c_rc, _ = rc.apply(lines, mode=ddp_amd.DAStar.RECOMPUTE)
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
+        d, _ = pyoracle.dijkstra(*cur, s, t)
+        want = int(d[t]) if d[t] >= 0 else ddp_amd.INF_COST
+        say("pile DA* batch", bi, c_pl, c_gr, c_rc, want, "rounds", st["iterations"], "invalidated", st["invalidated"])
+        assert c_pl == c_gr == c_rc == want, bi
+        if want != ddp_amd.INF_COST:
+            p = pl.path()
+            assert p[0] == s and p[-1] == t and path_cost(*cur, p) == want
+        if bi == 5:
+            pl.compact()
+    pl.close()
+    gr.close()
+    rc.close()

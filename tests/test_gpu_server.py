@@ -259,24 +259,26 @@ def test_server_config5_served(ctx):
             assert got[i] == d[qt[i]]
 
 
-def test_server_replay_log_full_returns_capacity(ctx):
-    """A full inserted-edge log is not a transient condition: replay returns
-    DDP_ERR_CAPACITY instead of polling forever, and the updates and queries
-    submitted before it still finish correctly."""
+def test_server_oversized_update_returns_capacity(ctx):
+    """A batch with more inserts than the whole log cannot be taken even after
+    a compaction: replay returns DDP_ERR_CAPACITY instead of polling forever;
+    the requests before it finished, and later queries run on the version the
+    accepted batches made."""
     hg = ddp_amd.HostGraph.random(2000, 4, 1, 50, 61)
     dg = ddp_amd.DeviceGraph(ctx, hg)
     src, dst = hg.reachable_pairs(8, 62)
-    ub = hg.gen_updates(4, 30, 10, 1, 50, 63).batches  # 30 inserts each; the log holds 64
+    ub = hg.gen_updates(3, 30, 10, 1, 50, 63).batches
+    big = hg.gen_updates(1, 100, 0, 1, 50, 64).batches[0]  # 100 inserts; the log holds 64
     sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, max_inserts=64, slots=4)
     n = 8
     types = np.array([0, 1, 0, 1, 0, 1, 0, 0], np.int32)
-    ups = [None, ub[0], None, ub[1], None, ub[2], None, None]
+    ups = [None, ub[0], None, ub[1], None, big, None, None]
     with pytest.raises(ddp_amd.DDPError) as e:
         sv.replay(np.zeros(n), types, src[:n], dst[:n], ups)
     assert "DDP_ERR_CAPACITY" in str(e.value)
     sv.drain()
     sv.poll(64)
-    assert sv.info()["version"] == 2  # the third update was refused, the first two applied
+    assert sv.info()["version"] == 2  # the oversized third update was refused, the first two applied
     g2 = pyoracle.apply_batch(*pyoracle.apply_batch(hg.off, hg.col, hg.w, ub[0])[:3], ub[1])[:3]
     for k in range(4):
         assert sv.submit_query(int(src[k]), int(dst[k]), 100 + k) == 0
@@ -287,4 +289,136 @@ def test_server_replay_log_full_returns_capacity(ctx):
         r = done[100 + k]
         assert r["version"] == 2 and r["status"] == 0
         assert r["cost"] == (d[dst[k]] if d[dst[k]] >= 0 else ddp_amd.INF_COST)
+    sv.close()
+
+
+@pytest.mark.timeout(900)
+def test_server_log_compaction_many_batches(ctx):
+    """5,000 update batches of the C5 mix (deletes of live edges, random
+    inserts) on a small graph whose log holds 40 batches: the server folds
+    the log into its rows over a hundred times, and every query -- submitted
+    between the batches, before and after compactions -- returns the oracle's
+    Dijkstra cost on the graph version it was bound to."""
+    hg = ddp_amd.HostGraph.random(3000, 6, 1, 1000, 91)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    nb, per = 5000, 25
+    rng = np.random.default_rng(92)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, max_inserts=40 * per, slots=16)
+    cur = (hg.off, hg.col, hg.w)
+    pending = {}  # tag -> (version, s, t)
+    checks = {}
+    for b in range(nb):
+        o = np.append(cur[0], len(cur[1])).astype(np.int64)
+        deg = np.diff(o)
+        us = rng.integers(0, hg.N, per)
+        us = us[deg[us] > 0]
+        lines = [(0, int(u), int(cur[1][o[u] + rng.integers(0, deg[u])]), 0) for u in us]
+        lines += [(1, int(a), int(c), int(x)) for a, c, x in
+                  zip(rng.integers(0, hg.N, per), rng.integers(0, hg.N, per), rng.integers(1, 1001, per))]
+        assert sv.submit_update(lines, 10**6 + b) == 0
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
+        if b % 50 == 49:  # a few queries on this version, checked against the oracle below
+            v = b + 1
+            for k in range(2):
+                s, t = int(rng.integers(0, hg.N)), int(rng.integers(0, hg.N))
+                tag = 10 * v + k
+                assert sv.submit_query(s, t, tag) == 0
+                d, _ = pyoracle.dijkstra(*cur, s)
+                checks[tag] = (v, int(d[t]) if d[t] >= 0 else ddp_amd.INF_COST)
+        for r in sv.poll(4096):
+            pending[r["tag"]] = r
+        if b % 1000 == 999:
+            say("compaction test: batch", b + 1)
+    sv.drain()
+    for r in sv.poll(65536):
+        pending[r["tag"]] = r
+    assert all(r["status"] == 0 for r in pending.values())
+    for tag, (v, want) in checks.items():
+        r = pending[tag]
+        assert r["version"] == v and r["cost"] == want, (tag, r, want)
+    assert sv.info()["version"] == nb
+    sv.close()
+
+
+def test_server_bidi_versions_and_costs(ctx):
+    """The bidirectional serving engine on a versioned graph: deletes hide
+    transposed entries too, inserts appear on per-target lists, and every
+    query equals Dijkstra on its version (relabelled and input numbering)."""
+    hg = ddp_amd.HostGraph.random(6000, 5, 1, 100, 41)
+    arrival, types, src, dst, ups = make_trace(hg, 400, 20, 42)
+    for reorder, block in ((False, 256), (True, 1024)):
+        dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True, reorder=reorder)
+        say("bidi server block", block, "relabelled", reorder)
+        sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, block=block, slots=32, engine=ddp_amd.ENGINE_BIDI)
+        res = sv.replay(arrival, types, src, dst, ups)
+        check(res, types, src, dst, ups, hg)
+        sv.close()
+        dg.close()
+
+
+def test_server_bidi_euclid_matches_batch(ctx):
+    hg = ddp_amd.HostGraph.geometric(20000, 10, 100000, 43)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True, reorder=True)
+    ub = geometric_updates(hg, 3, 200, 44)
+    src, dst = hg.geo_pairs(60, 500, 100000, 45)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_EUCLID, slots=16, engine=ddp_amd.ENGINE_BIDI)
+    plan = []
+    for v in range(4):
+        if v > 0:
+            assert sv.submit_update(ub[v - 1], 10000 + v) == 0
+        for i in range(v * 15, v * 15 + 15):
+            assert sv.submit_query(int(src[i]), int(dst[i]), i) == 0
+            plan.append((v, i))
+    sv.drain()
+    got = {r["tag"]: r for r in sv.poll(1000)}
+    sv.close()
+    vers = versions(hg, ub)
+    for v in range(4):
+        g = ddp_amd.HostGraph(*vers[v], xy=hg.xy)
+        dgv = ddp_amd.DeviceGraph(ctx, g)
+        idx = [i for (vv, i) in plan if vv == v]
+        cost, _, _ = ddp_amd.astar_batch(ctx, dgv, src[idx], dst[idx], heur=ddp_amd.H_EUCLID)
+        for k, i in enumerate(idx):
+            assert got[i]["version"] == v and got[i]["status"] == 0 and got[i]["cost"] == cost[k], (v, i)
+        dgv.close()
+
+
+@pytest.mark.timeout(600)
+def test_server_bidi_compaction(ctx):
+    """Compaction with the bidirectional engine rebuilds the transposed rows
+    and their delete versions: 600 update batches through a log of 20 batches,
+    queries checked against Dijkstra on their versions."""
+    hg = ddp_amd.HostGraph.random(2000, 6, 1, 1000, 93)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)
+    nb, per = 600, 20
+    rng = np.random.default_rng(94)
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, max_inserts=20 * per, slots=16, block=256,
+                        engine=ddp_amd.ENGINE_BIDI)
+    cur = (hg.off, hg.col, hg.w)
+    got, checks = {}, {}
+    for b in range(nb):
+        o = np.append(cur[0], len(cur[1])).astype(np.int64)
+        deg = np.diff(o)
+        us = rng.integers(0, hg.N, per)
+        us = us[deg[us] > 0]
+        lines = [(0, int(u), int(cur[1][o[u] + rng.integers(0, deg[u])]), 0) for u in us]
+        lines += [(1, int(a), int(c), int(x)) for a, c, x in
+                  zip(rng.integers(0, hg.N, per), rng.integers(0, hg.N, per), rng.integers(1, 1001, per))]
+        assert sv.submit_update(lines, 10**6 + b) == 0
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
+        if b % 15 == 14:
+            for k in range(2):
+                s, t = int(rng.integers(0, hg.N)), int(rng.integers(0, hg.N))
+                tag = 10 * (b + 1) + k
+                assert sv.submit_query(s, t, tag) == 0
+                d, _ = pyoracle.dijkstra(*cur, s)
+                checks[tag] = (b + 1, int(d[t]) if d[t] >= 0 else ddp_amd.INF_COST)
+        for r in sv.poll(4096):
+            got[r["tag"]] = r
+    sv.drain()
+    for r in sv.poll(65536):
+        got[r["tag"]] = r
+    assert all(r["status"] == 0 for r in got.values())
+    for tag, (v, want) in checks.items():
+        assert got[tag]["version"] == v and got[tag]["cost"] == want, (tag, got[tag], want)
     sv.close()
