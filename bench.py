@@ -145,7 +145,7 @@ def geometric_updates(hg, n, per, seed):
     return out
 
 
-def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=0, block=0):
+def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=0, block=0, engine=0):
     """Open-loop replay: Poisson arrivals at `qps` (all at once when qps <= 0),
     95% point queries and 5% update batches.  Returns (results, types, query
     index per request, secs, server info)."""
@@ -160,7 +160,7 @@ def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=
         ups.append(upd_batches[k] if ty else None)
         k += int(ty)
     arrival = np.zeros(nreq) if qps <= 0 else np.cumsum(rng.exponential(1000.0 / qps, nreq))
-    sv = ddp_amd.Server(ctx, dg, heur=heur, heap_cap=heap_cap, block=block)
+    sv = ddp_amd.Server(ctx, dg, heur=heur, heap_cap=heap_cap, block=block, engine=engine)
     t0 = time.perf_counter()
     out = sv.replay(arrival, types, src[qi], dst[qi], ups)
     el = time.perf_counter() - t0
@@ -366,7 +366,8 @@ def run_c2(args):
     cost = None
     for name, K, mode in (("512-workgroup", 512, ddp_amd.MODE_WORKGROUP), ("1024-workgroup", 1024,
                           ddp_amd.MODE_WORKGROUP), ("1024-piles", 1024, ddp_amd.MODE_PILES),
-                          ("512-bidi", 512, ddp_amd.MODE_BIDI), ("1024-bidi", 1024, ddp_amd.MODE_BIDI),
+                          ("256-bidi", 256, ddp_amd.MODE_BIDI), ("512-bidi", 512, ddp_amd.MODE_BIDI),
+                          ("1024-bidi", 1024, ddp_amd.MODE_BIDI),
                           ("512-grid", 512, ddp_amd.MODE_GRID),
                           ("2048-grid", 2048, ddp_amd.MODE_GRID), ("8192-grid", 8192, ddp_amd.MODE_GRID)):
         r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_MANHATTAN, K, args.steps, args.warmup, mode=mode)
@@ -427,22 +428,33 @@ def run_c3(args):
     ug = ddp_amd.DeviceGraph(ctx, ddp_amd.HostGraph(uoff, vs[order], ws[order]), reverse=True)
     h, _ = ug.bfs_hops(t, reverse=True, scale=int(ws.min()))
     ug.close()
-    rep = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K)
-    rec = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K)
-    rep_ms, rec_ms, rep_search, rec_search, per = [], [], [], [], []
+    # repair and full recompute, each with the GA* grid kernel (K queues) and
+    # with one workgroup's pile engine
+    sessions = {
+        "repair_grid": (ddp_amd.DAStar.REPAIR, ddp_amd.MODE_AUTO),
+        "repair_piles": (ddp_amd.DAStar.REPAIR, ddp_amd.MODE_PILES),
+        "recompute_grid": (ddp_amd.DAStar.RECOMPUTE, ddp_amd.MODE_AUTO),
+        "recompute_piles": (ddp_amd.DAStar.RECOMPUTE, ddp_amd.MODE_PILES),
+    }
+    das = {k: ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K, mode=m) for k, (_, m) in sessions.items()}
+    if len({d.cost for d in das.values()}) != 1:
+        raise SystemExit(f"C3 initial costs differ: {[d.cost for d in das.values()]}")
+    ms = {k: [] for k in sessions}
+    dev = {k: [] for k in sessions}
+    per = []
     cur = (hg.off, hg.col, hg.w)
     checked = 0
     for bi, lines in enumerate(upd):
-        c1, st1 = rep.apply(lines, mode=ddp_amd.DAStar.REPAIR)
-        c2, st2 = rec.apply(lines, mode=ddp_amd.DAStar.RECOMPUTE)
-        if c1 != c2:
-            raise SystemExit(f"C3 batch {bi}: repair {c1} != recompute {c2}")
-        rep_ms.append(st1["total_ms"])
-        rec_ms.append(st2["total_ms"])
-        rep_search.append(st1["repair_ms"] + st1["search_ms"])
-        rec_search.append(st2["search_ms"])
-        per.append({"batch": bi, "cost": c1, "invalidated": st1["invalidated"], "repair_ms": round(st1["total_ms"], 3),
-                    "recompute_ms": round(st2["total_ms"], 3)})
+        costs = {}
+        for k, (mode, _) in sessions.items():
+            c, st = das[k].apply(lines, mode=mode)
+            costs[k] = c
+            ms[k].append(st["total_ms"])
+            dev[k].append(st["repair_ms"] + st["search_ms"])
+        if len(set(costs.values())) != 1:
+            raise SystemExit(f"C3 batch {bi}: sessions disagree {costs}")
+        c1 = costs["repair_piles"]
+        per.append({"batch": bi, "cost": c1, **{k: round(v[-1], 3) for k, v in ms.items()}})
         if bi % max(1, nb // 5) == 0 or bi == nb - 1:
             cur_full = cur
             for b2 in upd[checked:bi + 1]:
@@ -453,17 +465,20 @@ def run_c3(args):
             if c1 != want:
                 raise SystemExit(f"C3 batch {bi}: GPU {c1} oracle {want}")
         log(f"C3 batch {bi}: {per[-1]}")
-    rep.close()
-    rec.close()
-    mean_rep, mean_rec = float(np.mean(rep_ms)), float(np.mean(rec_ms))
-    res = base_record(args, round(1000.0 / mean_rep, 3), "batches/s", round(mean_rep, 3),
+    for d in das.values():
+        d.close()
+    mean = {k: float(np.mean(v)) for k, v in ms.items()}
+    best_rep = min(("repair_grid", "repair_piles"), key=lambda k: mean[k])
+    best_rec = min(("recompute_grid", "recompute_piles"), key=lambda k: mean[k])
+    res = base_record(args, round(1000.0 / mean[best_rep], 3), "batches/s", round(mean[best_rep], 3),
                       f"C3 DA*: random N={n_nodes:,}, out-degree 8, w in [1,1000], seed 3, {nb} batches of "
                       f"{args.c3_batch // 2:,} inserts + {args.c3_batch // 2:,} deletes, h on the union graph")
-    res["config"].update({"N": hg.N, "E": hg.E, "K": args.c3_K, "s": s, "t": t})
-    res.update({"repair_ms_mean": round(mean_rep, 3), "recompute_ms_mean": round(mean_rec, 3),
-                "repair_device_ms_mean": round(float(np.mean(rep_search)), 3),
-                "recompute_device_ms_mean": round(float(np.mean(rec_search)), 3),
-                "speedup_vs_recompute": round(mean_rec / mean_rep, 2), "batches": per})
+    res["config"].update({"N": hg.N, "E": hg.E, "K": args.c3_K, "s": s, "t": t, "repair": best_rep})
+    res.update({"ms_per_batch_mean": {k: round(v, 3) for k, v in mean.items()},
+                "device_ms_per_batch_mean": {k: round(float(np.mean(v)), 3) for k, v in dev.items()},
+                "repair_ms_mean": round(mean[best_rep], 3), "recompute_ms_mean": round(mean[best_rec], 3),
+                "best_recompute": best_rec,
+                "speedup_vs_recompute": round(mean[best_rec] / mean[best_rep], 2), "batches": per})
     ctx.close()
     return res
 
@@ -502,7 +517,10 @@ def run_c5(args):
     # batched kernel on all queries (unmodified graph), one-way piles and bidirectional
     batched = {}
     cost_b = None
-    for name, eng in (("piles", ddp_amd.ENGINE_PILES), ("bidi", ddp_amd.ENGINE_BIDI)):
+    engines = (("piles", ddp_amd.ENGINE_PILES), ("bidi", ddp_amd.ENGINE_BIDI))
+    if args.c5_engine:
+        engines = tuple(e for e in engines if e[0] == args.c5_engine)
+    for name, eng in engines:
         c, _, bst = ddp_amd.astar_batch(ctx, dg, src, dst, heur=ddp_amd.H_ZERO, heap_cap=args.c5_heap_cap, engine=eng)
         if bst["errors"]:
             raise SystemExit(f"C5 batch ({name}): {bst['errors']} queries failed")
@@ -514,6 +532,9 @@ def run_c5(args):
                          "expansions": bst["expansions"], "block": bst["block"], "slots": bst["slots"],
                          "hbm_algorithmic_bytes": 16 * bst["expansions"] + 16 * bst["edges"] + 8 * bst["improvements"]}
         log(f"C5 batch {name}: {batched[name]}")
+    if args.c5_batch_only:
+        ctx.close()
+        return {"batched": batched}
     lat1 = []
     for i in range(min(args.single_sample, 8)):
         q0 = time.perf_counter()
@@ -526,7 +547,7 @@ def run_c5(args):
     # served: closed loop (every request at once) for the capacity, then open
     # loop at fractions of it for the latency curve
     o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, 0.0, args.c5_requests, ups, 11,
-                                       heap_cap=args.c5_heap_cap)
+                                       heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine)
     closed = served_summary(o_, ty, qi, el, info, 0.0, cost_b)
     log(f"C5 served closed loop {closed}")
     cap = closed["queries_per_s"]
@@ -534,7 +555,7 @@ def run_c5(args):
     sweep = [closed]
     for qps in offered:
         o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_sweep_requests, ups, 12,
-                                           heap_cap=args.c5_heap_cap)
+                                           heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine)
         sweep.append(served_summary(o_, ty, qi, el, info, qps, cost_b))
         log(f"C5 served {sweep[-1]}")
     under = [x for x in sweep if 0 < x["offered_qps"] < cap] or sweep
@@ -546,7 +567,8 @@ def run_c5(args):
                       f"95% queries, 5% update batches of 1,000 deletes + 1,000 inserts; value = closed-loop "
                       f"served capacity")
     res["config"].update({"N": hg.N, "E": hg.E, "relabel": not args.no_reorder, "slots": info["slots"],
-                          "serve_block": info["block"]})
+                          "serve_block": info["block"],
+                          "serve_engine": ENGINE_NAMES.get(args.c5_serve_engine, "piles")})
     res.update({"p50_ms": lat_at["p50_ms"], "p99_ms": lat_at["p99_ms"], "latency_at_qps": lat_at["offered_qps"],
                 "sweep": sweep, "batched": batched, "batched_queries_per_s": best_batched,
                 "one_at_a_time": ({"queries_per_s": round(single_qps, 3), "p50_ms": pct(lat1, 50),
@@ -559,8 +581,8 @@ def run_c5(args):
 
 def summary_c3(r):
     """The C3 line reduced to what the headline line carries."""
-    keep = ("value", "unit", "repair_ms_mean", "recompute_ms_mean", "repair_device_ms_mean",
-            "recompute_device_ms_mean", "speedup_vs_recompute")
+    keep = ("value", "unit", "repair_ms_mean", "recompute_ms_mean", "best_recompute", "ms_per_batch_mean",
+            "device_ms_per_batch_mean", "speedup_vs_recompute")
     out = {k: r[k] for k in keep}
     out["config"] = r["config"]
     out["batches"] = len(r["batches"])
@@ -610,6 +632,9 @@ def main():
                     help="offered loads of the open-loop served points (default: 25/50/75/90%% of the "
                          "closed-loop capacity)")
     ap.add_argument("--c5-sweep-requests", type=int, default=256, help="requests per open-loop point")
+    ap.add_argument("--c5-batch-only", action="store_true", help="c5: the batched launches only (for profiling)")
+    ap.add_argument("--c5-engine", default="", help="c5: batch engine to run (piles or bidi; default both)")
+    ap.add_argument("--c5-serve-engine", type=int, default=0, help="c5: serving engine (0 = library default)")
     ap.add_argument("--no-extra", action="store_true",
                     help="c4 only: skip the C3 and C5 legs that the default line carries as extra keys")
     args = ap.parse_args()

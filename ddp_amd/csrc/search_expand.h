@@ -83,10 +83,17 @@ constexpr int expand_unroll()
 // Without FINAL_SYNC the function returns without its closing barrier; the
 // caller must pass a barrier before the next expand_rows (or before reading
 // what the pushes wrote).
+//
+// Fb: an upper bound the caller no longer needs to beat (the target's f when
+// it only wants the target's cost).  Where the heuristic needs no load (zero,
+// Manhattan) an edge whose candidate f is already >= Fb is skipped before its
+// (f, parent) word is read: it cannot lead to a cheaper path to t, and the
+// word's random 8-byte read costs a whole cache line.  Callers that keep
+// searching for a later, larger bound (a resumed DA* search) pass UINT32_MAX.
 template <int B, int HK, int P, class Push, bool FINAL_SYNC = true>
 __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
                                             const int32_t (&u)[P], const RowSpan (&rs)[P], ExpandLds<B * P> &L,
-                                            ExpandCounters &C, Push &push)
+                                            ExpandCounters &C, Push &push, uint32_t Fb = 0xFFFFFFFFu)
 {
     constexpr int R = expand_unroll<B>();
     constexpr int V = B * P;
@@ -189,6 +196,9 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (G.dver && dv[r] <= G.ver) c[r] = -1;  // deleted at or before this version
+            if constexpr (HK == DDP_H_ZERO || HK == DDP_H_MANHATTAN)
+                if (c[r] >= 0 && L.gu[jj[r]] + wt[r] + (uint32_t)heur_from<HK>(H, c[r], HeurIn{0, 0}) >= Fb)
+                    c[r] = -1;  // cannot beat the bound: no word read
             if (c[r] >= 0) {  // negative id = deleted edge
                 hin[r] = heur_load<HK>(H, c[r]);
                 cur[r] = ld_gp(gp + c[r]);

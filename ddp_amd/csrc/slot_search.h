@@ -657,7 +657,9 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
             S.phase_n += n;
         }
         PilePush<B, B * W, RESUME> push{&P, far, &S, touched, Z};
-        expand_rows<B, HK, W, PilePush<B, B * W, RESUME>, false>(G, hv, gp, t, u, rs, S.L, C, push);
+        // a resumed search keeps every relaxation: a later batch may raise F(t)
+        expand_rows<B, HK, W, PilePush<B, B * W, RESUME>, false>(G, hv, gp, t, u, rs, S.L, C, push,
+                                                                  RESUME ? 0xFFFFFFFFu : F);
         if (push.err) atomicOr(&S.err, push.err);
         __syncthreads();  // the round's pushes and errors are visible to the next round
         if (S.err) break;
