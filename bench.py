@@ -436,37 +436,34 @@ def run_c3(args):
         "recompute_grid": (ddp_amd.DAStar.RECOMPUTE, ddp_amd.MODE_AUTO),
         "recompute_piles": (ddp_amd.DAStar.RECOMPUTE, ddp_amd.MODE_PILES),
     }
-    das = {k: ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K, mode=m) for k, (_, m) in sessions.items()}
-    if len({d.cost for d in das.values()}) != 1:
-        raise SystemExit(f"C3 initial costs differ: {[d.cost for d in das.values()]}")
+    # one session at a time (20 batches each), so that one's device state does
+    # not evict another's from the caches
     ms = {k: [] for k in sessions}
     dev = {k: [] for k in sessions}
-    per = []
-    cur = (hg.off, hg.col, hg.w)
-    checked = 0
-    for bi, lines in enumerate(upd):
-        costs = {}
-        for k, (mode, _) in sessions.items():
-            c, st = das[k].apply(lines, mode=mode)
-            costs[k] = c
+    costs = {k: [] for k in sessions}
+    init = set()
+    for k, (mode, m) in sessions.items():
+        da = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=args.c3_K, mode=m)
+        init.add(da.cost)
+        for bi, lines in enumerate(upd):
+            c, st = da.apply(lines, mode=mode)
+            costs[k].append(c)
             ms[k].append(st["total_ms"])
             dev[k].append(st["repair_ms"] + st["search_ms"])
-        if len(set(costs.values())) != 1:
-            raise SystemExit(f"C3 batch {bi}: sessions disagree {costs}")
-        c1 = costs["repair_piles"]
-        per.append({"batch": bi, "cost": c1, **{k: round(v[-1], 3) for k, v in ms.items()}})
+        da.close()
+        log(f"C3 {k}: {float(np.mean(ms[k])):.3f} ms per batch")
+    if len(init) != 1 or any(costs[k] != costs["repair_grid"] for k in sessions):
+        raise SystemExit(f"C3: sessions disagree ({init}, {costs})")
+    per = [{"batch": bi, "cost": costs["repair_grid"][bi], **{k: round(ms[k][bi], 3) for k in sessions}}
+           for bi in range(nb)]
+    cur = (hg.off, hg.col, hg.w)
+    for bi, lines in enumerate(upd):  # the oracle on every 4th version and the last
+        cur = pyoracle.apply_batch(*cur, lines)[:3]
         if bi % max(1, nb // 5) == 0 or bi == nb - 1:
-            cur_full = cur
-            for b2 in upd[checked:bi + 1]:
-                cur_full = pyoracle.apply_batch(*cur_full, b2)[:3]
-            cur, checked = cur_full, bi + 1
             d, _ = pyoracle.dijkstra(*cur, s, t)
             want = int(d[t]) if d[t] >= 0 else ddp_amd.INF_COST
-            if c1 != want:
-                raise SystemExit(f"C3 batch {bi}: GPU {c1} oracle {want}")
-        log(f"C3 batch {bi}: {per[-1]}")
-    for d in das.values():
-        d.close()
+            if costs["repair_grid"][bi] != want:
+                raise SystemExit(f"C3 batch {bi}: GPU {costs['repair_grid'][bi]} oracle {want}")
     mean = {k: float(np.mean(v)) for k, v in ms.items()}
     best_rep = min(("repair_grid", "repair_piles"), key=lambda k: mean[k])
     best_rec = min(("recompute_grid", "recompute_piles"), key=lambda k: mean[k])
