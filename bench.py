@@ -524,10 +524,16 @@ def run_c5(args):
         if cost_b is not None and not np.array_equal(c, cost_b):
             raise SystemExit(f"C5 batch: {name} disagrees with piles")
         cost_b = c
+        # per expansion: 8 B row offsets + the 8-byte (f, parent) word; per scanned
+        # edge: 4 B column + 4 B weight + the target's word; per improvement: the word
+        abytes = 16 * bst["expansions"] + 16 * bst["edges"] + 8 * bst["improvements"]
         batched[name] = {"queries_per_s": round(len(src) / (bst["kernel_ms"] * 1e-3), 2),
                          "kernel_ms": round(bst["kernel_ms"], 2), "edges": bst["edges"],
                          "expansions": bst["expansions"], "block": bst["block"], "slots": bst["slots"],
-                         "hbm_algorithmic_bytes": 16 * bst["expansions"] + 16 * bst["edges"] + 8 * bst["improvements"]}
+                         "hbm_algorithmic_bytes": abytes,
+                         "roofline": measured_traffic(roofline(abytes, bst["kernel_ms"], "batch_kernel"),
+                                                      workload="c5", block=bst["block"],
+                                                      relabel=not args.no_reorder, queries=len(src), engine=name)}
         log(f"C5 batch {name}: {batched[name]}")
     if args.c5_batch_only:
         ctx.close()
