@@ -601,6 +601,20 @@ def summary_c5(r):
     return out
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs visible"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -650,6 +664,8 @@ def main():
         # the other served/dynamic configurations ride along as extra keys
         res["c3"] = summary_c3(run_c3(args))
         res["c5"] = summary_c5(run_c5(args))
+    if isinstance(res.get("cpu_baseline"), dict):
+        res["cpu_baseline"]["cpu"] = cpu_model()
     print(json.dumps(res), flush=True)
 
 

@@ -68,6 +68,108 @@ constexpr int expand_unroll()
     return B == 512 ? 2 : 4;
 }
 
+// Stages 2-4 of one window of R edges per lane: edge r leads to c[r] (< 0:
+// none) with weight wt[r] from the node par(r), whose g is gu(r).  Loads are
+// issued stage by stage (heuristic inputs and (f, parent) words, then one
+// atomic per improving edge), and improvements go to the push policy.  Fb
+// as in expand_rows.
+template <int R, int HK, class GuOf, class ParOf, class Push>
+__device__ __forceinline__ void relax_window(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
+                                             int32_t (&c)[R], const uint32_t (&wt)[R], GuOf gu_of, ParOf par_of,
+                                             uint32_t Fb, ExpandCounters &C, Push &push)
+{
+    // stage 2: heuristic inputs and the current (f, parent) words, all
+    // loads issued before the first heuristic is computed.  hin, cur and
+    // got below are only written and read by lanes whose edge takes part:
+    // giving the others a value would make the compiler merge the two and
+    // wait for each load (or CAS) before issuing the next.
+    HeurIn hin[R];
+    uint64_t cur[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if constexpr (HK == DDP_H_ZERO || HK == DDP_H_MANHATTAN)
+            if (c[r] >= 0 && gu_of(r) + wt[r] + (uint32_t)heur_from<HK>(H, c[r], HeurIn{0, 0}) >= Fb)
+                c[r] = -1;  // cannot beat the bound: no word read
+        if (c[r] >= 0) {  // negative id = deleted edge
+            hin[r] = heur_load<HK>(H, c[r]);
+            cur[r] = ld_gp(gp + c[r]);
+        }
+    }
+    int32_t hc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) hc[r] = c[r] >= 0 ? heur_from<HK>(H, c[r], hin[r]) : 0;
+    // stage 3: one CAS attempt per improving edge, all issued together
+    uint64_t want[R], got[R];
+    bool tried[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        tried[r] = false;
+        want[r] = 0;
+        if (c[r] >= 0 && (HK != DDP_H_ARRAY || hc[r] >= 0)) {  // h = -1: cannot reach t
+            const uint32_t cand = gu_of(r) + wt[r] + (uint32_t)hc[r];
+            want[r] = ((uint64_t)cand << 32) | (uint32_t)par_of(r);
+            tried[r] = (uint32_t)(cur[r] >> 32) > cand;
+        }
+    }
+    // Every load above has returned by now (the comparisons used them);
+    // saying so keeps the compiler from waiting again before each CAS.
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    const bool by_min = G.relax_min != 0;  // uniform
+    if (by_min) {
+        // one atomicMin per improving edge: the smallest (f, parent) of
+        // the round wins with no retry round trips
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (tried[r]) got[r] = (uint64_t)atomicMin((unsigned long long *)(gp + c[r]),
+                                                       (unsigned long long)want[r]);
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (tried[r])
+                got[r] = (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
+                                             (unsigned long long)want[r]);
+    }
+    // stage 4: settle contended CASes, then hand improvements to the push policy
+    bool has[R], first[R];
+    uint64_t key[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        has[r] = false;
+        first[r] = false;
+        key[r] = 0;
+        if (tried[r]) {
+            const uint32_t cand = (uint32_t)(want[r] >> 32);
+            bool ok = by_min ? (uint32_t)(got[r] >> 32) > cand : got[r] == cur[r];
+            uint64_t prev = by_min ? got[r] : cur[r];
+            if (!ok && !by_min) {
+                uint64_t now = got[r];
+                while ((uint32_t)(now >> 32) > cand) {
+                    const uint64_t old = atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)now,
+                                                   (unsigned long long)want[r]);
+                    if (old == now) {
+                        ok = true;
+                        prev = now;
+                        break;
+                    }
+                    now = old;
+                }
+            }
+            if (ok) {
+                C.improved++;
+                first[r] = (uint32_t)(prev >> 32) == DDP_INF_F;
+                if (c[r] == t) {
+                    push.target(cand);
+                } else {
+                    has[r] = true;
+                    key[r] = ((uint64_t)cand << 32) | (uint32_t)c[r];
+                }
+            }
+        }
+    }
+    push.template touch_all<R>(first, c);
+    push.template all<R>(has, key);
+}
+
 // Push policy interface (block-collective members: every lane calls them):
 //   void target(uint32_t f)                    -- t improved to f (t is never queued)
 //   void touch(int32_t c, uint64_t prev)       -- c's word was prev before the update
@@ -186,97 +288,12 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
                 if (G.dver) dv[r] = G.dver[pos[r]];
             }
         }
-        // stage 2: heuristic inputs and the current (f, parent) words, all
-        // loads issued before the first heuristic is computed.  hin, cur and
-        // got below are only written and read by lanes whose edge takes part:
-        // giving the others a value would make the compiler merge the two and
-        // wait for each load (or CAS) before issuing the next.
-        HeurIn hin[R];
-        uint64_t cur[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
+        for (int r = 0; r < R; ++r)
             if (G.dver && dv[r] <= G.ver) c[r] = -1;  // deleted at or before this version
-            if constexpr (HK == DDP_H_ZERO || HK == DDP_H_MANHATTAN)
-                if (c[r] >= 0 && L.gu[jj[r]] + wt[r] + (uint32_t)heur_from<HK>(H, c[r], HeurIn{0, 0}) >= Fb)
-                    c[r] = -1;  // cannot beat the bound: no word read
-            if (c[r] >= 0) {  // negative id = deleted edge
-                hin[r] = heur_load<HK>(H, c[r]);
-                cur[r] = ld_gp(gp + c[r]);
-            }
-        }
-        int32_t hc[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) hc[r] = c[r] >= 0 ? heur_from<HK>(H, c[r], hin[r]) : 0;
-        // stage 3: one CAS attempt per improving edge, all issued together
-        uint64_t want[R], got[R];
-        bool tried[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            tried[r] = false;
-            want[r] = 0;
-            if (c[r] >= 0 && (HK != DDP_H_ARRAY || hc[r] >= 0)) {  // h = -1: cannot reach t
-                const uint32_t cand = L.gu[jj[r]] + wt[r] + (uint32_t)hc[r];
-                want[r] = ((uint64_t)cand << 32) | (uint32_t)L.node[jj[r]];
-                tried[r] = (uint32_t)(cur[r] >> 32) > cand;
-            }
-        }
-        // Every load above has returned by now (the comparisons used them);
-        // saying so keeps the compiler from waiting again before each CAS.
-        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-        const bool by_min = G.relax_min != 0;  // uniform
-        if (by_min) {
-            // one atomicMin per improving edge: the smallest (f, parent) of
-            // the round wins with no retry round trips
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (tried[r]) got[r] = (uint64_t)atomicMin((unsigned long long *)(gp + c[r]),
-                                                           (unsigned long long)want[r]);
-        } else {
-#pragma unroll
-            for (int r = 0; r < R; ++r)
-                if (tried[r])
-                    got[r] = (uint64_t)atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)cur[r],
-                                                 (unsigned long long)want[r]);
-        }
-        // stage 4: settle contended CASes, then hand improvements to the push policy
-        bool has[R], first[R];
-        uint64_t key[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            has[r] = false;
-            first[r] = false;
-            key[r] = 0;
-            if (tried[r]) {
-                const uint32_t cand = (uint32_t)(want[r] >> 32);
-                bool ok = by_min ? (uint32_t)(got[r] >> 32) > cand : got[r] == cur[r];
-                uint64_t prev = by_min ? got[r] : cur[r];
-                if (!ok && !by_min) {
-                    uint64_t now = got[r];
-                    while ((uint32_t)(now >> 32) > cand) {
-                        const uint64_t old = atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)now,
-                                                       (unsigned long long)want[r]);
-                        if (old == now) {
-                            ok = true;
-                            prev = now;
-                            break;
-                        }
-                        now = old;
-                    }
-                }
-                if (ok) {
-                    C.improved++;
-                    first[r] = (uint32_t)(prev >> 32) == DDP_INF_F;
-                    if (c[r] == t) {
-                        push.target(cand);
-                    } else {
-                        has[r] = true;
-                        key[r] = ((uint64_t)cand << 32) | (uint32_t)c[r];
-                    }
-                }
-            }
-        }
-        push.template touch_all<R>(first, c);
-        push.template all<R>(has, key);
+        relax_window<R, HK>(
+            G, H, gp, t, c, wt, [&](int r) { return L.gu[jj[r]]; }, [&](int r) { return L.node[jj[r]]; }, Fb, C,
+            push);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
