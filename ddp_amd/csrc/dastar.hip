@@ -356,16 +356,61 @@ __global__ void relax_inserts(const int32_t *ins, int32_t n, const int32_t *h, u
 // batches, which may raise F(t) again.
 constexpr int kPileBlock = 1024;
 
+// open_split counters
+enum { SP_LIVE = 0, SP_KEEP = 1, SP_OVF = 2, SP_WORDS = 4 };
+
+// Ahead of the one-workgroup search, the whole GPU splits the retained open
+// set and this batch's repair pushes around F(t): current entries below it
+// (live) seed the search, current entries at or above it stay dormant
+// (keep), and stale ones -- the node's word has moved on, so an entry with
+// its current f exists elsewhere -- are dropped.  The search then scans only
+// the live entries, where it used to scan the whole retained set (1.3M+
+// entries on the 1M-node graph of 8 out-edges) on its one CU every batch.
+__global__ void open_split(const uint64_t *open, uint32_t n_open, const uint64_t *push, const uint32_t *push_n,
+                           uint32_t pcap, const uint64_t *gp, int32_t t, uint64_t *live, uint64_t *keep, uint32_t cap,
+                           uint32_t *cnt)
+{
+    const uint32_t np0 = __hip_atomic_load(push_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t np = np0 < pcap ? np0 : pcap;
+    if (np0 > pcap && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(cnt + SP_OVF, 1u);
+    const uint64_t n = (uint64_t)n_open + np;
+    const uint32_t F0 = (uint32_t)(ld_gp(gp + t) >> 32);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // uniform trips per block: the appends are wave-wide
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const uint64_t i = i0 + threadIdx.x;
+        bool sel[2] = {false, false};
+        uint64_t key = 0;
+        if (i < n) {
+            key = i < n_open ? open[i] : push[i - n_open];
+            const int32_t v = (int32_t)(uint32_t)key;
+            const uint32_t f = (uint32_t)(key >> 32);
+            if (v != t && (uint32_t)(ld_gp(gp + v) >> 32) == f) {
+                sel[0] = f < F0;
+                sel[1] = !sel[0];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const bool s1[1] = {sel[k]};
+            uint32_t pos[1];
+            wave_append<1>(s1, cnt + k, pos);
+            if (sel[k]) {
+                if (pos[0] < cap) (k ? keep : live)[pos[0]] = key;
+                else atomicOr(cnt + SP_OVF, 1u);
+            }
+        }
+    }
+}
+
 struct DaPileArgs {
     GraphView g;
     HeurView h;
     uint64_t *gp;
-    uint64_t *open, *alt, *dorm;  // open set in, far-pile partner, dormant pile out (cap entries each)
+    uint64_t *live, *alt, *dorm;  // live entries in, far-pile partner, dormant pile (kept entries first)
     uint64_t *near;               // ncap entries
-    uint32_t cap, ncap, n_open, lring, delta0;
-    const uint64_t *push;         // repair pushes of this batch
-    const uint32_t *push_n;
-    uint32_t pcap;
+    uint32_t cap, ncap, lring, delta0;
+    const uint32_t *split;        // open_split counters
     int32_t N, t, maxit;
     uint32_t *res;                // iterations, F(t), error flags, dormant entries
     unsigned long long *stats;
@@ -376,16 +421,12 @@ __global__ void __launch_bounds__(kPileBlock) dastar_pile_kernel(DaPileArgs a)
     extern __shared__ uint64_t s_ring[];
     __shared__ SlotLds<kPileBlock, kPileBlock> S;
     const int lane = threadIdx.x;
-    // the repair pushes join the open set at the end of its buffer
-    const uint32_t np0 = __hip_atomic_load(a.push_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t np = np0 < a.pcap ? np0 : a.pcap;
-    const bool fits = (uint64_t)a.n_open + np <= a.cap;
-    if (fits)
-        for (uint32_t i = lane; i < np; i += kPileBlock) a.open[a.n_open + i] = a.push[i];
-    block_sync_global();
+    const uint32_t n_live = __hip_atomic_load(a.split + SP_LIVE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t n_keep = __hip_atomic_load(a.split + SP_KEEP, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const bool fits = __hip_atomic_load(a.split + SP_OVF, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     const uint32_t F0 = (uint32_t)(ld_gp(a.gp + a.t) >> 32);
-    const PilePtrs P{a.near, a.open, a.alt, a.ncap, a.cap, s_ring, a.lring};
-    const PileResume Z{a.dorm, a.cap, fits ? a.n_open + np : 0u, F0};
+    const PilePtrs P{a.near, a.live, a.alt, a.ncap, a.cap, s_ring, a.lring};
+    const PileResume Z{a.dorm, a.cap, fits ? n_live : 0u, F0, fits ? n_keep : 0u};
     ExpandCounters C;
     uint32_t stale = 0;
     slot_search_piles<kPileBlock, DDP_H_ARRAY, 1, true>(a.g, a.h, a.gp, nullptr, P, a.N, -1, a.t, a.maxit, a.delta0, S,
@@ -393,7 +434,7 @@ __global__ void __launch_bounds__(kPileBlock) dastar_pile_kernel(DaPileArgs a)
     if (lane == 0) {
         a.res[0] = (uint32_t)S.iters;
         a.res[1] = S.F;
-        a.res[2] = S.err | (fits && np0 <= a.pcap ? 0u : (uint32_t)QERR_HEAP);
+        a.res[2] = S.err | (fits ? 0u : (uint32_t)QERR_HEAP);
         a.res[3] = S.dorm_n;
     }
     slot_flush_stats<kPileBlock>(a.stats, C, stale);
@@ -455,6 +496,7 @@ struct ddp_dastar {
     uint64_t *pbuf[3] = {nullptr, nullptr, nullptr};
     uint64_t *pnear = nullptr;
     uint32_t *pres = nullptr;
+    uint32_t *d_split = nullptr;  // open_split counters
     uint32_t pcap_open = 0, pncap = 0;
     int32_t open_at = 0;   // which buffer holds the open set
     uint32_t open_n = 0;   // its entries
@@ -476,7 +518,7 @@ static void dastar_free(ddp_dastar *d)
     log_free(d->L);
     void *ptrs[] = {d->L.fhead, d->L.rhead, d->L.ftail, d->L.rtail, d->d_reid, d->d_h,
                     d->d_mark, d->d_front, d->d_marked, d->d_rctrl, d->d_work,
-                    d->pbuf[0], d->pbuf[1], d->pbuf[2], d->pnear, d->pres};
+                    d->pbuf[0], d->pbuf[1], d->pbuf[2], d->pnear, d->pres, d->d_split};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (d->g) ddp_graph_destroy(d->g);
@@ -621,18 +663,24 @@ static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
     a.g.relax_min = d->g->minw >= 1;
     a.h = d->hv;
     a.gp = d->S.gp;
-    a.open = d->pbuf[d->open_at];
-    a.alt = d->pbuf[(d->open_at + 1) % 3];
+    // the open set (pbuf[open_at]) is split into live (the next buffer) and
+    // kept entries (the one after, which the search's dormant pile extends);
+    // the open set's own buffer is free then and serves as the far-pile partner
+    uint64_t *open = d->pbuf[d->open_at];
+    a.live = d->pbuf[(d->open_at + 1) % 3];
+    a.alt = open;
     a.dorm = d->pbuf[(d->open_at + 2) % 3];
     a.near = d->pnear;
     a.cap = d->pcap_open;
     a.ncap = d->pncap;
-    a.n_open = d->open_n;
     a.lring = 8u * kPileBlock;
     a.delta0 = std::max<uint32_t>(1, d->g->meanw);
-    a.push = d->S.pbuf + d->P.pcap;
-    a.push_n = d->S.ctrl + 2 * GA_SLOT_WORDS + GA_S_PUSHCNT;
-    a.pcap = (uint32_t)d->P.pcap;
+    a.split = d->d_split;
+    hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)d->d_split, (int64_t)SP_WORDS, 0);
+    hipLaunchKernelGGL(open_split, dim3(1024), dim3(256), 0, ctx->stream, (const uint64_t *)open, d->open_n,
+                       (const uint64_t *)(d->S.pbuf + d->P.pcap),
+                       (const uint32_t *)(d->S.ctrl + 2 * GA_SLOT_WORDS + GA_S_PUSHCNT), (uint32_t)d->P.pcap,
+                       (const uint64_t *)d->S.gp, d->t, a.live, a.dorm, d->pcap_open, d->d_split);
     a.N = d->N;
     a.t = d->t;
     a.maxit = d->P.maxit;
@@ -802,6 +850,7 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
         for (int k = 0; k < 3 && e2 == hipSuccess; ++k) e2 = hipMalloc((void **)&d->pbuf[k], (size_t)d->pcap_open * 8);
         if (e2 == hipSuccess) e2 = hipMalloc((void **)&d->pnear, (size_t)nc * 8);
         if (e2 == hipSuccess) e2 = hipMalloc((void **)&d->pres, 64);
+        if (e2 == hipSuccess) e2 = hipMalloc((void **)&d->d_split, SP_WORDS * 4);
         if (e2 != hipSuccess) {
             dastar_free(d);
             delete d;

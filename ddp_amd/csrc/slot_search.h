@@ -275,13 +275,15 @@ __device__ __forceinline__ bool pile_place(const PilePtrs &P, uint32_t thr, bool
 // A resumed search (DA*): the open set arrives as the first n_in entries of
 // the far pile, the target's current f is F0, and entries that cannot beat
 // F(t) now are kept in a dormant pile instead of being dropped -- a later
-// batch may raise F(t) again.  (f, parent) words persist between searches,
-// so nothing is logged for a reset.
+// batch may raise F(t) again.  The dormant pile may already hold dorm0
+// entries (kept from before); new ones go behind them.  (f, parent) words
+// persist between searches, so nothing is logged for a reset.
 struct PileResume {
     uint64_t *dorm;  // dcap entries
     uint32_t dcap;
     uint32_t n_in;
     uint32_t F0;
+    uint32_t dorm0;
 };
 
 // Wave-collective: appends the selected keys to the dormant pile.
@@ -429,6 +431,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
         if (lane == 0) {
             S.F = Z->F0;
             S.far_n = Z->n_in;  // the open set; the first refill sets the window
+            S.dorm_n = Z->dorm0;
         }
     } else if (run) {
         set_target<HK>(hv, t);
