@@ -61,7 +61,7 @@ float ms_since(clk::time_point t0) { return std::chrono::duration<float, std::mi
 
 enum : int { RC_NEXT = 0, RC_MARKED = 1, RC_ROOTS = 2, RC_REPULLED = 3, RC_PUSHES = 4, RC_WORDS = 16 };
 
-enum : int { RC_DEL_BASE = 5, RC_DEL_LOG = 6, RC_WALKED = 7 };
+enum : int { RC_DEL_BASE = 5, RC_DEL_LOG = 6, RC_WALKED = 7, RC_WALK_ERR = 8 };
 
 // The log of inserted edges: entry x is u -> v with weight w, live while
 // del[x] == INT32_MAX.  head/next links pack (entry << 32 | 0); the end of a
@@ -77,46 +77,96 @@ struct InsLog {
 
 __device__ __forceinline__ bool link_end(uint64_t link) { return (uint32_t)link != 0u; }
 
+// Appends the children claimed in one group to the next level (nxt[k - base]
+// for draw k of *next_cnt) and to the marked list: wave-collective (every
+// lane of the wave calls it), one counter atomic per wave and list.  Per-node
+// atomics on two shared counters serialised in L2 and made the walk of a C3
+// batch (12,500 invalidated nodes) take about 340 us.  Both lists hold at
+// most cap (N) entries, as each node is claimed once per epoch; a store past
+// that is dropped rather than written out of bounds.
+template <int K>
+__device__ __forceinline__ void append_claims(const bool (&sel)[K], const int32_t (&y)[K], uint32_t *next_cnt,
+                                              uint32_t base, int32_t *nxt, uint32_t *marked_cnt, int32_t *marked,
+                                              uint32_t cap)
+{
+    uint32_t pos[K];
+    wave_append<K>(sel, next_cnt, pos);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (sel[k] && pos[k] - base < cap) nxt[pos[k] - base] = y[k];
+    wave_append<K>(sel, marked_cnt, pos);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (sel[k] && pos[k] < cap) marked[pos[k]] = y[k];
+}
+
 // Deletes, one thread per line: the first live instance of u -> v, base row
 // first, then u's inserted edges oldest first (DA_star/GA_merge.cu:1019-1037
 // on the merged row).  Concurrent deletes of the same pair CAS distinct
 // instances, so k deletes remove the first k live ones, as a sequential pass
 // would.  A deleted edge that was v's parent edge makes v a root of the
-// subtree to invalidate.
+// subtree to invalidate.  Every lane reaches the counting at the end, where
+// each wave adds its counts with one atomic per counter.
 __global__ void apply_deletes(const int32_t *pairs, int32_t n, GraphView F, int32_t *col, int32_t *rcol,
                               const int32_t *roff, const int32_t *reid, InsLog L, const uint64_t *gp, int32_t *mark,
                               int32_t epoch, int32_t *front, int32_t *marked, uint32_t *rctrl)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int32_t u = pairs[2 * i], v = pairs[2 * i + 1];
-    bool found = false;
-    for (int32_t j = F.off[u]; j < F.off[u + 1] && !found; ++j) {
-        if (__hip_atomic_load(col + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != v) continue;
-        if (atomicCAS(col + j, v, -1) != v) continue;
-        found = true;
-        for (int32_t r = roff[v]; r < roff[v + 1]; ++r)
-            if (reid[r] == j) {
-                rcol[r] = -1;
-                break;
+    int32_t u = -1, v = -1;
+    bool in_base = false, in_log = false;
+    if (i < n) {
+        u = pairs[2 * i];
+        v = pairs[2 * i + 1];
+        // the row in groups of 8 loads issued together; a CAS only where the
+        // column matches, in row order (the first live instance wins)
+        constexpr int kG = 8;
+        const int32_t rb = F.off[u], re = F.off[u + 1];
+        int32_t hit = -1;
+        for (int32_t j0 = rb; j0 < re && hit < 0; j0 += kG) {
+            int32_t c[kG];
+#pragma unroll
+            for (int k = 0; k < kG; ++k)
+                c[k] = j0 + k < re ? __hip_atomic_load(col + j0 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : -1;
+#pragma unroll
+            for (int k = 0; k < kG; ++k)
+                if (hit < 0 && c[k] == v && atomicCAS(col + j0 + k, v, -1) == v) hit = j0 + k;
+        }
+        if (hit >= 0) {
+            in_base = true;
+            const int32_t qb = roff[v], qe = roff[v + 1];
+            bool done = false;
+            for (int32_t r0 = qb; r0 < qe && !done; r0 += kG) {
+                int32_t id[kG];
+#pragma unroll
+                for (int k = 0; k < kG; ++k) id[k] = r0 + k < qe ? reid[r0 + k] : -1;
+#pragma unroll
+                for (int k = 0; k < kG; ++k)
+                    if (!done && id[k] == hit) {
+                        rcol[r0 + k] = -1;
+                        done = true;
+                    }
             }
-        atomicAdd(rctrl + RC_DEL_BASE, 1u);
-    }
-    for (uint64_t link = L.fhead[u]; !found && !link_end(link);) {
-        const int32_t x = (int32_t)(link >> 32);
-        link = L.fnext[x];
-        if (L.v[x] == v && atomicCAS(L.del + x, INT32_MAX, 0) == INT32_MAX) {
-            found = true;
-            atomicAdd(rctrl + RC_DEL_LOG, 1u);
+        }
+        for (uint64_t link = L.fhead[u]; !in_base && !in_log && !link_end(link);) {
+            const int32_t x = (int32_t)(link >> 32);
+            link = L.fnext[x];
+            if (L.v[x] == v && atomicCAS(L.del + x, INT32_MAX, 0) == INT32_MAX) in_log = true;
         }
     }
-    if (!found) return;
-    const uint64_t word = ld_gp(gp + v);
-    if ((uint32_t)(word >> 32) == DDP_INF_F || (uint32_t)word != (uint32_t)u) return;
-    if (atomicMax(mark + v, epoch) == epoch) return;
-    front[atomicAdd(rctrl + RC_NEXT, 1u)] = v;
-    marked[atomicAdd(rctrl + RC_MARKED, 1u)] = v;
-    atomicAdd(rctrl + RC_ROOTS, 1u);
+    bool root[1] = {false};
+    const int32_t rv[1] = {v};
+    if (in_base || in_log) {
+        const uint64_t word = ld_gp(gp + v);
+        root[0] = (uint32_t)(word >> 32) != DDP_INF_F && (uint32_t)word == (uint32_t)u &&
+                  atomicMax(mark + v, epoch) != epoch;
+    }
+    const uint64_t mb = __ballot(in_base), ml = __ballot(in_log), mr = __ballot(root[0]);
+    if ((threadIdx.x & 63) == 0) {
+        if (mb) atomicAdd(rctrl + RC_DEL_BASE, (uint32_t)__popcll(mb));
+        if (ml) atomicAdd(rctrl + RC_DEL_LOG, (uint32_t)__popcll(ml));
+        if (mr) atomicAdd(rctrl + RC_ROOTS, (uint32_t)__popcll(mr));
+    }
+    append_claims<1>(root, rv, rctrl + RC_NEXT, 0u, front, rctrl + RC_MARKED, marked, (uint32_t)F.N);
 }
 
 // New log entries: one thread per group of equal keys (entries arrive sorted
@@ -163,101 +213,166 @@ __global__ void fill_u64(uint64_t *p, int64_t n, uint64_t v)
         p[i] = v;
 }
 
-// y is a child of a frontier node: marks it once per epoch and lists it;
-// nxt[k - base] receives the child that drew k from rctrl[RC_NEXT].
-__device__ __forceinline__ void claim(int32_t y, int32_t *mark, int32_t epoch, int32_t *nxt, uint32_t base,
-                                      int32_t *marked, uint32_t *rctrl)
-{
-    if (atomicMax(mark + y, epoch) == epoch) return;
-    nxt[atomicAdd(rctrl + RC_NEXT, 1u) - base] = y;
-    marked[atomicAdd(rctrl + RC_MARKED, 1u)] = y;
-}
+// Where the children claimed in a level go (wave-collective append<K>):
+// GlobalSink lists them in nxt[k - base] for draw k of a device counter (the
+// wide levels, one launch over the grid each); WalkSink keeps a level in LDS
+// and sends what does not fit to the device buffer `out` (the narrow levels
+// of subtree_walk).  Both add each claimed node to the marked list.
+struct GlobalSink {
+    uint32_t *next_cnt, *marked_cnt;
+    uint32_t base;
+    int32_t *nxt, *marked;
+    uint32_t cap;  // entries of nxt and marked (N)
+    template <int K>
+    __device__ void append(const bool (&sel)[K], const int32_t (&y)[K])
+    {
+        append_claims<K>(sel, y, next_cnt, base, nxt, marked_cnt, marked, cap);
+    }
+};
 
-__device__ __forceinline__ void claim_child(int32_t y, int32_t x, const uint64_t *gp, int32_t *mark, int32_t epoch,
-                                            int32_t *nxt, uint32_t base, int32_t *marked, uint32_t *rctrl)
-{
-    if ((uint32_t)ld_gp(gp + y) == (uint32_t)x) claim(y, mark, epoch, nxt, base, marked, rctrl);
-}
-
-// One node of a level of the parent-subtree walk: the children y of frontier
-// node x are its live out-neighbours whose parent is x.
+// One node x of a level of the parent-subtree walk (x < 0: none): its
+// children are its live out-neighbours y whose parent is x.  A node has one
+// parent and every root is marked before the walk, so a child is reached
+// from one lane only (its parent's) once per epoch: reading its mark with
+// its word and storing the epoch claims it without an atomic round trip.
+// Wave-collective: the loop runs while any lane of the wave has work.
+template <class Sink>
 __device__ __forceinline__ void claim_children(const GraphView &G, const InsLog &L, int32_t x, const uint64_t *gp,
-                                               int32_t *mark, int32_t epoch, int32_t *nxt, uint32_t base,
-                                               int32_t *marked, uint32_t *rctrl)
+                                               int32_t *mark, int32_t epoch, Sink &sink)
 {
-    // the row in groups of 8: all column loads, then all parent-word loads,
-    // then the claims (one round trip per stage instead of two per edge)
+    // Each step takes a group of 8 row entries and one entry of the inserted
+    // list together: their columns and list fields in one round trip, then
+    // all parent words and marks in another, then the claims.  A node with at
+    // most 8 edges and one inserted edge costs one step (the row and list
+    // head come in the round trip before).
     constexpr int kG = 8;
-    const int32_t rb = G.off[x], re = G.off[x + 1];
-    for (int32_t j0 = rb; j0 < re; j0 += kG) {
-        int32_t y[kG];
-        uint32_t par[kG];
+    int32_t rb = 0, re = 0;
+    uint64_t link = 1ull;  // a non-zero low word ends a list
+    if (x >= 0) {
+        rb = G.off[x];
+        re = G.off[x + 1];
+        link = L.fhead[x];
+    }
+    for (int32_t j0 = rb; __any(j0 < re || !link_end(link)); j0 += kG) {
+        int32_t y[kG + 1], mk[kG + 1];
+        uint32_t par[kG + 1];
+        bool sel[kG + 1];
 #pragma unroll
         for (int k = 0; k < kG; ++k) y[k] = j0 + k < re ? G.col[j0 + k] : -1;
+        y[kG] = -1;
+        if (!link_end(link)) {
+            const int32_t e = (int32_t)(link >> 32);
+            link = L.fnext[e];
+            const int32_t v = L.v[e];
+            if (L.del[e] == INT32_MAX) y[kG] = v;
+        }
 #pragma unroll
-        for (int k = 0; k < kG; ++k) par[k] = y[k] >= 0 ? (uint32_t)ld_gp(gp + y[k]) : 0xFFFFFFFFu;
+        for (int k = 0; k <= kG; ++k) {
+            par[k] = 0xFFFFFFFFu;
+            mk[k] = epoch;
+            if (y[k] >= 0) {
+                par[k] = (uint32_t)ld_gp(gp + y[k]);
+                mk[k] = __hip_atomic_load(mark + y[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        // Parallel edges (or a base edge and an inserted one) may name the
+        // same child twice in one step: it is claimed at its first place
+        // only.  Across steps the lane's own mark store precedes its next
+        // loads of that address, so a later step sees the mark.
 #pragma unroll
-        for (int k = 0; k < kG; ++k)
-            if (y[k] >= 0 && par[k] == (uint32_t)x) claim(y[k], mark, epoch, nxt, base, marked, rctrl);
-    }
-    for (uint64_t link = L.fhead[x]; !link_end(link);) {
-        const int32_t e = (int32_t)(link >> 32);
-        link = L.fnext[e];
-        if (L.del[e] == INT32_MAX) claim_child(L.v[e], x, gp, mark, epoch, nxt, base, marked, rctrl);
+        for (int k = 0; k <= kG; ++k) {
+            sel[k] = y[k] >= 0 && par[k] == (uint32_t)x && mk[k] != epoch;
+#pragma unroll
+            for (int q = 0; q < k; ++q) sel[k] = sel[k] && y[q] != y[k];
+            if (sel[k]) mark[y[k]] = epoch;
+        }
+        sink.template append<kG + 1>(sel, y);
     }
 }
 
-__global__ void subtree_level(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch,
-                              const int32_t *cur, int32_t ncur, int32_t *nxt, int32_t *marked, uint32_t *rctrl)
+// Device-wide barrier of the subtree walk over nblocks co-resident
+// workgroups (bar[0] arrivals, bar[1] generation, bar[2] published count):
+// the last workgroup to arrive reads *cnt -- every workgroup's draws of the
+// level are in by then -- and publishes it with the new generation, so each
+// workgroup learns the level's total without racing the next level's draws.
+// Waits are bounded by limit_ticks of the 100 MHz clock (false on timeout).
+__device__ __forceinline__ bool walk_barrier(uint32_t *bar, uint32_t nblocks, uint32_t &gen, const uint32_t *cnt,
+                                             uint64_t limit_ticks, int32_t *s_flag, uint32_t *s_total)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < ncur) claim_children(G, L, cur[i], gp, mark, epoch, nxt, 0u, marked, rctrl);
-}
-
-// The narrow levels of the subtree walk in one workgroup, so that they cost
-// no launch and host round trip each.  Starts from the rctrl[RC_NEXT] nodes in
-// cur and runs level after level while a level holds at most `wide` nodes.
-// rctrl[RC_NEXT] keeps counting across levels (a level's children take the
-// draws from its base on), so no reset has to be ordered against the next
-// level's draws, and the barriers order everything else: one workgroup, no
-// device-wide fence.  On exit rctrl[RC_NEXT] is the size of the level left to
-// walk (0 when done), and rctrl[RC_WALKED] the number of levels walked: when it
-// is odd, that level sits in the buffer passed as nxt.  Every node is marked at
-// most once per epoch, so the loop ends after at most N levels and the count
-// stays below 2N.
-constexpr int kWalkBlock = 1024;
-
-__global__ void __launch_bounds__(kWalkBlock)
-    subtree_walk(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch, int32_t *cur,
-                 int32_t *nxt, int32_t *marked, uint32_t *rctrl, int32_t wide)
-{
-    __shared__ uint32_t s_n;
-    if (threadIdx.x == 0) s_n = __hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    block_sync_global();  // every wave's draws and stores reach L2 first
+    if (threadIdx.x == 0) {
+        bool ok = true;
+        __threadfence();
+        const uint32_t arrived = atomicAdd(&bar[0], 1u);
+        if (arrived == nblocks - 1) {
+            __threadfence();
+            const uint32_t total = __hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[2], total, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = dev_clock_100mhz();
+            while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                __builtin_amdgcn_s_sleep(1);
+                if (dev_clock_100mhz() - t0 > limit_ticks) {
+                    ok = false;
+                    break;
+                }
+            }
+        }
+        __threadfence();
+        *s_total = __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *s_flag = ok ? 1 : 0;
+    }
+    gen += 1;
     __syncthreads();
-    uint32_t base = s_n;  // draws so far; the current level holds ncur nodes
-    int32_t ncur = (int32_t)base;
-    uint32_t walked = 0;
-    while (ncur > 0 && ncur <= wide) {
-        for (int32_t i = threadIdx.x; i < ncur; i += kWalkBlock)
-            claim_children(G, L, cur[i], gp, mark, epoch, nxt, base, marked, rctrl);
-        // every draw has returned; waiting on the memory counter before the
-        // barrier (a bare s_barrier) lands this wave's nxt stores, so the next
-        // level reads them
-        block_sync_global();
-        if (threadIdx.x == 0) s_n = __hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        const uint32_t total = s_n;
-        ncur = (int32_t)(total - base);
+    return *s_flag != 0;
+}
+
+// The parent-subtree walk in one launch over kWalkGrid co-resident
+// workgroups: level after level, each ended by walk_barrier.  Node i of a
+// level goes to workgroup i % kWalkGrid, so even a level of a few hundred
+// nodes spreads over kWalkGrid CUs: one workgroup walking alone was bound
+// by its CU's gather rate (about 35 ns per node, 42 us for a level of 1,200
+// nodes).  The roots are the rctrl[RC_NEXT] draws apply_deletes made into
+// buf0; later levels alternate between buf1 and buf0 (each at most N nodes:
+// every node is claimed once per epoch).  On exit rctrl[RC_WALKED] holds the
+// number of levels, and RC_WALK_ERR is set if a barrier timed out.
+constexpr int kWalkGrid = 64, kWalkGridBlock = 256;
+
+__global__ void __launch_bounds__(kWalkGridBlock)
+    subtree_walk(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch, int32_t *buf0,
+                 int32_t *buf1, int32_t *marked, uint32_t *rctrl, uint32_t *bar, uint64_t limit_ticks)
+{
+    __shared__ int32_t s_flag;
+    __shared__ uint32_t s_total;
+    const uint32_t N = (uint32_t)G.N;
+    uint32_t gen = 0;
+    int32_t *cur = buf0, *nxt = buf1;
+    uint32_t base = __hip_atomic_load(rctrl + RC_NEXT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t ncur = base < N ? base : N;
+    uint32_t levels = 0;
+    while (ncur > 0) {
+        GlobalSink sink{rctrl + RC_NEXT, rctrl + RC_MARKED, base, nxt, marked, N};
+        const uint32_t per = (ncur + kWalkGrid - 1) / kWalkGrid;  // this workgroup's share, at most
+        for (uint32_t s0 = 0; s0 < per; s0 += kWalkGridBlock) {   // uniform trips: the appends are wave-wide
+            const uint32_t slot = s0 + threadIdx.x;
+            const uint32_t i = slot * kWalkGrid + blockIdx.x;
+            claim_children(G, L, (slot < per && i < ncur) ? cur[i] : -1, gp, mark, epoch, sink);
+        }
+        if (!walk_barrier(bar, gridDim.x, gen, rctrl + RC_NEXT, limit_ticks, &s_flag, &s_total)) {
+            if (threadIdx.x == 0) atomicOr(rctrl + RC_WALK_ERR, 1u);
+            break;
+        }
+        const uint32_t total = s_total;
+        ncur = total - base < N ? total - base : N;
         base = total;
         int32_t *tmp = cur;
         cur = nxt;
         nxt = tmp;
-        ++walked;  // thread 0 writes s_n again only after the next level's first barrier
+        ++levels;
     }
-    if (threadIdx.x == 0) {
-        rctrl[RC_NEXT] = (uint32_t)ncur;
-        rctrl[RC_WALKED] = walked;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) rctrl[RC_WALKED] = levels;
 }
 
 struct RepairPush {
@@ -277,52 +392,100 @@ struct RepairPush {
         if (!mask) return;
         const int lane = threadIdx.x & 63;
         const int leader = __builtin_ctzll(mask);
+        const uint32_t fmin = wave_min_u32(has ? f : 0xFFFFFFFFu);  // one atomicMin per wave
         uint32_t base = 0;
         if (lane == leader) {
             base = atomicAdd(slot + GA_S_PUSHCNT, (uint32_t)__popcll(mask));
             atomicAdd(rctrl + RC_PUSHES, (uint32_t)__popcll(mask));
+            atomicMin(slot + GA_S_PUSHMIN, fmin);
         }
         base = __shfl(base, leader, 64);
         if (has) {
             const uint32_t idx = base + lane_rank(mask);
             if (idx < (uint32_t)pcap) buf[idx] = ((uint64_t)f << 32) | (uint32_t)v;
             else atomicOr(slot + GA_S_ERR, 2u);
-            atomicMin(slot + GA_S_PUSHMIN, f);
         }
     }
 };
 
 // Invalidated node x: best live in-edge from a node outside the invalidated
 // set with a finite value; x's word is overwritten (INF if none).
-__global__ void repull(GraphView R, InsLog L, const int32_t *h, uint64_t *gp, const int32_t *mark, int32_t epoch,
-                       const int32_t *marked, int32_t n, RepairPush P)
+// Invalidated node marked[i] (i < n) takes its best in-edge from a node left
+// valid, as a fresh label; it joins the repair pushes.  Wave-collective.
+__device__ __forceinline__ void repull_one(const GraphView &R, const InsLog &L, const int32_t *h, uint64_t *gp,
+                                           const int32_t *mark, int32_t epoch, const int32_t *marked, int32_t i,
+                                           int32_t n, RepairPush &P)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t x = -1;
     uint32_t best = 0xFFFFFFFFu, bp = 0xFFFFFFFFu;
     if (i < n) {
         x = marked[i];
         const uint32_t hx = (uint32_t)h[x];
-        auto consider = [&](int32_t p, uint32_t w) {
-            if (p < 0 || p == x || mark[p] == epoch) return;
-            const uint32_t fp = (uint32_t)(ld_gp(gp + p) >> 32);
+        auto consider = [&](int32_t p, uint32_t w, int32_t mp, uint64_t word, int32_t hp) {
+            if (p < 0 || p == x || mp == epoch) return;
+            const uint32_t fp = (uint32_t)(word >> 32);
             if (fp == DDP_INF_F) return;
-            const uint32_t cand = (fp - (uint32_t)h[p]) + w + hx;  // unsigned, as GA_v2.cu:137
+            const uint32_t cand = (fp - (uint32_t)hp) + w + hx;  // unsigned, as GA_v2.cu:137
             if (cand < DDP_INF_F && (cand < best || (cand == best && (uint32_t)p < bp))) {
                 best = cand;
                 bp = (uint32_t)p;
             }
         };
-        for (int32_t j = R.off[x]; j < R.off[x + 1]; ++j) consider(R.col[j], R.w[j]);
+        // in-edges in groups of 8: sources and weights, then their marks,
+        // words and heuristics, all issued together
+        constexpr int kG = 8;
+        const int32_t rb = R.off[x], re = R.off[x + 1];
+        for (int32_t j0 = rb; j0 < re; j0 += kG) {
+            int32_t p[kG], mp[kG], hp[kG];
+            uint32_t w[kG];
+            uint64_t word[kG];
+#pragma unroll
+            for (int k = 0; k < kG; ++k) {
+                p[k] = -1;
+                w[k] = 0;
+                if (j0 + k < re) {
+                    p[k] = R.col[j0 + k];
+                    w[k] = R.w[j0 + k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kG; ++k) {
+                mp[k] = epoch;
+                word[k] = DDP_GP_INF;
+                hp[k] = 0;
+                if (p[k] >= 0 && p[k] != x) {
+                    mp[k] = mark[p[k]];
+                    word[k] = ld_gp(gp + p[k]);
+                    hp[k] = h[p[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kG; ++k) consider(p[k], w[k], mp[k], word[k], hp[k]);
+        }
         for (uint64_t link = L.rhead[x]; !link_end(link);) {
             const int32_t e = (int32_t)(link >> 32);
             link = L.rnext[e];
-            if (L.del[e] == INT32_MAX) consider(L.u[e], L.w[e]);
+            if (L.del[e] == INT32_MAX) {
+                const int32_t q = L.u[e];
+                if (q >= 0 && q != x) consider(q, L.w[e], mark[q], ld_gp(gp + q), h[q]);
+            }
         }
         gp[x] = best == 0xFFFFFFFFu ? DDP_GP_INF : (((uint64_t)best << 32) | bp);
-        if (best != 0xFFFFFFFFu) atomicAdd(P.rctrl + RC_REPULLED, 1u);
     }
+    const uint64_t got = __ballot(x >= 0 && best != 0xFFFFFFFFu);
+    if ((threadIdx.x & 63) == 0 && got) atomicAdd(P.rctrl + RC_REPULLED, (uint32_t)__popcll(got));
     P.push(x >= 0 && best != 0xFFFFFFFFu, x, best);
+}
+
+// Grid-stride over the marked list, whose length (rctrl[RC_MARKED], at most
+// N) is read on the device: no host round trip between the walk and this.
+__global__ void repull(GraphView R, InsLog L, const int32_t *h, uint64_t *gp, const int32_t *mark, int32_t epoch,
+                       const int32_t *marked, const uint32_t *rctrl, RepairPush P)
+{
+    const uint32_t n0 = __hip_atomic_load(rctrl + RC_MARKED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t n = (int32_t)(n0 < (uint32_t)R.N ? n0 : (uint32_t)R.N);
+    for (int32_t i0 = blockIdx.x * blockDim.x; i0 < n; i0 += gridDim.x * blockDim.x)  // uniform trips per block
+        repull_one(R, L, h, gp, mark, epoch, marked, i0 + (int32_t)threadIdx.x, n, P);
 }
 
 // Newly inserted u->v (w): relax from u.
@@ -366,39 +529,82 @@ enum { SP_LIVE = 0, SP_KEEP = 1, SP_OVF = 2, SP_WORDS = 4 };
 // its current f exists elsewhere -- are dropped.  The search then scans only
 // the live entries, where it used to scan the whole retained set (1.3M+
 // entries on the 1M-node graph of 8 out-edges) on its one CU every batch.
-__global__ void open_split(const uint64_t *open, uint32_t n_open, const uint64_t *push, const uint32_t *push_n,
-                           uint32_t pcap, const uint64_t *gp, int32_t t, uint64_t *live, uint64_t *keep, uint32_t cap,
-                           uint32_t *cnt)
+// Live entries are always checked; kept ones only with check_all (a random
+// word read each: the caller asks for it when the retained set has grown),
+// since a stale kept entry costs room, not correctness.
+constexpr int kSplitBlock = 256, kSplitPer = 8;  // entries per thread: a workgroup splits 2,048
+
+__global__ void __launch_bounds__(kSplitBlock)
+    open_split(const uint64_t *open, uint32_t n_open, const uint64_t *push, const uint32_t *push_n, uint32_t pcap,
+               const uint64_t *gp, int32_t t, uint64_t *live, uint64_t *keep, uint32_t cap, uint32_t *cnt,
+               int32_t check_all)
 {
+    __shared__ uint32_t s_wc[2][kSplitBlock / 64], s_base[2];
     const uint32_t np0 = __hip_atomic_load(push_n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t np = np0 < pcap ? np0 : pcap;
     if (np0 > pcap && blockIdx.x == 0 && threadIdx.x == 0) atomicOr(cnt + SP_OVF, 1u);
     const uint64_t n = (uint64_t)n_open + np;
+    const uint64_t b0 = (uint64_t)blockIdx.x * kSplitBlock * kSplitPer;
+    if (b0 >= n) return;  // uniform per workgroup
     const uint32_t F0 = (uint32_t)(ld_gp(gp + t) >> 32);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // uniform trips per block: the appends are wave-wide
-    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
-        const uint64_t i = i0 + threadIdx.x;
-        bool sel[2] = {false, false};
-        uint64_t key = 0;
-        if (i < n) {
-            key = i < n_open ? open[i] : push[i - n_open];
-            const int32_t v = (int32_t)(uint32_t)key;
-            const uint32_t f = (uint32_t)(key >> 32);
-            if (v != t && (uint32_t)(ld_gp(gp + v) >> 32) == f) {
-                sel[0] = f < F0;
-                sel[1] = !sel[0];
-            }
-        }
+    // this workgroup's entries, kSplitPer per thread (coalesced: entry b0 + k * block + thread)
+    uint64_t key[kSplitPer];
+    bool sel[2][kSplitPer];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const bool s1[1] = {sel[k]};
-            uint32_t pos[1];
-            wave_append<1>(s1, cnt + k, pos);
-            if (sel[k]) {
-                if (pos[0] < cap) (k ? keep : live)[pos[0]] = key;
-                else atomicOr(cnt + SP_OVF, 1u);
-            }
+    for (int k = 0; k < kSplitPer; ++k) {
+        const uint64_t i = b0 + (uint64_t)k * kSplitBlock + threadIdx.x;
+        sel[0][k] = sel[1][k] = false;
+        key[k] = 0;
+        if (i < n) key[k] = i < n_open ? open[i] : push[i - n_open];
+    }
+    uint64_t word[kSplitPer];
+#pragma unroll
+    for (int k = 0; k < kSplitPer; ++k) {
+        const uint64_t i = b0 + (uint64_t)k * kSplitBlock + threadIdx.x;
+        const int32_t v = (int32_t)(uint32_t)key[k];
+        const bool need = i < n && v != t && ((uint32_t)(key[k] >> 32) < F0 || check_all);
+        word[k] = need ? ld_gp(gp + v) : 0;
+    }
+    uint32_t mine[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < kSplitPer; ++k) {
+        const uint64_t i = b0 + (uint64_t)k * kSplitBlock + threadIdx.x;
+        const int32_t v = (int32_t)(uint32_t)key[k];
+        const uint32_t f = (uint32_t)(key[k] >> 32);
+        if (i < n && v != t) {
+            if (f < F0) sel[0][k] = (uint32_t)(word[k] >> 32) == f;
+            else sel[1][k] = !check_all || (uint32_t)(word[k] >> 32) == f;
+        }
+        mine[0] += sel[0][k];
+        mine[1] += sel[1][k];
+    }
+    // one reservation per workgroup and list: prefix over the threads
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t pre[2];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const uint32_t x = (uint32_t)wave_incl_scan((int32_t)mine[l]);
+        pre[l] = x - mine[l];
+        if (lane == 63) s_wc[l][wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (int w = 0; w < kSplitBlock / 64; ++w) tot += s_wc[threadIdx.x][w];
+        s_base[threadIdx.x] = tot ? atomicAdd(cnt + threadIdx.x, tot) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        uint32_t pos = s_base[l] + pre[l];
+        for (int w = 0; w < wid; ++w) pos += s_wc[l][w];
+        uint64_t *dst = l ? keep : live;
+#pragma unroll
+        for (int k = 0; k < kSplitPer; ++k) {
+            if (!sel[l][k]) continue;
+            if (pos < cap) dst[pos] = key[k];
+            else atomicOr(cnt + SP_OVF, 1u);
+            ++pos;
         }
     }
 }
@@ -481,8 +687,12 @@ struct ddp_dastar {
     int32_t *d_h = nullptr;
     int32_t *d_mark = nullptr, *d_front = nullptr, *d_marked = nullptr;
     uint32_t *d_rctrl = nullptr;
+    uint32_t *d_walkbar = nullptr;  // subtree_walk's device-wide barrier words
+    hipEvent_t ev_r0 = nullptr, ev_r1 = nullptr;  // around the repair's device work
     int32_t *d_work = nullptr;  // per-batch lines: delete pairs, insert triples, list groups
     size_t c_work = 0;
+    int32_t *h_work = nullptr;  // pinned host staging of d_work (an asynchronous upload)
+    size_t c_hwork = 0;
     int32_t epoch = 0;
     // search
     GaGridPlan P{};
@@ -500,6 +710,7 @@ struct ddp_dastar {
     uint32_t pcap_open = 0, pncap = 0;
     int32_t open_at = 0;   // which buffer holds the open set
     uint32_t open_n = 0;   // its entries
+    uint32_t open_checked = 0;  // open_n after the last split that checked every entry
 };
 
 static void log_free(InsLog &L)
@@ -514,10 +725,13 @@ static void log_free(InsLog &L)
 
 static void dastar_free(ddp_dastar *d)
 {
+    if (d->h_work) hipHostFree(d->h_work);
+    if (d->ev_r0) hipEventDestroy(d->ev_r0);
+    if (d->ev_r1) hipEventDestroy(d->ev_r1);
     ddp_ga_free(&d->S);
     log_free(d->L);
     void *ptrs[] = {d->L.fhead, d->L.rhead, d->L.ftail, d->L.rtail, d->d_reid, d->d_h,
-                    d->d_mark, d->d_front, d->d_marked, d->d_rctrl, d->d_work,
+                    d->d_mark, d->d_front, d->d_marked, d->d_rctrl, d->d_walkbar, d->d_work,
                     d->pbuf[0], d->pbuf[1], d->pbuf[2], d->pnear, d->pres, d->d_split};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -655,7 +869,8 @@ static int dastar_compact(ddp_dastar *d)
 // One pile-mode search over the open set d->pbuf[open_at] (open_n entries)
 // plus the repair pushes counted in control slot 2; the dormant pile it leaves
 // becomes the next open set.
-static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
+// rctrl_out: also copies the repair counters back, in the same host round trip.
+static int pile_run(ddp_dastar *d, int32_t *iters, float *kms, uint32_t *rctrl_out = nullptr)
 {
     ddp_ctx *ctx = d->ctx;
     DaPileArgs a{};
@@ -667,6 +882,8 @@ static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
     // kept entries (the one after, which the search's dormant pile extends);
     // the open set's own buffer is free then and serves as the far-pile partner
     uint64_t *open = d->pbuf[d->open_at];
+    // kept entries are re-checked once the retained set has grown by a quarter
+    const bool check_all = d->open_n > d->open_checked + d->open_checked / 4 + 65536;
     a.live = d->pbuf[(d->open_at + 1) % 3];
     a.alt = open;
     a.dorm = d->pbuf[(d->open_at + 2) % 3];
@@ -677,10 +894,13 @@ static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
     a.delta0 = std::max<uint32_t>(1, d->g->meanw);
     a.split = d->d_split;
     hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, ctx->stream, (int32_t *)d->d_split, (int64_t)SP_WORDS, 0);
-    hipLaunchKernelGGL(open_split, dim3(1024), dim3(256), 0, ctx->stream, (const uint64_t *)open, d->open_n,
+    const uint64_t nsplit = (uint64_t)d->open_n + (uint64_t)d->P.pcap;  // the pushes' count is on the device
+    const uint32_t gsplit = (uint32_t)std::max<uint64_t>(1, (nsplit + kSplitBlock * kSplitPer - 1) /
+                                                                (kSplitBlock * kSplitPer));
+    hipLaunchKernelGGL(open_split, dim3(gsplit), dim3(kSplitBlock), 0, ctx->stream, (const uint64_t *)open, d->open_n,
                        (const uint64_t *)(d->S.pbuf + d->P.pcap),
                        (const uint32_t *)(d->S.ctrl + 2 * GA_SLOT_WORDS + GA_S_PUSHCNT), (uint32_t)d->P.pcap,
-                       (const uint64_t *)d->S.gp, d->t, a.live, a.dorm, d->pcap_open, d->d_split);
+                       (const uint64_t *)d->S.gp, d->t, a.live, a.dorm, d->pcap_open, d->d_split, (int32_t)check_all);
     a.N = d->N;
     a.t = d->t;
     a.maxit = d->P.maxit;
@@ -695,6 +915,8 @@ static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
     DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t res[4];
     DDP_HIP(ctx, hipMemcpyAsync(res, d->pres, 16, hipMemcpyDeviceToHost, ctx->stream));
+    if (rctrl_out)
+        DDP_HIP(ctx, hipMemcpyAsync(rctrl_out, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -706,6 +928,7 @@ static int pile_run(ddp_dastar *d, int32_t *iters, float *kms)
     d->F = res[1];
     d->open_at = (d->open_at + 2) % 3;
     d->open_n = res[3];
+    if (check_all) d->open_checked = d->open_n;
     return DDP_OK;
 }
 
@@ -733,6 +956,7 @@ static int fresh_search(ddp_dastar *d, int32_t *iters, float *kms)
         DDP_HIP(ctx, hipMemcpyAsync(d->pbuf[d->open_at], &key, 8, hipMemcpyHostToDevice, ctx->stream));
         if ((rc = ddp_ga_prepare_resume(ctx, d->S))) return rc;
         d->open_n = 1;
+        d->open_checked = 0;
         return pile_run(d, iters, kms);
     }
     if ((rc = ddp_ga_seed(ctx, d->P, d->S, d->s, (uint32_t)hs))) return rc;
@@ -823,6 +1047,9 @@ extern "C" int ddp_dastar_open(ddp_ctx *ctx, int32_t N, int32_t E, const int32_t
     if (e == hipSuccess) e = hipMalloc((void **)&d->d_front, (size_t)N * 8 + 64);
     if (e == hipSuccess) e = hipMalloc((void **)&d->d_marked, (size_t)N * 4 + 64);
     if (e == hipSuccess) e = hipMalloc((void **)&d->d_rctrl, RC_WORDS * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&d->d_walkbar, 16);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev_r0);
+    if (e == hipSuccess) e = hipEventCreate(&d->ev_r1);
     if (e != hipSuccess) {
         dastar_free(d);
         delete d;
@@ -911,14 +1138,25 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     sort_by_node(rg, N);
     int rc = log_reserve(d, d->log_used + S.inserts);
     if (rc) return rc;
-    // device work list: [pairs | ins | fg | rg]
-    std::vector<int32_t> work;
-    work.reserve(pairs.size() + ins.size() + 4 * fg.size());
-    work.insert(work.end(), pairs.begin(), pairs.end());
-    work.insert(work.end(), ins.begin(), ins.end());
-    for (const auto &x : fg) work.insert(work.end(), {x.first, x.second});
-    for (const auto &x : rg) work.insert(work.end(), {x.first, x.second});
-    if ((rc = dev_upload(ctx, &d->d_work, &d->c_work, work.data(), work.size()))) return rc;
+    // device work list: [pairs | ins | fg | rg], staged in pinned host memory
+    // (every earlier batch synchronised before returning, so it is free)
+    const size_t nwork = pairs.size() + ins.size() + 4 * fg.size();
+    if (d->c_hwork < nwork || !d->h_work) {
+        if (d->h_work) DDP_HIP(ctx, hipHostFree(d->h_work));
+        d->h_work = nullptr;
+        d->c_hwork = 0;
+        const size_t c = std::max<size_t>(nwork, 1 << 16);
+        DDP_HIP(ctx, hipHostMalloc((void **)&d->h_work, c * sizeof(int32_t)));
+        d->c_hwork = c;
+    }
+    {
+        int32_t *o = d->h_work;
+        o = std::copy(pairs.begin(), pairs.end(), o);
+        o = std::copy(ins.begin(), ins.end(), o);
+        for (const auto &x : fg) *o++ = x.first, *o++ = x.second;
+        for (const auto &x : rg) *o++ = x.first, *o++ = x.second;
+    }
+    if ((rc = dev_upload(ctx, &d->d_work, &d->c_work, (const int32_t *)d->h_work, nwork))) return rc;
     const int32_t *w_pairs = d->d_work, *w_ins = w_pairs + pairs.size(), *w_fg = w_ins + ins.size(),
                   *w_rg = w_fg + 2 * fg.size();
     const int T = 256;
@@ -926,6 +1164,7 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     // ---- 2. device: deletes (they never see this batch's inserts), then the appends
     d->epoch += 1;
     uint32_t rctrl[RC_WORDS] = {0};
+    DDP_HIP(ctx, hipEventRecord(d->ev_r0, ctx->stream));
     DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl, 0, RC_WORDS * 4, ctx->stream));
     const int32_t npairs = (int32_t)(pairs.size() / 2), nins = S.inserts;
     if (npairs)
@@ -940,70 +1179,65 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
                            d->L.rtail);
     }
     DDP_HIP(ctx, hipGetLastError());
-    DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
-    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     d->log_used += nins;
-    S.deletes_effective = (int32_t)(rctrl[RC_DEL_BASE] + rctrl[RC_DEL_LOG]);
-    d->live_base -= rctrl[RC_DEL_BASE];
-    d->live_log += nins - (int64_t)rctrl[RC_DEL_LOG];
+    // host preparation only: the device work below follows without a host
+    // round trip until the repair's counters are read
     S.host_ms = ms_since(t0);
     int32_t iters = 0;
     float kms = 0;
     if (mode == DDP_DASTAR_RECOMPUTE) {
         if ((rc = fresh_search(d, &iters, &kms))) return rc;
+        DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     } else {
-        const auto t1 = clk::now();
-        // ---- 3. invalidate parent subtrees of deleted tree edges (roots seeded by apply_deletes)
         if ((rc = ddp_ga_prepare_resume(ctx, d->S))) return rc;
         const GraphView F = fwd_view(d), R = rev_view(d);
-        int32_t *cur = d->d_front, *nxt = d->d_front + N;
-        int32_t ncur = (int32_t)rctrl[RC_NEXT];
-        S.roots = (int32_t)rctrl[RC_ROOTS];
-        // Narrow levels (up to kWalkWide nodes) run inside one subtree_walk
-        // launch; a wider level takes a launch of its own over the whole grid.
-        constexpr int32_t kWalkWide = 8 * kWalkBlock;
-        while (ncur > 0) {
-            if (ncur <= kWalkWide) {
-                hipLaunchKernelGGL(subtree_walk, dim3(1), dim3(kWalkBlock), 0, ctx->stream, F, d->L, d->S.gp,
-                                   d->d_mark, d->epoch, cur, nxt, d->d_marked, d->d_rctrl, kWalkWide);
-            } else {
-                DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl + RC_NEXT, 0, 4, ctx->stream));
-                hipLaunchKernelGGL(subtree_level, grid(ncur), dim3(T), 0, ctx->stream, F, d->L, d->S.gp, d->d_mark,
-                                   d->epoch, cur, ncur, nxt, d->d_marked, d->d_rctrl);
-            }
-            DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
-            DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
-            const bool walked = ncur <= kWalkWide;
-            ncur = (int32_t)rctrl[RC_NEXT];
-            if (!walked || (rctrl[RC_WALKED] & 1u)) std::swap(cur, nxt);
-        }
-        S.invalidated = (int32_t)rctrl[RC_MARKED];
-        // ---- 4. re-pull invalidated nodes, 5. relax inserted edges
+        // ---- 3. invalidate the parent subtrees of deleted tree edges (roots
+        // seeded by apply_deletes): every level in one launch
+        DDP_HIP(ctx, hipMemsetAsync(d->d_walkbar, 0, 16, ctx->stream));
+        hipLaunchKernelGGL(subtree_walk, dim3(kWalkGrid), dim3(kWalkGridBlock), 0, ctx->stream, F, d->L, d->S.gp,
+                           d->d_mark, d->epoch, d->d_front, d->d_front + N, d->d_marked, d->d_rctrl, d->d_walkbar,
+                           (uint64_t)100 * 1000 * 1000 * 5);  // 5 s of the 100 MHz clock per barrier
+        // ---- 4. re-pull invalidated nodes (their count is read on the device), 5. relax inserted edges
         RepairPush P{d->S.pbuf + d->P.pcap, d->S.ctrl + 2 * GA_SLOT_WORDS, d->P.pcap, d->d_rctrl, d->t};
-        if (S.invalidated)
-            hipLaunchKernelGGL(repull, grid(S.invalidated), dim3(T), 0, ctx->stream, R, d->L, d->d_h, d->S.gp,
-                               d->d_mark, d->epoch, d->d_marked, S.invalidated, P);
+        hipLaunchKernelGGL(repull, dim3(256), dim3(T), 0, ctx->stream, R, d->L, d->d_h, d->S.gp, d->d_mark, d->epoch,
+                           (const int32_t *)d->d_marked, (const uint32_t *)d->d_rctrl, P);
         if (nins)
             hipLaunchKernelGGL(relax_inserts, grid(nins), dim3(T), 0, ctx->stream, w_ins, nins, d->d_h, d->S.gp, P);
         DDP_HIP(ctx, hipGetLastError());
-        uint64_t tword = 0;
-        DDP_HIP(ctx, hipMemcpyAsync(&tword, d->S.gp + d->t, 8, hipMemcpyDeviceToHost, ctx->stream));
-        DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipEventRecord(d->ev_r1, ctx->stream));
         DDP_HIP(ctx, hipMemsetAsync(d->S.stats, 0, 64, ctx->stream));
-        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        // ---- 6. resume GA* from the retained queues plus the repair pushes
+        if (d->piles && d->s != d->t) {
+            // the pile search reads F(t) on the device: one host round trip
+            // for the repair counters and the search together
+            if ((rc = pile_run(d, &iters, &kms, rctrl))) return rc;
+        } else {
+            uint64_t tword = 0;
+            DDP_HIP(ctx, hipMemcpyAsync(&tword, d->S.gp + d->t, 8, hipMemcpyDeviceToHost, ctx->stream));
+            DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
+            DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (d->s == d->t) {
+                d->F = (uint32_t)(tword >> 32);
+            } else if (d->piles) {
+                if ((rc = pile_run(d, &iters, &kms))) return rc;
+            } else if ((rc = ddp_ga_run(ctx, F, d->hv, d->P, d->S, d->t, (uint32_t)(tword >> 32), d->rr, &d->F,
+                                        &d->rr, &iters, &kms))) {
+                return rc;
+            }
+        }
+        DDP_CHECK(rctrl[RC_WALK_ERR] == 0, ctx, DDP_ERR_TIMEOUT,
+                  "DA*: a device-wide barrier of the subtree walk timed out (are %d workgroups co-resident?)",
+                  kWalkGrid);
+        S.roots = (int32_t)rctrl[RC_ROOTS];
+        S.invalidated = (int32_t)std::min<uint32_t>(rctrl[RC_MARKED], (uint32_t)N);
         S.repulled = (int32_t)rctrl[RC_REPULLED];
         S.pushes = (int32_t)rctrl[RC_PUSHES];
-        S.repair_ms = ms_since(t1);
-        // ---- 6. resume GA* from the retained queues plus the repair pushes
-        if (d->s == d->t) {
-            d->F = (uint32_t)(tword >> 32);
-        } else if (d->piles) {
-            if ((rc = pile_run(d, &iters, &kms))) return rc;
-        } else if ((rc = ddp_ga_run(ctx, F, d->hv, d->P, d->S, d->t, (uint32_t)(tword >> 32), d->rr, &d->F, &d->rr,
-                                    &iters, &kms))) {
-            return rc;
-        }
+        DDP_HIP(ctx, hipEventElapsedTime(&S.repair_ms, d->ev_r0, d->ev_r1));
     }
+    S.deletes_effective = (int32_t)(rctrl[RC_DEL_BASE] + rctrl[RC_DEL_LOG]);
+    d->live_base -= rctrl[RC_DEL_BASE];
+    d->live_log += nins - (int64_t)rctrl[RC_DEL_LOG];
     S.iterations = iters;
     S.search_ms = kms;
     unsigned long long cnt[4];

@@ -42,8 +42,8 @@ typedef struct {
     int32_t pushes;             /* repair pushes handed to the GA* queues */
     int32_t iterations;         /* GA* iterations of the resumed search */
     int64_t expansions, edges, improvements;
-    float host_ms;              /* delete matching + delta rebuild on the host */
-    float repair_ms;            /* device invalidation, re-pull, insert relax */
+    float host_ms;              /* host preparation: splitting and grouping the lines, upload */
+    float repair_ms;            /* device time of the deletes, invalidation, re-pull and insert relax */
     float search_ms;            /* resumed (or recomputed) GA* kernel time */
     float total_ms;
     int32_t compacted;          /* 1 if the batch ended with a log compaction */
