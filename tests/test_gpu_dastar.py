@@ -160,9 +160,9 @@ def test_alternative_paths_loop(ctx):
 
 def test_dastar_wide_and_narrow_subtree_levels(ctx):
     """Deleting s->a invalidates a's whole subtree: a, a level of 20,000
-    leaves (wider than one workgroup's walk takes, so it gets a launch of its
-    own) and a 50-node chain below one leaf (narrow levels, walked in one
-    launch).  The leaf is then re-pulled through a costlier detour; a second
+    leaves (spread over every workgroup of the walk) and a 50-node chain below
+    one leaf (50 one-node levels, each ended by the walk's device-wide
+    barrier).  The leaf is then re-pulled through a costlier detour; a second
     batch puts s->a back."""
     nl, nc = 20000, 50
     a, leaf0, chain0 = 1, 2, 2 + nl
@@ -231,14 +231,17 @@ def test_dastar_many_roots_at_once(ctx):
 
 
 @pytest.mark.timeout(900)
-def test_dastar_config3_full_size(ctx):
+@pytest.mark.parametrize("mode", ["grid", "piles"])
+def test_dastar_config3_full_size(ctx, mode):
     """C3 at full size (N = 1,000,000, out-degree 8, w in [1,1000], seed 3),
     all 20 of its batches of 10,000 inserts + 10,000 deletes at K = 16,384
     (the K the benchmark runs), h = reverse-BFS hops x min weight on the union
-    of every version.  Each batch: repair == recompute.  Every 4th batch and
-    the last: the cost equals the oracle's Dijkstra on the oracle's own merged
-    graph and the path is a real path of that cost.  After the last: the
-    library's graph equals the oracle's edge for edge."""
+    of every version, the repairing session on the grid kernel or the
+    one-workgroup pile engine (both benchmark sessions).  Each batch: repair
+    == recompute.  Every 4th batch and the last: the cost equals the oracle's
+    Dijkstra on the oracle's own merged graph and the path is a real path of
+    that cost.  After the last: the library's graph equals the oracle's edge
+    for edge."""
     nb = 20
     hg = ddp_amd.HostGraph.random(1000000, 8, 1, 1000, 3)
     upd = hg.gen_updates(nb, 10000, 10000, 1, 1000, 33).batches
@@ -258,7 +261,8 @@ def test_dastar_config3_full_size(ctx):
     roff, rcol, _ = pyoracle.transpose(uoff, vs[order].astype(np.int32), ws[order].astype(np.uint32))
     hops, _ = pyoracle.bfs_hops(roff, rcol, t)
     h = np.where(hops < 0, -1, hops * int(ws.min())).astype(np.int32)
-    da = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=16384)
+    da = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=16384,
+                        mode=ddp_amd.MODE_PILES if mode == "piles" else ddp_amd.MODE_AUTO)
     rc = ddp_amd.DAStar(ctx, hg, s, t, h=h, K=16384)
     assert da.cost == rc.cost
     cur = (hg.off, hg.col, hg.w)
