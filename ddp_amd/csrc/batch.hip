@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(B) batch_kernel(BatchArgs a)
                               s_heap + a.lring / 2, a.lring / 2};
             const PilePtrs PF{home, home + a.ncap, home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring / 2};
             const BidiSide D{{a.g.off, a.gr.off}, {a.g.col, a.gr.col}, {a.g.w, a.gr.w}, {gp, gp + N},
-                             {touched, touched + N}, {&PF, &PB}};
+                             {touched, touched + N}, {PF, PB}};
             slot_search_bidi<B, HK>(D, a.h, a.g.relax_min != 0, N, s, t, a.maxit, a.delta0, S, C, stale);
             if (a.path_cap > 0) {
                 slot_path_bidi(gp, gp + N, N, s, t, a.path + (int64_t)q * a.path_cap, a.path_cap, S);

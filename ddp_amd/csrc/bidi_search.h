@@ -101,6 +101,18 @@ struct BidiLds {
 
 constexpr uint64_t kNoMeet = ((uint64_t)DDP_INF_F << 32) | 0xFFFFFFFFull;
 
+// a[one] for a direction that differs between lanes: both elements are read
+// with constant indices (uniform values) and one is selected per lane.
+// Indexing the pair with a per-lane value instead puts the whole structure in
+// scratch memory, and every such read becomes a memory round trip ahead of
+// the load it feeds.
+template <class T>
+__device__ __forceinline__ T pick2(const T (&a)[2], bool one)
+{
+    const T x0 = a[0], x1 = a[1];
+    return one ? x1 : x0;
+}
+
 // Everything a round needs about the two directions.  With a versioned view
 // (the server), base entry j of direction d is visible iff dver[d][j] > ver,
 // and inserted edges hang off per-node lists (direction 0: by source, 1: by
@@ -113,7 +125,7 @@ struct BidiSide {
     const uint32_t *w[2];
     uint64_t *gp[2];
     int32_t *touched[2];
-    const PilePtrs *P[2];
+    PilePtrs P[2];  // by value: kept in registers (a pointer would pin them in memory)
     const int32_t *dver[2];
     int32_t ver;
     const uint64_t *lhead[2];
@@ -140,7 +152,7 @@ __device__ __forceinline__ void bidi_settle(const BidiSide &D, int32_t s, int32_
     uint64_t oth[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
-        if (ok[r]) oth[r] = ld_gp(D.gp[dr[r] ? 0 : 1] + c[r]);
+        if (ok[r]) oth[r] = ld_gp(pick2(D.gp, !dr[r]) + c[r]);
 #pragma unroll
     for (int d = 0; d < 2; ++d) {
         bool fd[R], put[R];
@@ -159,8 +171,8 @@ __device__ __forceinline__ void bidi_settle(const BidiSide &D, int32_t s, int32_
         for (int r = 0; r < R; ++r)
             if (fd[r]) D.touched[d][tp[r]] = c[r];
         BidiDir &Q = S.d[d];
-        const bool lds_ok = D.P[d]->lcap && Q.near_tail == Q.near_head;
-        if (pile_place<R>(*D.P[d], Q.thr, lds_ok, Q.lring_head, Q.near_head, &Q.lring_tail, &Q.near_tail, far[d],
+        const bool lds_ok = D.P[d].lcap && Q.near_tail == Q.near_head;
+        if (pile_place<R>(D.P[d], Q.thr, lds_ok, Q.lring_head, Q.near_head, &Q.lring_tail, &Q.near_tail, far[d],
                           &Q.far_n, put, key))
             Q.ovf = 1;
     }
@@ -240,9 +252,9 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
             dr[r] = jj[r] >= n0;
             if (pos[r] >= 0) {
                 const int k = dr[r] ? 1 : 0;
-                c[r] = D.col[k][pos[r]];
-                wt[r] = D.w[k][pos[r]];
-                if constexpr (VERS) dv[r] = D.dver[k][pos[r]];
+                c[r] = pick2(D.col, k != 0)[pos[r]];
+                wt[r] = pick2(D.w, k != 0)[pos[r]];
+                if constexpr (VERS) dv[r] = pick2(D.dver, k != 0)[pos[r]];
             }
         }
         // stage 2: heuristic inputs and current words (written only by lanes that take part)
@@ -253,7 +265,7 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
             if (VERS && dv[r] <= D.ver) c[r] = -1;  // deleted at or before this version
             if (c[r] >= 0) {
                 hin[r] = heur_load<HK>(H, c[r]);
-                cur[r] = ld_gp(D.gp[dr[r] ? 1 : 0] + c[r]);
+                cur[r] = ld_gp(pick2(D.gp, dr[r]) + c[r]);
             }
         }
         // stage 3: candidates and one atomic per improving edge
@@ -275,13 +287,13 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 if (tried[r])
-                    got[r] = (uint64_t)atomicMin((unsigned long long *)(D.gp[dr[r] ? 1 : 0] + c[r]),
+                    got[r] = (uint64_t)atomicMin((unsigned long long *)(pick2(D.gp, dr[r]) + c[r]),
                                                  (unsigned long long)want[r]);
         } else {
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 if (tried[r])
-                    got[r] = (uint64_t)atomicCAS((unsigned long long *)(D.gp[dr[r] ? 1 : 0] + c[r]),
+                    got[r] = (uint64_t)atomicCAS((unsigned long long *)(pick2(D.gp, dr[r]) + c[r]),
                                                  (unsigned long long)cur[r], (unsigned long long)want[r]);
         }
         // stage 4: settle contended CASes
@@ -298,7 +310,7 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
                     uint64_t now = got[r];
                     while ((uint32_t)(now >> 32) > cand) {
                         const uint64_t old =
-                            atomicCAS((unsigned long long *)(D.gp[dr[r] ? 1 : 0] + c[r]), (unsigned long long)now,
+                            atomicCAS((unsigned long long *)(pick2(D.gp, dr[r]) + c[r]), (unsigned long long)now,
                                       (unsigned long long)want[r]);
                         if (old == now) {
                             good = true;
@@ -321,7 +333,8 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
         // Inserted edges: each lane walks its own node's list, oldest first,
         // stopping at the first entry newer than this version.
         const int d = lane >= n0 ? 1 : 0;
-        uint64_t link = u >= 0 ? __hip_atomic_load(D.lhead[d] + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
+        uint64_t link =
+            u >= 0 ? __hip_atomic_load(pick2(D.lhead, d != 0) + u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : ~0ull;
         for (;;) {
             bool active = false;
             bool ok[1] = {false}, first[1] = {false}, dr[1] = {d != 0};
@@ -330,7 +343,7 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
             int32_t x = -1;
             while ((uint32_t)link <= (uint32_t)D.ver) {
                 x = (int32_t)(link >> 32);
-                link = __hip_atomic_load(D.lnext[d] + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                link = __hip_atomic_load(pick2(D.lnext, d != 0) + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (D.ldel[x] > D.ver) {
                     active = true;
                     break;
@@ -338,11 +351,11 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
             }
             if (active) {
                 C.edges++;
-                cn[0] = D.lnb[d][x];
+                cn[0] = pick2(D.lnb, d != 0)[x];
                 const int32_t pc = pot_from<HK>(H, cn[0], heur_load<HK>(H, cn[0]));
                 const uint32_t cand = rs.gu + D.lw[x] + (uint32_t)(d ? -pc : pc);
                 uint64_t prev;
-                if (relax_gp(D.gp[d], cn[0], cand, (uint32_t)u, &prev)) {
+                if (relax_gp(pick2(D.gp, d != 0), cn[0], cand, (uint32_t)u, &prev)) {
                     C.improved++;
                     ok[0] = true;
                     first[0] = (uint32_t)(prev >> 32) == DDP_INF_F;
@@ -357,16 +370,17 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
 
 // Refill of direction d's near rings from its far pile (the refill of
 // slot_search_piles); a full refill (near rings empty) publishes the
-// direction's bound.  Block-collective; ends with a barrier.
-template <int B, int HK>
-__device__ __forceinline__ void bidi_refill(const BidiSide &D, int d, int32_t oroot, bool early, uint32_t F,
-                                            uint32_t lh, uint32_t hh, uint64_t *(&far)[2], uint64_t *(&far2)[2],
-                                            BidiLds<B> &S, ExpandCounters &C)
+// direction's bound.  Block-collective; ends with a barrier.  The direction
+// is a template argument so that every pair is indexed by a constant.
+template <int B, int HK, int d>
+__device__ __forceinline__ void bidi_refill(const BidiSide &D, int32_t oroot, bool early, uint32_t F, uint32_t lh,
+                                            uint32_t hh, uint64_t *(&far)[2], uint64_t *(&far2)[2], BidiLds<B> &S,
+                                            ExpandCounters &C)
 {
     constexpr uint32_t Vd = B / 2;  // lanes a direction gets when both have work
     const int lane = threadIdx.x;
     BidiDir &Q = S.d[d];
-    const PilePtrs &P = *D.P[d];
+    const PilePtrs &P = D.P[d];
     uint64_t *gp = D.gp[d];
     if (Q.ovf) {
         // rebuild the far pile from the touched log (entries were dropped)
@@ -481,7 +495,7 @@ __device__ __forceinline__ void bidi_refill(const BidiSide &D, int d, int32_t or
 // | s) and S.err the error flags; both (key, parent) arrays and touched logs
 // still describe the search for slot_path_bidi and slot_reset_bidi.
 template <int B, int HK, bool VERS = false>
-__device__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min, int32_t N, int32_t s, int32_t t,
+__device__ __forceinline__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min, int32_t N, int32_t s, int32_t t,
                                  int32_t maxit, uint32_t delta0, BidiLds<B> &S, ExpandCounters &C, uint32_t &stale)
 {
     constexpr uint32_t Vd = B / 2;
@@ -491,6 +505,7 @@ __device__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min,
         S.bound[0] = S.bound[1] = 0;
         S.err = 0;
         S.iters = 0;
+#pragma unroll
         for (int d = 0; d < 2; ++d) {
             S.d[d] = BidiDir{};
             S.d[d].delta = delta0 > 0 ? delta0 : 1;
@@ -510,16 +525,17 @@ __device__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min,
             const uint32_t k0 = (uint32_t)ps, k1 = (uint32_t)(-pt);  // g = 0 at the roots
             const int32_t root[2] = {s, t};
             const uint32_t k[2] = {k0, k1};
+#pragma unroll
             for (int d = 0; d < 2; ++d) {
                 const uint64_t key = ((uint64_t)k[d] << 32) | (uint32_t)root[d];
                 D.gp[d][root[d]] = ((uint64_t)k[d] << 32) | 0xFFFFFFFFull;
                 D.touched[d][0] = root[d];
                 S.d[d].ntouch = 1;
-                if (D.P[d]->lcap) {
-                    D.P[d]->lring[0] = key;
+                if (D.P[d].lcap) {
+                    D.P[d].lring[0] = key;
                     S.d[d].lring_tail = 1;
                 } else {
-                    D.P[d]->near[0] = key;
+                    D.P[d].near[0] = key;
                     S.d[d].near_tail = 1;
                 }
                 S.d[d].thr = sat_add_u32(k[d], S.d[d].delta);
@@ -530,7 +546,7 @@ __device__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min,
         S.err = QERR_BADQ;
     }
     block_sync_global();
-    uint64_t *far[2] = {D.P[0]->far0, D.P[1]->far0}, *far2[2] = {D.P[0]->far1, D.P[1]->far1};
+    uint64_t *far[2] = {D.P[0].far0, D.P[1].far0}, *far2[2] = {D.P[0].far1, D.P[1].far1};
     uint32_t lh[2] = {0, 0}, hh[2] = {0, 0};  // ring heads (uniform)
     int32_t it = 0;
     while (run) {
@@ -562,7 +578,8 @@ __device__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min,
         }
         if (rd >= 0) {
             if (early && from_lds[rd]) block_sync_global();  // the far pile is read below
-            bidi_refill<B, HK>(D, rd, rd ? s : t, early, F[rd], lh[rd], hh[rd], far, far2, S, C);
+            if (rd) bidi_refill<B, HK, 1>(D, s, early, F[1], lh[1], hh[1], far, far2, S, C);
+            else bidi_refill<B, HK, 0>(D, t, early, F[0], lh[0], hh[0], far, far2, S, C);
             if (S.err) break;
             continue;
         }
@@ -579,15 +596,24 @@ __device__ void slot_search_bidi(const BidiSide &D, HeurView hv, bool relax_min,
         if ((uint32_t)lane < n0 + n1) {
             const int d = (uint32_t)lane >= n0 ? 1 : 0;
             const uint32_t idx = d ? lane - n0 : lane;
-            const PilePtrs &P = *D.P[d];
-            const uint64_t key = from_lds[d] ? P.lring[(lh[d] + idx) & (P.lcap - 1)] : P.near[(hh[d] + idx) & (P.ncap - 1)];
+            // values, not lvalues, are selected: a conditional over two fields
+            // selects their addresses, and a load at a per-lane offset pins D
+            // in scratch memory
+            const bool one = d != 0;
+            uint64_t *const lring0 = D.P[0].lring, *const lring1 = D.P[1].lring;
+            uint64_t *const near0 = D.P[0].near, *const near1 = D.P[1].near;
+            const uint32_t lcap0 = D.P[0].lcap, lcap1 = D.P[1].lcap, ncap0 = D.P[0].ncap, ncap1 = D.P[1].ncap;
+            const uint64_t key = pick2(from_lds, one)
+                                     ? (one ? lring1 : lring0)[(pick2(lh, one) + idx) & ((one ? lcap1 : lcap0) - 1)]
+                                     : (one ? near1 : near0)[(pick2(hh, one) + idx) & ((one ? ncap1 : ncap0) - 1)];
             const uint32_t f = (uint32_t)(key >> 32);
             const int32_t node = (int32_t)(uint32_t)key;
             // the node's word, row and potential input in one round trip
-            const uint64_t word = ld_gp(D.gp[d] + node);
-            const int32_t rb = D.off[d][node], re = D.off[d][node + 1];
+            const uint64_t word = ld_gp(pick2(D.gp, one) + node);
+            const int32_t *off = pick2(D.off, one);
+            const int32_t rb = off[node], re = off[node + 1];
             const int32_t pn = pot_from<HK>(hv, node, heur_load<HK>(hv, node));
-            if (f < F[d] && (uint32_t)(word >> 32) == f) {
+            if (f < pick2(F, one) && (uint32_t)(word >> 32) == f) {
                 u = node;
                 rs.b = rb;
                 rs.deg = re - rb;

@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
         const PilePtrs PF{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring / 2};
         const PilePtrs PB{bh, bh + a.ncap, bh + a.ncap + a.fcap, a.ncap, a.fcap, s_heap + a.lring / 2, a.lring / 2};
         const BidiSide D{{a.g.off, a.gr.off}, {a.g.col, a.gr.col}, {a.g.w, a.gr.w}, {a.gp, a.gp + N},
-                         {a.touched, a.touched + N}, {&PF, &PB}};
+                         {a.touched, a.touched + N}, {PF, PB}};
         slot_search_bidi<B, HK>(D, a.h, a.g.relax_min != 0, N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
         if (a.path) slot_path_bidi(a.gp, a.gp + N, N, a.s, a.t, a.path, a.path_cap, S);
         if (lane == 0) {

@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                 const PilePtrs PB{bh, bh + a.ncap, bh + a.ncap + a.fcap, a.ncap, a.fcap, s_heap + a.lring / 2,
                                   a.lring / 2};
                 const BidiSide D{{a.g.off, a.gr.off},  {a.g.col, a.gr.col}, {a.g.w, a.gr.w},     {gp, gp + N},
-                                 {touched, touched + N}, {&PF, &PB},          {a.dver, a.rdver}, rq.version,
+                                 {touched, touched + N}, {PF, PB},          {a.dver, a.rdver}, rq.version,
                                  {a.lhead, a.rlhead},  {a.lnext, a.rlnext}, {a.lv, a.lu},       a.lw,
                                  a.ldel};
                 slot_search_bidi<B, HK, true>(D, a.h, G.relax_min != 0, N, rq.s, rq.t, a.maxit, a.delta0, S, C,
