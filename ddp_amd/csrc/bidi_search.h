@@ -550,8 +550,24 @@ __device__ __forceinline__ void slot_search_bidi(const BidiSide &D, HeurView hv,
     uint32_t lh[2] = {0, 0}, hh[2] = {0, 0};  // ring heads (uniform)
     int32_t it = 0;
     while (run) {
-        const uint32_t mu = (uint32_t)(S.meet >> 32);
+        // Every LDS word the round's decisions need, read together: they are
+        // independent, so one wait covers them (read one by one between the
+        // branches, they cost about 1,000 cycles of a 14,000-cycle C2 round).
+        // The counters were last changed by the previous round's LDS atomics,
+        // which its closing barrier ordered; only HBM ring contents need the
+        // store drain below.
+        const uint64_t meet = S.meet;
         const uint32_t b0 = S.bound[0], b1 = S.bound[1];
+        uint32_t lt[2], nt[2], fn[2], ov[2], ne[2];
+#pragma unroll
+        for (int d = 0; d < 2; ++d) {
+            lt[d] = S.d[d].lring_tail;
+            nt[d] = S.d[d].near_tail;
+            fn[d] = S.d[d].far_n;
+            ov[d] = S.d[d].ovf;
+            ne[d] = S.d[d].no_early;
+        }
+        const uint32_t mu = (uint32_t)(meet >> 32);
         if ((uint64_t)b0 + b1 >= mu) break;  // no open entry on either side can shorten mu
         const uint32_t F[2] = {b1 >= mu ? 0u : mu - b1, b0 >= mu ? 0u : mu - b0};
         bool from_lds[2];
@@ -559,7 +575,7 @@ __device__ __forceinline__ void slot_search_bidi(const BidiSide &D, HeurView hv,
         bool any_hbm = false;
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            from_lds[d] = S.d[d].lring_tail != lh[d];
+            from_lds[d] = lt[d] != lh[d];
             any_hbm |= !from_lds[d];
         }
         if (any_hbm) block_sync_global();  // HBM entries ahead: every wave's stores have landed
@@ -567,10 +583,9 @@ __device__ __forceinline__ void slot_search_bidi(const BidiSide &D, HeurView hv,
         bool early = false;
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
-            const BidiDir &Q = S.d[d];
-            avail[d] = from_lds[d] ? Q.lring_tail - lh[d] : Q.near_tail - hh[d];
-            const bool e = avail[d] > 0 && (Q.lring_tail - lh[d]) + (Q.near_tail - hh[d]) < Vd / kThinRing &&
-                           Q.far_n > 0 && !Q.ovf && !Q.no_early;
+            avail[d] = from_lds[d] ? lt[d] - lh[d] : nt[d] - hh[d];
+            const bool e = avail[d] > 0 && (lt[d] - lh[d]) + (nt[d] - hh[d]) < Vd / kThinRing && fn[d] > 0 &&
+                           !ov[d] && !ne[d];
             if (rd < 0 && (avail[d] == 0 || e)) {
                 rd = d;
                 early = avail[d] > 0;
