@@ -1,6 +1,7 @@
 // Dynamic A* (DA*): repair one query's search state after a batch of edge
-// inserts and deletes, then resume the GA* grid kernel from its retained
-// queues.
+// inserts and deletes, then resume the search from its retained open set --
+// the GA* grid kernel's queues, or the one-workgroup pile engine's dormant
+// pile (DDP_ASTAR_PILES).
 //
 // Invariant kept between batches (the one A* itself maintains): every finite
 // g is the cost of a real path, and every node that is not in a queue has
@@ -8,15 +9,16 @@
 //   1. deletes (matched on the device by apply_deletes, first live instance
 //      of u->v as in DA_star/GA_merge.cu:1019-1037) become tombstones in HBM;
 //   2. if a deleted u->v was v's parent edge, v's whole parent subtree is
-//      invalidated (found by walking children: y with parent(y) == x), which
-//      replaces the ancestor walks and serial cycle fixing of
+//      invalidated (found by walking children: y with parent(y) == x, level
+//      by level in one launch over 64 workgroups), which replaces the
+//      ancestor walks and serial cycle fixing of
 //      DA_star/DA_dyn_insert_del_nocycle.cu:376-483, 1023-1134;
 //   3. each invalidated node re-pulls its best value from live in-edges of
 //      nodes outside the invalidated set (a parent chain that avoids every
 //      invalidated node, so no cycle can form) and is queued;
 //   4. inserted edges u->v are relaxed from u (propogateAdd, NC:486-553);
-//   5. GA* resumes: queued nodes are expanded only while they can still beat
-//      the target, instead of the unbounded Bellman-Ford waves of
+//   5. the search resumes: queued nodes are expanded only while they can still
+//      beat the target, instead of the unbounded Bellman-Ford waves of
 //      insert_propagate / delete_propagate (NC:556-901).
 // The graph lives on the device: base rows with tombstones, plus a log of
 // inserted edges threaded onto per-source lists (insert order) and per-target
