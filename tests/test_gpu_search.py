@@ -488,9 +488,10 @@ def test_batch_config4_full_size(ctx):
 
 def test_astar_config2_full_size(ctx):
     """C2 at full size: the 4096x4096 maze (p_block 0.30, seed 2), corner to
-    corner, Manhattan heuristic.  Pile mode (K = 1,024) and grid mode
-    (K = 2,048) return the cost of the oracle's indexed-heap A* and real
-    paths of that cost."""
+    corner, Manhattan heuristic, in every mode and K the benchmark sweeps
+    (workgroup K = 512 / 1,024, piles K = 1,024, bidirectional K = 256 / 512 /
+    1,024, grid K = 512 / 2,048 / 8,192): each returns the cost of the oracle's
+    indexed-heap A* and a real path of that cost."""
     W = 4096
     hg = ddp_amd.HostGraph.grid_maze(W, W, 300, 2)
     s, t = 0, W * W - 1
@@ -498,9 +499,11 @@ def test_astar_config2_full_size(ctx):
     h = ((W - 1 - xs // W) + (W - 1 - xs % W)).astype(np.int32)
     found, want, _ = pyoracle.astar_indexed(hg.off, hg.col, hg.w, h, s, t)
     assert found and want >= 2 * (W - 1)
-    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reverse=True)  # transposed rows for the bidirectional mode
     dg.set_grid(W)
-    for K, mode in ((1024, ddp_amd.MODE_PILES), (2048, ddp_amd.MODE_GRID)):
+    for K, mode in ((512, ddp_amd.MODE_WORKGROUP), (1024, ddp_amd.MODE_WORKGROUP), (1024, ddp_amd.MODE_PILES),
+                    (256, ddp_amd.MODE_BIDI), (512, ddp_amd.MODE_BIDI), (1024, ddp_amd.MODE_BIDI),
+                    (512, ddp_amd.MODE_GRID), (2048, ddp_amd.MODE_GRID), (8192, ddp_amd.MODE_GRID)):
         say("c2 full", K, mode)
         r = ddp_amd.astar(ctx, dg, s, t, heur=ddp_amd.H_MANHATTAN, K=K, mode=mode)
         assert r.cost == want

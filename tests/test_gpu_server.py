@@ -160,7 +160,8 @@ def test_server_config4_full_size(ctx):
     heuristic admissible) in the middle.  Queries before the update return
     the batch kernel's costs on the original graph, queries after it the
     batch kernel's costs on the updated graph: the same answer alone and in
-    the mix."""
+    the mix.  Three queries on each side of the update also equal the
+    oracle's Dijkstra on their own version of the graph."""
     hg = ddp_amd.HostGraph.geometric(2000000, 10, 100000, 4)
     src, dst = hg.geo_pairs(4096, 100, 100000, 4)
     ub = geometric_updates(hg, 1, 100, 8)
@@ -195,6 +196,13 @@ def test_server_config4_full_size(ctx):
     assert (ver[:mid] == 0).all() and (ver[mid:] == 1).all()
     assert np.array_equal(got[:mid], before)
     assert np.array_equal(got[mid + 1:], after)
+    g1_rows = (g1.off, g1.col, g1.w)
+    for i in (0, mid // 2, mid - 1, mid + 1, mid + len(src) // 4, n - 1):
+        rows = (hg.off, hg.col, hg.w) if i < mid else g1_rows
+        dist, _ = pyoracle.dijkstra(*rows, int(qs[i]), int(qt[i]))
+        want = int(dist[qt[i]]) if dist[qt[i]] >= 0 else ddp_amd.INF_COST
+        say("c4 served: request", i, "version", ver[i], got[i], want)
+        assert got[i] == want
 
 
 @pytest.mark.timeout(1200)
