@@ -194,13 +194,18 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     double best_ms = 1e300;
     ddp_astar_opts_t win;
     std::memset(&win, 0, sizeof win);
-    // workgroup mode with 256..1024 queues, pile mode with 512/1024 lanes, grid mode with 1024..8192
+    // workgroup mode with 256..1024 queues, pile mode with 512/1024 lanes, the
+    // search from both ends with 256..1024 lanes (closed-form heuristics; on a
+    // graph without transposed rows ddp_astar refuses it before any launch and
+    // the candidate drops out like any that fails), grid mode with 1024..8192
     struct Cand {
         int32_t K, B, mode;
     };
     std::vector<Cand> cands;
     for (int32_t K : {256, 512, 1024}) cands.push_back({K, 0, DDP_ASTAR_WORKGROUP});
     for (int32_t K : {512, 1024}) cands.push_back({K, 0, DDP_ASTAR_PILES});
+    if (h.kind != DDP_H_ARRAY)
+        for (int32_t K : {256, 512, 1024}) cands.push_back({K, 0, DDP_ASTAR_BIDI});
     for (int32_t K : {1024, 2048, 4096, 8192})
         for (int32_t B : {64, 256}) cands.push_back({K, B, DDP_ASTAR_GRID});
     auto time_opts = [&](const ddp_astar_opts_t &o) {
@@ -228,7 +233,7 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     if (best_ms >= 1e300) return DDP_ERR_STATE;
     // Heap depth of the winning shape: LDS levels per queue (grid and
     // workgroup modes; the rest spills to HBM) and HBM entries per queue.
-    if (win.mode != DDP_ASTAR_PILES) {
+    if (win.mode != DDP_ASTAR_PILES && win.mode != DDP_ASTAR_BIDI) {
         for (int32_t lc : {8, 16, 32}) {
             ddp_astar_opts_t o = win;
             o.lds_cap = lc;
