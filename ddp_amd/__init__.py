@@ -572,9 +572,9 @@ class Server:
     QUERY, UPDATE = 0, 1
 
     def __init__(self, ctx, dg, heur=H_ZERO, scale=1, block=0, slots=0, heap_cap=0, ring=0, max_inserts=0,
-                 idle_ms=0, engine=ENGINE_AUTO):
+                 idle_ms=0, engine=ENGINE_AUTO, grid_k=0):
         self.ctx, self.dg = ctx, dg
-        o = _capi.ServerOpts(block, slots, heap_cap, ring, max_inserts, idle_ms, engine)
+        o = _capi.ServerOpts(block, slots, heap_cap, ring, max_inserts, idle_ms, engine, grid_k)
         h = c_void_p()
         _check(_capi.lib().ddp_server_create(ctx.handle, dg.handle, _heur(heur, scale), byref(o), byref(h)),
                ctx.handle)
@@ -585,6 +585,11 @@ class Server:
         v = [c_int32() for _ in range(3)]
         _check(_capi.lib().ddp_server_info(self._h, *[byref(x) for x in v]))
         return dict(zip(["slots", "block", "version"], [x.value for x in v]))
+
+    def stats(self):
+        st = _capi.ServerStats()
+        _check(_capi.lib().ddp_server_stats(self._h, byref(st)))
+        return {f: getattr(st, f) for f, _ in _capi.ServerStats._fields_}
 
     def submit_query(self, s, t, tag):
         return _capi.lib().ddp_server_submit_query(self._h, s, t, tag)

@@ -81,7 +81,11 @@ class DAStats(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("block", c_int32), ("slots", c_int32), ("heap_cap", c_int32), ("ring", c_int32),
-                ("max_inserts", c_int32), ("idle_ms", c_int32), ("engine", c_int32), ("reserved", c_int32)]
+                ("max_inserts", c_int32), ("idle_ms", c_int32), ("engine", c_int32), ("grid_k", c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", c_int64), ("grid_queries", c_int64), ("launches", c_int32), ("compactions", c_int32)]
 
 
 class ServerResult(ctypes.Structure):
@@ -138,6 +142,7 @@ SIGNATURES = {
     # ddp_server.h
     "ddp_server_create": (c_int, [c_void_p, c_void_p, Heur, POINTER(ServerOpts), PP]),
     "ddp_server_info": (c_int, [c_void_p, P_I32, P_I32, P_I32]),
+    "ddp_server_stats": (c_int, [c_void_p, c_void_p]),
     "ddp_server_submit_query": (c_int, [c_void_p, c_int32, c_int32, c_uint64]),
     "ddp_server_submit_update": (c_int, [c_void_p, POINTER(Update), c_int32, c_uint64]),
     "ddp_server_poll": (c_int, [c_void_p, POINTER(ServerResult), c_int32, P_I32]),

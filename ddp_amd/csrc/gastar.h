@@ -16,6 +16,9 @@ enum : int {
     GA_CTRL_WORDS = 96,
 };
 
+// Error bits of GA_RES_ERR.
+enum : uint32_t { GA_ERR_HEAP = 1, GA_ERR_PBUF = 2, GA_ERR_ITERS = 4, GA_ERR_BARRIER = 8 };
+
 struct GaGridPlan {
     int32_t K, B, nblocks, lcap, cap, pcap, maxit;
     bool wg = false;     // one workgroup (ddp_astar only): lane heaps, or piles when `piles`
@@ -54,6 +57,12 @@ int ddp_ga_prepare_resume(ddp_ctx *ctx, const GaGridState &S);
 int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
                const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
                int32_t *rr_out, int32_t *iters_out, float *kernel_ms);
+// Grid mode, asynchronous: resets the state, seeds s and enqueues the grid
+// kernel on ctx->stream without waiting.  Stream-ordered work after it reads
+// the outcome from S.ctrl (GA_RES_F: f of t, DDP_INF_F if unreachable;
+// GA_RES_ERR: GA_ERR_* bits).
+int ddp_ga_launch(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+                  const GaGridState &S, int32_t N, int32_t s, uint32_t hs, int32_t t);
 // Workgroup mode: one search from s to completion (no seed needed).  The
 // bidirectional plan also needs the transposed rows (gr) and walks the path on
 // the device into dpath[0..dpath_cap) (length to *plen: 0 if unreachable,

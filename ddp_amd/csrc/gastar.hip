@@ -33,7 +33,10 @@
 
 namespace {
 
-enum : uint32_t { ERR_HEAP = 1, ERR_PBUF = 2, ERR_ITERS = 4, ERR_BARRIER = 8, ERR_PATH = 16 };
+enum : uint32_t {
+    ERR_HEAP = GA_ERR_HEAP, ERR_PBUF = GA_ERR_PBUF, ERR_ITERS = GA_ERR_ITERS, ERR_BARRIER = GA_ERR_BARRIER,
+    ERR_PATH = 16
+};
 
 struct GridArgs {
     GraphView g;
@@ -552,10 +555,10 @@ int ddp_ga_stats(ddp_ctx *ctx, const GaGridState &S, unsigned long long out[4])
     return DDP_OK;
 }
 
-// Runs the GA* loop from the current state (fresh or resumed).
-int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
-               const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
-               int32_t *rr_out, int32_t *iters_out, float *kernel_ms)
+namespace {
+
+GridArgs grid_args(const GraphView &gv, const HeurView &hv, const GaGridPlan &P, const GaGridState &S, int32_t t,
+                   uint32_t F0, int32_t rr0)
 {
     GridArgs a{};
     a.g = gv;
@@ -575,19 +578,48 @@ int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGr
     a.F0 = F0;
     a.rr0 = rr0;
     a.barrier_ticks = 100ull * 1000 * 1000 * 20;  // 20 s of the 100 MHz clock
+    return a;
+}
+
+// Ordinary launch: ddp_ga_plan sized the grid to what the device holds at
+// once and the grid barrier is software with bounded waits (a cooperative
+// launch would also reserve the global wave-sync unit, one process only).
+int grid_enqueue(ddp_ctx *ctx, GridArgs &a, const GaGridPlan &P, int hk)
+{
     void *args[] = {&a};
-    KernelFn fn = pick(P.B, hv.kind);
+    KernelFn fn = pick(P.B, hk);
     DDP_HIP(ctx, hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)((size_t)P.lcap * P.B * 8)));
+    DDP_HIP(ctx, hipLaunchKernel((const void *)fn, dim3(P.nblocks), dim3(P.B), args, (size_t)P.lcap * P.B * 8,
+                                 ctx->stream));
+    return DDP_OK;
+}
+
+}  // namespace
+
+int ddp_ga_launch(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+                  const GaGridState &S, int32_t N, int32_t s, uint32_t hs, int32_t t)
+{
+    DDP_CHECK(!P.wg, ctx, DDP_ERR_STATE, "GA*: not a grid plan");
+    int rc;
+    if ((rc = ddp_ga_reset(ctx, N, P, S))) return rc;
+    if ((rc = ddp_ga_seed(ctx, P, S, s, hs))) return rc;  // also clears the barrier and the result words
+    GridArgs a = grid_args(gv, hv, P, S, t, DDP_INF_F, 0);
+    return grid_enqueue(ctx, a, P, hv.kind);
+}
+
+// Runs the GA* loop from the current state (fresh or resumed).
+int ddp_ga_run(ddp_ctx *ctx, const GraphView &gv, const HeurView &hv, const GaGridPlan &P,
+               const GaGridState &S, int32_t t, uint32_t F0, int32_t rr0, uint32_t *F_out,
+               int32_t *rr_out, int32_t *iters_out, float *kernel_ms)
+{
+    GridArgs a = grid_args(gv, hv, P, S, t, F0, rr0);
     // a previous launch that timed out may have left the barrier mid-count
     DDP_HIP(ctx, hipMemsetAsync(S.ctrl + GA_BAR, 0, 8, ctx->stream));
     DDP_HIP(ctx, hipMemsetAsync(S.ctrl + GA_RES_ERR, 0, 4, ctx->stream));
     DDP_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    // Ordinary launch: ddp_ga_plan sized the grid to what the device holds at
-    // once and the grid barrier is software with bounded waits (a cooperative
-    // launch would also reserve the global wave-sync unit, one process only).
-    DDP_HIP(ctx, hipLaunchKernel((const void *)fn, dim3(P.nblocks), dim3(P.B), args, (size_t)P.lcap * P.B * 8,
-                                 ctx->stream));
+    int rc = grid_enqueue(ctx, a, P, hv.kind);
+    if (rc) return rc;
     DDP_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t res[4];
     DDP_HIP(ctx, hipMemcpyAsync(res, S.ctrl + GA_RES_IT, 16, hipMemcpyDeviceToHost, ctx->stream));

@@ -19,6 +19,14 @@
 //
 // Every device wait is bounded; idle workgroups exit after idle_ms and the host
 // relaunches the kernel when new requests arrive.
+//
+// Grid mode at low load (opts.grid_k): a query submitted while no other
+// request is in flight is searched by the single-query GA* grid kernel (K lane
+// heaps over the whole GPU, gastar.hip) instead of one slot.  Its ring entry
+// is marked so that the serve kernel skips it, and its search runs on the
+// server's stream after any serve kernel (an idle one is told to stop) and
+// before the next launch; a one-lane kernel then writes its result into the
+// ring like a slot would.
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -32,6 +40,7 @@
 #include "../../include/ddp_server.h"
 #include "bidi_search.h"
 #include "compact.h"
+#include "gastar.h"
 
 namespace {
 
@@ -43,6 +52,7 @@ struct Req {       // pinned, written by the host
     int32_t ins_off, ins_n;  // updates: (u,v,w,entry) quads, sorted by u
     int32_t rins_off;        // bidirectional engine: (v,entry) pairs, sorted by v (ins_n of them)
     int32_t relax_min;       // queries: every weight at this version is >= 1
+    int32_t grid;            // queries: searched in grid mode, skipped by the serve kernel
 };
 
 struct Res {       // pinned, written by the device
@@ -55,7 +65,10 @@ struct Res {       // pinned, written by the device
 
 enum : uint32_t { SERR_WAIT = 16, SERR_LOG = 32 };
 // ctrl words, one 64-byte line apart: claimed count, device copy of the host's
-// published count, applied version, poller lease, sticky error, stop flag copy.
+// published count, applied version, poller lease, sticky error, copy of the
+// host's stop word.  Launches are numbered (gen) and the stop word holds the
+// newest launch told to stop, so a launch exits once it is >= its own number
+// and no flag is ever cleared while a kernel might still read it.
 // Counts are unsigned and compared by difference, so they may wrap.
 enum : int { CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_ERR = 64, CT_STOP = 80, CT_WORDS = 96 };
 
@@ -93,6 +106,7 @@ struct ServeArgs {
     uint32_t ncap, fcap, delta0;  // pile engine: near ring, far buffers, first window
     uint32_t lring;               // pile engine: entries of the LDS near ring (dynamic LDS)
     uint64_t idle_ticks, wait_ticks;
+    uint32_t gen;                 // this launch's number
 };
 
 enum { SENG_HEAPS = 0, SENG_PILES = 1, SENG_BIDI = 2 };
@@ -229,14 +243,15 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                     }
                     continue;
                 }
-                if (ld_u32(a.ctrl + CT_STOP) || dev_clock_100mhz() - t0 > a.idle_ticks) break;
+                if ((int32_t)(ld_u32(a.ctrl + CT_STOP) - a.gen) >= 0 || dev_clock_100mhz() - t0 > a.idle_ticks)
+                    break;
                 if (!poller) poller = atomicCAS(a.ctrl + CT_POLL, 0u, 1u) == 0u;
                 if (poller) {
                     // the lease holder is the only writer of the tail copy
                     const uint32_t ht = __hip_atomic_load(a.tail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                     if ((int32_t)(ht - tail) > 0) st_u32(a.ctrl + CT_TAIL, ht);
-                    if (__hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
-                        atomicExch(a.ctrl + CT_STOP, 1u);
+                    const uint32_t sg = __hip_atomic_load(a.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    if ((int32_t)(sg - a.gen) >= 0) atomicExch(a.ctrl + CT_STOP, sg);
                     __builtin_amdgcn_s_sleep(16);
                 } else {
                     backoff = backoff < 8 ? backoff + 1 : 8;
@@ -266,6 +281,10 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         rq.ins_n = sys_ld_i(&rp->ins_n);
         rq.rins_off = sys_ld_i(&rp->rins_off);
         rq.relax_min = sys_ld_i(&rp->relax_min);
+        if (sys_ld_i(&rp->grid)) {  // searched in grid mode, which writes its result
+            __syncthreads();        // every lane has read s_ticket before lane 0 claims again
+            continue;
+        }
         const uint64_t tstart = dev_clock_100mhz();
         // ---- wait for the graph version this request needs (bounded)
         const uint32_t need = rq.type == DDP_REQ_UPDATE ? (uint32_t)rq.version - 1 : (uint32_t)rq.version;
@@ -354,6 +373,29 @@ __global__ void serve_fill_i32(int32_t *p, int64_t n, int32_t v)
         p[i] = v;
 }
 
+// Grid-mode queries: the device clock when the query's work starts, and its
+// result written into the ring entry (GA_RES_F is the f of t, its cost, as
+// h(t) = 0 for every closed-form heuristic).
+__global__ void serve_grid_begin(uint64_t *stamp)
+{
+    if (threadIdx.x == 0) *stamp = dev_clock_100mhz();
+}
+
+__global__ void serve_grid_finish(const uint32_t *gctrl, const uint64_t *stamp, Res *r, uint32_t ticket,
+                                  int32_t version)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t err = gctrl[GA_RES_ERR];
+    const int32_t status = ((err & (GA_ERR_HEAP | GA_ERR_PBUF)) ? 1 : 0) | ((err & GA_ERR_ITERS) ? 2 : 0) |
+                           ((err & GA_ERR_BARRIER) ? 8 : 0);
+    __hip_atomic_store(&r->cost, (int32_t)gctrl[GA_RES_F], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&r->status, status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&r->version, version, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&r->t0, *stamp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&r->t1, dev_clock_100mhz(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&r->seq, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 using ServeFn = void (*)(ServeArgs);
 
 template <int B, int ENG>
@@ -401,6 +443,7 @@ struct ddp_server {
     ddp_ctx *ctx = nullptr;
     int device = 0;
     const ddp_graph *g = nullptr;
+    ddp_heur_t h{};
     HeurView hv{};
     int32_t B = 0, slots = 0, lcap = 0, cap = 0, maxit = 0;
     int32_t eng = SENG_PILES;
@@ -446,6 +489,15 @@ struct ddp_server {
     clk::time_point t_create;
     hipEvent_t done_ev = nullptr;
     bool running = false;
+    uint32_t gen = 0;              // number of the newest launch
+    // grid mode for queries submitted while nothing is in flight
+    bool grid_on = false;
+    GaGridPlan gplan{};
+    GaGridState gstate{};
+    uint64_t *gstamp = nullptr;    // device clock at the start of the current grid query
+    hipEvent_t grid_ev = nullptr;  // after the newest grid query's result
+    std::vector<uint8_t> grid_of;  // per ring entry: searched in grid mode
+    int64_t requests = 0, grid_queries = 0;
 };
 
 static double ms_now(const ddp_server *sv)
@@ -465,19 +517,36 @@ static void server_free(ddp_server *sv)
     for (void *p : host)
         if (p) hipHostFree(p);
     if (sv->done_ev) hipEventDestroy(sv->done_ev);
+    ddp_ga_free(&sv->gstate);
+    if (sv->gstamp) hipFree(sv->gstamp);
+    if (sv->grid_ev) hipEventDestroy(sv->grid_ev);
+}
+
+// Base rows (the caller's graph, or the merged rows after a compaction) with
+// the versioned fields; queries set ver and relax_min.
+static GraphView server_view(const ddp_server *sv)
+{
+    GraphView v = ddp_view(sv->g);
+    if (sv->own.off) {
+        v.off = sv->own.off;
+        v.col = sv->own.col;
+        v.w = sv->own.w;
+    }
+    v.relax_min = sv->base_relax_min;
+    v.dver = sv->dver;
+    v.lhead = sv->lhead;
+    v.lnext = sv->lnext;
+    v.lv = sv->lv;
+    v.lw = sv->lw;
+    v.ldel = sv->ldel;
+    return v;
 }
 
 static int server_launch(ddp_server *sv)
 {
     ddp_ctx *ctx = sv->ctx;
     ServeArgs a{};
-    a.g = ddp_view(sv->g);
-    if (sv->own.off) {  // compacted base rows
-        a.g.off = sv->own.off;
-        a.g.col = sv->own.col;
-        a.g.w = sv->own.w;
-    }
-    a.g.relax_min = sv->base_relax_min;
+    a.g = server_view(sv);
     if (sv->eng == SENG_BIDI) {
         a.gr = a.g;
         a.gr.off = sv->own_r.off ? sv->own_r.off : sv->g->roff;
@@ -490,12 +559,6 @@ static int server_launch(ddp_server *sv)
         a.rlnext = sv->rlnext;
         a.rltail = sv->rltail;
     }
-    a.g.dver = sv->dver;
-    a.g.lhead = sv->lhead;
-    a.g.lnext = sv->lnext;
-    a.g.lv = sv->lv;
-    a.g.lw = sv->lw;
-    a.g.ldel = sv->ldel;
     a.h = sv->hv;
     a.dver = sv->dver;
     a.lhead = sv->lhead;
@@ -526,6 +589,7 @@ static int server_launch(ddp_server *sv)
     a.lring = sv->lring;
     a.idle_ticks = (uint64_t)sv->idle_ms * 100000ull;
     a.wait_ticks = 100000ull * 60000;  // a version wait longer than a minute is an error
+    a.gen = ++sv->gen;
     void *args[] = {&a};
     DDP_HIP(ctx, hipLaunchKernel((const void *)pick_serve(sv->B, sv->hv.kind, sv->eng), dim3(sv->slots), dim3(sv->B),
                                  args, sv->lds, ctx->stream));
@@ -544,9 +608,22 @@ static int server_ensure(ddp_server *sv)
         sv->running = false;
     }
     // with every workgroup gone, a request without a result was never claimed
+    // (grid-mode queries get theirs from their own kernels)
     for (uint32_t k = sv->collected; k != sv->submitted; ++k)
-        if (__atomic_load_n(&sv->res_h[k & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) != k + 1) return server_launch(sv);
+        if (!sv->grid_of[k & (sv->ring - 1)] &&
+            __atomic_load_n(&sv->res_h[k & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) != k + 1)
+            return server_launch(sv);
     return DDP_OK;
+}
+
+// Every submitted request has its result: nothing runs or waits on the device.
+static bool server_quiet(const ddp_server *sv)
+{
+    const uint32_t mask = sv->ring - 1;
+    for (uint32_t k = sv->collected; k != sv->submitted; ++k)
+        if (!sv->delivered[k & mask] && __atomic_load_n(&sv->res_h[k & mask].seq, __ATOMIC_ACQUIRE) != k + 1)
+            return false;
+    return true;
 }
 
 extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *o,
@@ -564,10 +641,13 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     DDP_CHECK(engine >= DDP_SERVER_AUTO && engine <= DDP_SERVER_BIDI, ctx, DDP_ERR_ARG, "unknown engine %d", engine);
     DDP_CHECK(engine != DDP_SERVER_BIDI || (g->roff && g->reid), ctx, DDP_ERR_STATE,
               "the bidirectional engine needs a graph uploaded with DDP_GRAPH_REVERSE");
+    const int32_t grid_k = o ? o->grid_k : 0;
+    DDP_CHECK(grid_k >= 0, ctx, DDP_ERR_ARG, "grid_k must be >= 0 (got %d)", grid_k);
     ddp_server *sv = new ddp_server();
     sv->ctx = ctx;
     sv->device = ctx->device;
     sv->g = g;
+    sv->h = h;
     sv->hv = hv;
     sv->N = g->N;
     sv->E = g->E;
@@ -607,6 +687,20 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
         return fail(ddp_set_err(ctx, DDP_ERR_ARG, "block must be 64..1024 and ring a power of two >= 64"));
     const void *fn = (const void *)pick_serve(sv->B, hv.kind, sv->eng);
     if (!fn) return fail(ddp_set_err(ctx, DDP_ERR_ARG, "no serving kernel for block %d and this engine", sv->B));
+    if (grid_k > 0) {
+        // the grid-mode state is allocated first: slots take what HBM is left
+        ddp_astar_opts_t ao{};
+        ao.K = grid_k;
+        if ((rc = ddp_ga_plan(ctx, g, &ao, &sv->gplan))) return fail(rc);
+        // room for 8N queued entries over the K heaps (a hub-heavy graph pushes
+        // each node several times over a full search)
+        sv->gplan.cap = (int32_t)std::max<int64_t>(sv->gplan.cap, (8LL * g->N + sv->gplan.K - 1) / sv->gplan.K);
+        if ((rc = ddp_ga_alloc_owned(ctx, g->N, sv->gplan, &sv->gstate))) return fail(rc);
+        hipError_t ge = hipMalloc((void **)&sv->gstamp, 64);
+        if (ge == hipSuccess) ge = hipEventCreateWithFlags(&sv->grid_ev, hipEventDisableTiming);
+        if (ge != hipSuccess) return fail(ddp_hip_err(ctx, ge, "grid-mode state", __FILE__, __LINE__));
+        sv->grid_on = true;
+    }
     int per_cu = 0;
     sv->lring = piles ? 8u * (uint32_t)sv->B : 0u;  // LDS near ring, as for batches (half per direction: bidi)
     sv->lds = piles ? (size_t)sv->lring * 8 : (size_t)sv->lcap * sv->B * 8;
@@ -659,6 +753,7 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     hal((void **)&sv->arena_h, (void **)&sv->arena_d, sv->arena_cap * 4);
     hal((void **)&sv->flags_h, (void **)&sv->flags_d, 64);
     sv->delivered.assign(sv->ring, 0);
+    sv->grid_of.assign(sv->ring, 0);
     sv->tags.assign(sv->ring, 0);
     sv->types.assign(sv->ring, 0);
     sv->submit_ms.assign(sv->ring, 0.0);
@@ -691,10 +786,23 @@ extern "C" int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *bl
     return DDP_OK;
 }
 
+extern "C" int ddp_server_stats(const ddp_server *sv, ddp_server_stats_t *out)
+{
+    if (!sv || !out) return DDP_ERR_ARG;
+    out->requests = sv->requests;
+    out->grid_queries = sv->grid_queries;
+    out->launches = (int32_t)sv->gen;
+    out->compactions = sv->compactions;
+    return DDP_OK;
+}
+
 static void publish(ddp_server *sv, const Req &r, uint64_t tag, int32_t type)
 {
     const uint32_t k = sv->submitted & (sv->ring - 1);
     sv->ring_h[k] = r;
+    sv->grid_of[k] = r.grid ? 1 : 0;
+    sv->requests += 1;
+    sv->grid_queries += r.grid ? 1 : 0;
     Res *res = sv->res_h + k;
     // cleared to the ticket itself: never equal to ticket + 1, even when it wraps
     __atomic_store_n(&res->seq, sv->submitted, __ATOMIC_RELAXED);
@@ -703,6 +811,45 @@ static void publish(ddp_server *sv, const Req &r, uint64_t tag, int32_t type)
     sv->submit_ms[k] = ms_now(sv);
     ++sv->submitted;
     __atomic_store_n(&sv->flags_h[0], sv->submitted, __ATOMIC_RELEASE);
+}
+
+// A query submitted while nothing is in flight: grid mode.  The entry is
+// published (its result word cleared) before any of its kernels is queued,
+// so the result they write cannot be overwritten.
+static int server_grid_query(ddp_server *sv, Req r, uint64_t tag)
+{
+    ddp_ctx *ctx = sv->ctx;
+    HeurView hq;  // the heuristic aimed at this query's target
+    int rc = ddp_make_heur(sv->g, sv->h, r.t, &hq);
+    if (rc) return rc;
+    int32_t hs = 0;
+    if ((rc = ddp_heur_at(sv->g, hq, r.s, &hs))) return rc;
+    GraphView gv = server_view(sv);
+    gv.ver = r.version;
+    gv.relax_min = sv->base_relax_min && r.relax_min;
+    // an idle serve kernel would hold the stream for idle_ms: tell it to stop
+    if (sv->running) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
+    r.grid = 1;
+    const uint32_t ticket = sv->submitted;
+    publish(sv, r, tag, DDP_REQ_QUERY);
+    hipLaunchKernelGGL(serve_grid_begin, dim3(1), dim3(64), 0, ctx->stream, sv->gstamp);
+    rc = ddp_ga_launch(ctx, gv, hq, sv->gplan, sv->gstate, sv->N, r.s, (uint32_t)hs, r.t);
+    if (!rc)
+        hipLaunchKernelGGL(serve_grid_finish, dim3(1), dim3(64), 0, ctx->stream, sv->gstate.ctrl, sv->gstamp,
+                           sv->res_d + (ticket & (sv->ring - 1)), ticket, r.version);
+    const hipError_t e = rc ? hipSuccess : hipGetLastError();
+    if (rc || e != hipSuccess) {
+        // nothing will write this entry's result: fail it from here
+        Res *res = sv->res_h + (ticket & (sv->ring - 1));
+        res->cost = DDP_INF_COST;
+        res->status = 8;
+        res->version = r.version;
+        res->t0 = res->t1 = 0;
+        __atomic_store_n(&res->seq, ticket + 1, __ATOMIC_RELEASE);
+        return rc ? rc : ddp_hip_err(ctx, e, "grid-mode query", __FILE__, __LINE__);
+    }
+    DDP_HIP(ctx, hipEventRecord(sv->grid_ev, ctx->stream));
+    return DDP_OK;
 }
 
 extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uint64_t tag)
@@ -717,6 +864,7 @@ extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uin
     r.t = ddp_to_dev(sv->g, t);
     r.version = sv->version;
     r.relax_min = sv->version < sv->zero_w_version;  // the base graph's own minimum is in a.g
+    if (sv->grid_on && r.s != r.t && server_quiet(sv)) return server_grid_query(sv, r, tag);
     publish(sv, r, tag, DDP_REQ_QUERY);
     return server_ensure(sv);
 }
@@ -733,23 +881,13 @@ static int server_compact(ddp_server *sv)
     ddp_ctx *ctx = sv->ctx;
     int rc = ddp_server_drain(sv, 600000);
     if (rc) return rc;
-    if (sv->running) {  // idle slots exit on the stop flag
-        __atomic_store_n(&sv->flags_h[1], 1u, __ATOMIC_RELEASE);
+    if (sv->running) {  // idle slots exit on the stop word
+        __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
         DDP_HIP(ctx, hipEventSynchronize(sv->done_ev));
         sv->running = false;
     }
-    __atomic_store_n(&sv->flags_h[1], 0u, __ATOMIC_RELEASE);
-    const uint32_t zero = 0;
-    DDP_HIP(ctx, hipMemcpy(sv->ctrl + CT_STOP, &zero, 4, hipMemcpyHostToDevice));
-    const GraphView base = [&] {
-        GraphView v = ddp_view(sv->g);
-        if (sv->own.off) {
-            v.off = sv->own.off;
-            v.col = sv->own.col;
-            v.w = sv->own.w;
-        }
-        return v;
-    }();
+    DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));  // and any grid-mode query
+    const GraphView base = server_view(sv);
     const MergeSrc m{base.off, base.col, base.w, sv->N, sv->dver, sv->version, sv->lhead, sv->lnext,
                      sv->lv, sv->lw, sv->ldel};
     MergedCsr f, fr;
@@ -927,8 +1065,9 @@ extern "C" void ddp_server_destroy(ddp_server *sv)
 {
     if (!sv) return;
     hipSetDevice(sv->device);
-    if (sv->flags_h) __atomic_store_n(&sv->flags_h[1], 1u, __ATOMIC_RELEASE);
+    if (sv->flags_h) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
     if (sv->running && sv->done_ev) hipEventSynchronize(sv->done_ev);
+    if (sv->grid_ev) hipEventSynchronize(sv->grid_ev);  // a grid-mode query still running
     server_free(sv);
     delete sv;
 }

@@ -44,7 +44,13 @@ typedef struct {
                                that does not fit triggers ddp_server_compact first */
     int32_t idle_ms;        /* the kernel exits after this long without work; 0 = 2000 */
     int32_t engine;         /* DDP_SERVER_AUTO / _HEAPS / _PILES (as DDP_BATCH_*) */
-    int32_t reserved;
+    int32_t grid_k;         /* 0 = off.  K > 0: a query submitted while no other request
+                               is in flight is searched in grid mode (ddp_astar's
+                               DDP_ASTAR_GRID with K lane heaps over the whole GPU) instead
+                               of one slot; requests arriving meanwhile start in slots
+                               after it.  Worth it where one query is a large share of the
+                               graph (a full search of a power-law graph): alone on the
+                               GPU it finishes many times sooner than in one slot */
 } ddp_server_opts_t;
 
 /* Queue discipline of a query slot, as for batches (include/ddp_batch.h):
@@ -67,6 +73,7 @@ typedef struct {
     int32_t cost;           /* queries: Cx of t, DDP_INF_COST if unreachable */
     int32_t version;        /* graph version the request saw (queries) or made (updates) */
     int32_t status;         /* 0 ok; 1 heap_cap exceeded; 2 iteration limit; 4 bad id;
+                               8 a grid-mode search's device-wide barrier wait ran out;
                                16 an update's wait for the previous version ran out;
                                32 an update carried an invalid log entry.  16 and 32
                                are sticky: once an update fails, every later request
@@ -79,6 +86,15 @@ typedef struct {
 int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *opts,
                       ddp_server **out);
 int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_t *version);
+
+typedef struct {
+    int64_t requests;       /* submitted so far */
+    int64_t grid_queries;   /* of them, queries searched in grid mode (grid_k) */
+    int32_t launches;       /* serve-kernel launches (the first request, and relaunches
+                               after an idle exit, a grid-mode query or a compaction) */
+    int32_t compactions;    /* inserted-edge log compactions */
+} ddp_server_stats_t;
+int ddp_server_stats(const ddp_server *sv, ddp_server_stats_t *out);
 /* Non-blocking; DDP_ERR_NOMEM if the ring (or, for updates, the payload arena)
  * is full: try again after polling.  An update whose inserts do not fit in
  * the rest of the inserted-edge log first compacts it (ddp_server_compact,

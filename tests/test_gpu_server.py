@@ -430,3 +430,97 @@ def test_server_bidi_compaction(ctx):
     for tag, (v, want) in checks.items():
         assert got[tag]["version"] == v and got[tag]["cost"] == want, (tag, got[tag], want)
     sv.close()
+
+
+def grid_mode_session(ctx, hg, dg, heur, engine, block, grid_k, src, dst, ub, max_inserts):
+    """Lone queries (each submitted with nothing in flight: grid mode), bursts
+    (the first alone, the rest while it runs: slots), update batches between
+    them and a log small enough to be compacted.  Returns {tag: result} and
+    {tag: (version, s, t)}."""
+    sv = ddp_amd.Server(ctx, dg, heur=heur, block=block, slots=8, engine=engine, grid_k=grid_k,
+                        max_inserts=max_inserts)
+    got, plan = {}, {}
+    qi, v = 0, 0
+
+    def query(tag):
+        nonlocal qi
+        s, t = int(src[qi % len(src)]), int(dst[qi % len(dst)])
+        qi += 1
+        assert sv.submit_query(s, t, tag) == 0
+        plan[tag] = (v, s, t)
+
+    def settle():
+        sv.drain(120000)
+        for r in sv.poll(4096):
+            got[r["tag"]] = r
+
+    for b in range(len(ub) + 1):
+        for k in range(3):  # one at a time
+            query(1000 * b + k)
+            settle()
+        for k in range(3, 8):  # a burst
+            query(1000 * b + k)
+        settle()
+        if b < len(ub):
+            assert sv.submit_update(ub[b], 10**6 + b) == 0
+            v += 1
+            settle()
+    st = sv.stats()
+    say("  grid mode session: stats", st)
+    sv.close()
+    return got, plan, st
+
+
+def test_server_grid_mode_at_low_load(ctx):
+    """grid_k: a query submitted while no other request is in flight runs in
+    grid mode and one submitted meanwhile in a slot; either way its cost is
+    Dijkstra's on the graph version it was bound to -- across update batches,
+    compactions and an idle serve kernel told to stop."""
+    hg = ddp_amd.HostGraph.random(6000, 5, 1, 100, 51)
+    src, dst = hg.reachable_pairs(80, 52)
+    ub = hg.gen_updates(6, 40, 40, 1, 100, 53).batches
+    vers = versions(hg, ub)
+    for engine, block, grid_k, reverse in ((ddp_amd.ENGINE_PILES, 1024, 1024, False),
+                                           (ddp_amd.ENGINE_BIDI, 256, 4096, True)):
+        say("grid mode: engine", engine, "block", block, "K", grid_k)
+        dg = ddp_amd.DeviceGraph(ctx, hg, reverse=reverse)
+        got, plan, st = grid_mode_session(ctx, hg, dg, ddp_amd.H_ZERO, engine, block, grid_k, src, dst, ub, 100)
+        dg.close()
+        assert len(got) == len(plan) + len(ub)
+        cache = {}
+        for tag, (v, s, t) in plan.items():
+            r = got[tag]
+            if (v, s) not in cache:
+                cache[(v, s)], _ = pyoracle.dijkstra(*vers[v], s)
+            d = cache[(v, s)][t]
+            assert r["status"] == 0 and r["version"] == v, (tag, r)
+            assert r["cost"] == (int(d) if d >= 0 else ddp_amd.INF_COST), (tag, r, d)
+            assert r["device_ms"] > 0
+        lone = 3 * (len(ub) + 1)
+        # every lone query, and the first of each burst, ran in grid mode; the
+        # rest of each burst in slots (a burst's first query can only finish
+        # before the next submit on a tiny graph, so allow that)
+        assert lone + (len(ub) + 1) <= st["grid_queries"] < len(plan), st
+        assert st["compactions"] >= 1 and st["launches"] >= len(ub) + 1, st
+
+
+def test_server_grid_mode_euclid(ctx):
+    """Grid mode aims the closed-form heuristic at each query's own target:
+    Euclidean A* on a relabelled geometric graph with inserts that keep it
+    admissible, checked against Dijkstra on every version."""
+    hg = ddp_amd.HostGraph.geometric(20000, 10, 100000, 54)
+    dg = ddp_amd.DeviceGraph(ctx, hg, reorder=True)
+    ub = geometric_updates(hg, 3, 200, 55)
+    src, dst = hg.geo_pairs(40, 500, 100000, 56)
+    got, plan, st = grid_mode_session(ctx, hg, dg, ddp_amd.H_EUCLID, ddp_amd.ENGINE_PILES, 1024, 2048, src, dst,
+                                      ub, 0)
+    dg.close()
+    vers = versions(hg, ub)
+    cache = {}
+    for tag, (v, s, t) in plan.items():
+        r = got[tag]
+        if (v, s) not in cache:
+            cache[(v, s)], _ = pyoracle.dijkstra(*vers[v], s)
+        d = cache[(v, s)][t]
+        assert r["status"] == 0 and r["version"] == v and r["cost"] == int(d), (tag, r, d)
+    assert st["grid_queries"] >= 3 * (len(ub) + 1), st
