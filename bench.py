@@ -145,10 +145,10 @@ def geometric_updates(hg, n, per, seed):
     return out
 
 
-def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=0, block=0, engine=0):
+def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=0, block=0, engine=0, grid_k=0):
     """Open-loop replay: Poisson arrivals at `qps` (all at once when qps <= 0),
     95% point queries and 5% update batches.  Returns (results, types, query
-    index per request, secs, server info)."""
+    index per request, secs, server info with the server's counters)."""
     import ddp_amd
     rng = np.random.default_rng(seed)
     types = (rng.random(nreq) < 0.05).astype(np.int32)
@@ -160,11 +160,12 @@ def serve_trace(ctx, dg, heur, src, dst, qps, nreq, upd_batches, seed, heap_cap=
         ups.append(upd_batches[k] if ty else None)
         k += int(ty)
     arrival = np.zeros(nreq) if qps <= 0 else np.cumsum(rng.exponential(1000.0 / qps, nreq))
-    sv = ddp_amd.Server(ctx, dg, heur=heur, heap_cap=heap_cap, block=block, engine=engine)
+    sv = ddp_amd.Server(ctx, dg, heur=heur, heap_cap=heap_cap, block=block, engine=engine, grid_k=grid_k)
     t0 = time.perf_counter()
     out = sv.replay(arrival, types, src[qi], dst[qi], ups)
     el = time.perf_counter() - t0
     info = sv.info()
+    info.update(sv.stats())
     sv.close()
     return out, types, qi, el, info
 
@@ -180,9 +181,13 @@ def served_summary(out, types, qi, el, info, qps, costs=None):
             if out[i]["cost"] != costs[qi[i]]:
                 raise SystemExit(f"served query {i} disagrees with the batch: {out[i]['cost']} vs {costs[qi[i]]}")
     lat = [r["done_ms"] - r["submit_ms"] for r in q]
-    return {"offered_qps": qps, "requests": len(out), "updates": int(types.sum()), "slots": info["slots"],
-            "queries_per_s": round(len(q) / el, 1), "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+    summ = {"offered_qps": qps, "requests": len(out), "updates": int(types.sum()), "slots": info["slots"],
+            "queries_per_s": round(len(q) / el, 1), "p50_ms": pct(lat, 50), "p90_ms": pct(lat, 90),
+            "p99_ms": pct(lat, 99),
             "device_p50_ms": pct([r["device_ms"] for r in q], 50)}
+    if info.get("grid_queries"):
+        summ["grid_queries"] = info["grid_queries"]
+    return summ
 
 
 def run_c4(args):
@@ -563,6 +568,20 @@ def run_c5(args):
         log(f"C5 served {sweep[-1]}")
     under = [x for x in sweep if 0 < x["offered_qps"] < cap] or sweep
     lat_at = max(under, key=lambda x: x["offered_qps"])
+    # low load: the same open-loop trace with and without grid mode for
+    # queries that arrive while nothing is in flight
+    low = []
+    for f in args.c5_low_load:
+        qps = round(f * cap, 1)
+        pair = {"offered_qps": qps}
+        for name, gk in (("slots", 0), ("grid", args.c5_grid_k)):
+            o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_low_requests, ups,
+                                               13, heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine,
+                                               grid_k=gk)
+            pair[name] = served_summary(o_, ty, qi, el, info, qps, cost_b)
+        pair["grid_k"] = args.c5_grid_k
+        low.append(pair)
+        log(f"C5 low load {pair}")
     best_batched = max(v["queries_per_s"] for v in batched.values())
     res = base_record(args, cap, "queries/s", None,
                       f"C5 online serving: R-MAT scale {scale} ({hg.N:,} nodes, {hg.E:,} edges, a/b/c/d = "
@@ -573,7 +592,7 @@ def run_c5(args):
                           "serve_block": info["block"],
                           "serve_engine": ENGINE_NAMES.get(args.c5_serve_engine, "piles")})
     res.update({"p50_ms": lat_at["p50_ms"], "p99_ms": lat_at["p99_ms"], "latency_at_qps": lat_at["offered_qps"],
-                "sweep": sweep, "batched": batched, "batched_queries_per_s": best_batched,
+                "sweep": sweep, "low_load": low, "batched": batched, "batched_queries_per_s": best_batched,
                 "one_at_a_time": ({"queries_per_s": round(single_qps, 3), "p50_ms": pct(lat1, 50),
                                    "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": 8192} if lat1 else None),
                 "batched_over_one_at_a_time": round(best_batched / single_qps, 2) if single_qps else None,
@@ -594,8 +613,8 @@ def summary_c3(r):
 
 
 def summary_c5(r):
-    keep = ("value", "unit", "p50_ms", "p99_ms", "latency_at_qps", "sweep", "batched", "batched_queries_per_s",
-            "one_at_a_time", "batched_over_one_at_a_time", "served_over_one_at_a_time")
+    keep = ("value", "unit", "p50_ms", "p99_ms", "latency_at_qps", "sweep", "low_load", "batched",
+            "batched_queries_per_s", "one_at_a_time", "batched_over_one_at_a_time", "served_over_one_at_a_time")
     out = {k: r[k] for k in keep}
     out["config"] = r["config"]
     return out
@@ -652,6 +671,11 @@ def main():
     ap.add_argument("--c5-batch-only", action="store_true", help="c5: the batched launches only (for profiling)")
     ap.add_argument("--c5-engine", default="", help="c5: batch engine to run (piles or bidi; default both)")
     ap.add_argument("--c5-serve-engine", type=int, default=0, help="c5: serving engine (0 = library default)")
+    ap.add_argument("--c5-low-load", type=float, nargs="*", default=[0.025, 0.05],
+                    help="c5: fractions of the closed-loop capacity for the low-load points, each served with "
+                         "and without grid mode")
+    ap.add_argument("--c5-low-requests", type=int, default=64, help="c5: requests per low-load point")
+    ap.add_argument("--c5-grid-k", type=int, default=8192, help="c5: grid-mode lanes of the low-load points")
     ap.add_argument("--no-extra", action="store_true",
                     help="c4 only: skip the C3 and C5 legs that the default line carries as extra keys")
     args = ap.parse_args()

@@ -85,7 +85,8 @@ class ServerOpts(ctypes.Structure):
 
 
 class ServerStats(ctypes.Structure):
-    _fields_ = [("requests", c_int64), ("grid_queries", c_int64), ("launches", c_int32), ("compactions", c_int32)]
+    _fields_ = [("requests", c_int64), ("grid_queries", c_int64), ("direct_updates", c_int64), ("launches", c_int32),
+                ("compactions", c_int32)]
 
 
 class ServerResult(ctypes.Structure):

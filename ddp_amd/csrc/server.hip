@@ -52,7 +52,8 @@ struct Req {       // pinned, written by the host
     int32_t ins_off, ins_n;  // updates: (u,v,w,entry) quads, sorted by u
     int32_t rins_off;        // bidirectional engine: (v,entry) pairs, sorted by v (ins_n of them)
     int32_t relax_min;       // queries: every weight at this version is >= 1
-    int32_t grid;            // queries: searched in grid mode, skipped by the serve kernel
+    int32_t direct;          // handled by kernels of its own (a grid-mode query, or an update
+                             // applied while no slot runs), skipped by the serve kernel
 };
 
 struct Res {       // pinned, written by the device
@@ -66,11 +67,16 @@ struct Res {       // pinned, written by the device
 enum : uint32_t { SERR_WAIT = 16, SERR_LOG = 32 };
 // ctrl words, one 64-byte line apart: claimed count, device copy of the host's
 // published count, applied version, poller lease, sticky error, copy of the
-// host's stop word.  Launches are numbered (gen) and the stop word holds the
-// newest launch told to stop, so a launch exits once it is >= its own number
-// and no flag is ever cleared while a kernel might still read it.
+// host's stop word, busy word.  Launches are numbered (gen) and the stop word
+// holds the newest launch told to stop, so a launch exits once it is >= its
+// own number and no flag is ever cleared while a kernel might still read it.
+// The busy word counts slots running a request, plus kClosed once the launch
+// closes for lack of work (cleared by the host before each launch).
 // Counts are unsigned and compared by difference, so they may wrap.
-enum : int { CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_ERR = 64, CT_STOP = 80, CT_WORDS = 96 };
+enum : int {
+    CT_HEAD = 0, CT_TAIL = 16, CT_APPLIED = 32, CT_POLL = 48, CT_ERR = 64, CT_STOP = 80, CT_BUSY = 96, CT_WORDS = 112
+};
+constexpr uint32_t kClosed = 0x80000000u;
 
 struct ServeArgs {
     GraphView g;           // base rows + versioned view fields (ver set per query)
@@ -226,6 +232,18 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         // flag) into them with plain relaxed loads of host memory, so idle
         // slots neither flood the host link nor invalidate caches that the
         // running searches depend on.
+        //
+        // A slot counts itself busy before it claims, and the launch closes
+        // (every slot leaves) only by swapping a busy count of zero for
+        // kClosed: slots never leave one by one while others still run, which
+        // would strand later requests behind fewer slots until the last
+        // running one finished and the host could relaunch.
+        //
+        // The next request is claimed only once the graph version it needs is
+        // applied (entries handled by kernels of their own are skipped at
+        // once), so no slot holds a claim while it waits: the version may come
+        // from an update kernel queued behind this launch, which a stop or an
+        // idle close then lets run.
         if (lane == 0) {
             uint32_t ticket = 0;
             bool have = false;
@@ -235,15 +253,34 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             for (;;) {
                 const uint32_t h = ld_u32(a.ctrl + CT_HEAD);
                 const uint32_t tail = ld_u32(a.ctrl + CT_TAIL);
+                const uint32_t bw = ld_u32(a.ctrl + CT_BUSY);
+                if (bw & kClosed) break;
+                bool ready = false;
                 if ((int32_t)(tail - h) > 0) {
+                    // a stale h reads an entry that its failed claim below ignores
+                    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+                    const Req *hp = a.ring + (h & a.ring_mask);
+                    ready = sys_ld_i(&hp->direct) != 0;
+                    if (!ready) {
+                        const int32_t ver = sys_ld_i(&hp->version);
+                        const uint32_t need =
+                            sys_ld_i(&hp->type) == DDP_REQ_UPDATE ? (uint32_t)ver - 1 : (uint32_t)ver;
+                        ready = __hip_atomic_load(a.ctrl + CT_APPLIED, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >=
+                                need;
+                    }
+                }
+                if (ready) {
+                    if (atomicCAS(a.ctrl + CT_BUSY, bw, bw + 1) != bw) continue;
                     if (atomicCAS(a.ctrl + CT_HEAD, h, h + 1) == h) {
                         ticket = h;
                         have = true;
                         break;
                     }
+                    atomicSub(a.ctrl + CT_BUSY, 1u);
                     continue;
                 }
-                if ((int32_t)(ld_u32(a.ctrl + CT_STOP) - a.gen) >= 0 || dev_clock_100mhz() - t0 > a.idle_ticks)
+                if ((int32_t)(ld_u32(a.ctrl + CT_STOP) - a.gen) >= 0) break;
+                if (dev_clock_100mhz() - t0 > a.idle_ticks && bw == 0 && atomicCAS(a.ctrl + CT_BUSY, 0u, kClosed) == 0u)
                     break;
                 if (!poller) poller = atomicCAS(a.ctrl + CT_POLL, 0u, 1u) == 0u;
                 if (poller) {
@@ -281,7 +318,8 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         rq.ins_n = sys_ld_i(&rp->ins_n);
         rq.rins_off = sys_ld_i(&rp->rins_off);
         rq.relax_min = sys_ld_i(&rp->relax_min);
-        if (sys_ld_i(&rp->grid)) {  // searched in grid mode, which writes its result
+        if (sys_ld_i(&rp->direct)) {  // its own kernels write its result
+            if (lane == 0) atomicSub(a.ctrl + CT_BUSY, 1u);
             __syncthreads();        // every lane has read s_ticket before lane 0 claims again
             continue;
         }
@@ -355,6 +393,7 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
             __hip_atomic_store(&r->t0, tstart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&r->t1, dev_clock_100mhz(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             __hip_atomic_store(&r->seq, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            atomicSub(a.ctrl + CT_BUSY, 1u);
         }
         __syncthreads();
     }
@@ -371,6 +410,49 @@ __global__ void serve_fill_i32(int32_t *p, int64_t n, int32_t v)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
+}
+
+// Updates applied while no slot runs, by one workgroup launched for just that
+// entry (stream order puts it after every earlier request's work, so the
+// previous version is already applied).
+template <bool BIDI>
+__global__ void __launch_bounds__(1024) serve_update_kernel(ServeArgs a, uint32_t ticket)
+{
+    __shared__ uint32_t s_uerr;
+    const int lane = threadIdx.x;
+    const uint64_t tstart = dev_clock_100mhz();
+    const Req *rp = a.ring + (ticket & a.ring_mask);
+    Req rq{};
+    rq.type = sys_ld_i(&rp->type);
+    rq.version = sys_ld_i(&rp->version);
+    rq.del_off = sys_ld_i(&rp->del_off);
+    rq.del_n = sys_ld_i(&rp->del_n);
+    rq.ins_off = sys_ld_i(&rp->ins_off);
+    rq.ins_n = sys_ld_i(&rp->ins_n);
+    rq.rins_off = sys_ld_i(&rp->rins_off);
+    if (lane == 0) {
+        s_uerr = ld_u32(a.ctrl + CT_ERR);
+        if (__hip_atomic_load(a.ctrl + CT_APPLIED, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+            (uint32_t)rq.version - 1)
+            s_uerr |= SERR_WAIT;
+    }
+    __syncthreads();
+    (void)__hip_atomic_load(a.ctrl + CT_APPLIED, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    if (!s_uerr) apply_update<1024, BIDI>(a, rq, &s_uerr);
+    __syncthreads();
+    if (lane == 0) {
+        const uint32_t status = s_uerr;
+        if (status) atomicOr(a.ctrl + CT_ERR, status);
+        __threadfence();
+        atomicMax(a.ctrl + CT_APPLIED, (uint32_t)rq.version);
+        Res *r = a.res + (ticket & a.ring_mask);
+        __hip_atomic_store(&r->cost, (int32_t)DDP_INF_COST, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&r->status, (int32_t)status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&r->version, rq.version, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&r->t0, tstart, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&r->t1, dev_clock_100mhz(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&r->seq, ticket + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // Grid-mode queries: the device clock when the query's work starts, and its
@@ -495,9 +577,13 @@ struct ddp_server {
     GaGridPlan gplan{};
     GaGridState gstate{};
     uint64_t *gstamp = nullptr;    // device clock at the start of the current grid query
-    hipEvent_t grid_ev = nullptr;  // after the newest grid query's result
-    std::vector<uint8_t> grid_of;  // per ring entry: searched in grid mode
-    int64_t requests = 0, grid_queries = 0;
+    hipEvent_t direct_ev = nullptr;  // after the newest direct request's kernels
+    std::vector<uint8_t> direct_of;  // per ring entry: handled by kernels of its own
+    int64_t requests = 0, grid_queries = 0, direct_updates = 0;
+    double grid_ms = 0, slot_ms = 0;  // moving averages of finished queries' device times (0: none yet)
+    double gap_ms = 0, last_submit_ms = -1;  // long-run mean gap between submissions
+    int64_t gaps = 0;
+    bool grid_regime = false;  // arrivals leave grid mode room (with hysteresis)
 };
 
 static double ms_now(const ddp_server *sv)
@@ -519,7 +605,7 @@ static void server_free(ddp_server *sv)
     if (sv->done_ev) hipEventDestroy(sv->done_ev);
     ddp_ga_free(&sv->gstate);
     if (sv->gstamp) hipFree(sv->gstamp);
-    if (sv->grid_ev) hipEventDestroy(sv->grid_ev);
+    if (sv->direct_ev) hipEventDestroy(sv->direct_ev);
 }
 
 // Base rows (the caller's graph, or the merged rows after a compaction) with
@@ -542,9 +628,8 @@ static GraphView server_view(const ddp_server *sv)
     return v;
 }
 
-static int server_launch(ddp_server *sv)
+static ServeArgs server_args(const ddp_server *sv)
 {
-    ddp_ctx *ctx = sv->ctx;
     ServeArgs a{};
     a.g = server_view(sv);
     if (sv->eng == SENG_BIDI) {
@@ -589,8 +674,16 @@ static int server_launch(ddp_server *sv)
     a.lring = sv->lring;
     a.idle_ticks = (uint64_t)sv->idle_ms * 100000ull;
     a.wait_ticks = 100000ull * 60000;  // a version wait longer than a minute is an error
+    return a;
+}
+
+static int server_launch(ddp_server *sv)
+{
+    ddp_ctx *ctx = sv->ctx;
+    ServeArgs a = server_args(sv);
     a.gen = ++sv->gen;
     void *args[] = {&a};
+    DDP_HIP(ctx, hipMemsetAsync(sv->ctrl + CT_BUSY, 0, 4, ctx->stream));  // the last launch closed it
     DDP_HIP(ctx, hipLaunchKernel((const void *)pick_serve(sv->B, sv->hv.kind, sv->eng), dim3(sv->slots), dim3(sv->B),
                                  args, sv->lds, ctx->stream));
     DDP_HIP(ctx, hipEventRecord(sv->done_ev, ctx->stream));
@@ -610,20 +703,79 @@ static int server_ensure(ddp_server *sv)
     // with every workgroup gone, a request without a result was never claimed
     // (grid-mode queries get theirs from their own kernels)
     for (uint32_t k = sv->collected; k != sv->submitted; ++k)
-        if (!sv->grid_of[k & (sv->ring - 1)] &&
+        if (!sv->direct_of[k & (sv->ring - 1)] &&
             __atomic_load_n(&sv->res_h[k & (sv->ring - 1)].seq, __ATOMIC_ACQUIRE) != k + 1)
             return server_launch(sv);
     return DDP_OK;
 }
 
-// Every submitted request has its result: nothing runs or waits on the device.
-static bool server_quiet(const ddp_server *sv)
+// Requests without a result yet: queries and updates on the slot path, and
+// requests handled by kernels of their own (grid-mode queries, updates applied
+// directly); *remain: the longest a slot-path query is expected to run on,
+// from its submission and the slots' moving average.
+struct InFlight {
+    int64_t slot_q = 0, slot_u = 0, direct = 0;
+    double remain = 0;
+};
+
+static InFlight server_in_flight(const ddp_server *sv)
 {
+    InFlight f;
     const uint32_t mask = sv->ring - 1;
-    for (uint32_t k = sv->collected; k != sv->submitted; ++k)
-        if (!sv->delivered[k & mask] && __atomic_load_n(&sv->res_h[k & mask].seq, __ATOMIC_ACQUIRE) != k + 1)
-            return false;
-    return true;
+    const double now = ms_now(sv);
+    for (uint32_t k = sv->collected; k != sv->submitted; ++k) {
+        if (sv->delivered[k & mask] || __atomic_load_n(&sv->res_h[k & mask].seq, __ATOMIC_ACQUIRE) == k + 1)
+            continue;
+        if (sv->direct_of[k & mask]) {
+            ++f.direct;
+        } else if (sv->types[k & mask] == DDP_REQ_UPDATE) {
+            ++f.slot_u;
+        } else {
+            ++f.slot_q;
+            f.remain = std::max(f.remain, sv->submit_ms[k & mask] + sv->slot_ms - now);
+        }
+    }
+    return f;
+}
+
+// Grid mode for the next query?
+//  * never while an update is in flight on the slot path: the grid search
+//    reads the graph without waiting for versions, and that update might not
+//    be applied before it runs;
+//  * always when nothing is in flight: alone on the GPU a query finishes
+//    sooner in grid mode (a caller sets grid_k only where that holds);
+//  * otherwise only with measurements to go by, while arrivals leave grid
+//    mode room, and if the slot queries still running plus the grid queries
+//    in flight plus this one are expected to finish before one slot query
+//    would.  "Room" has hysteresis: grid mode is entered while a grid query
+//    takes under 70% of the mean gap between submissions and left once it
+//    takes more than the whole gap.  The mean is long-run (the running mean
+//    of the first 50 gaps, then a moving average of weight 1/50) and counts
+//    only after 16 gaps, so Poisson noise seldom crosses either threshold:
+//    each flip costs the queries that straddle it (a slot query waits for the
+//    grid queue, a grid query for the running slots).
+// Routing a query to grid mode stops the serve kernel (its running slots
+// finish first): requests sent to slots later wait for the grid queue.
+static bool server_route_grid(ddp_server *sv)
+{
+    if (!sv->grid_on) return false;
+    const InFlight f = server_in_flight(sv);
+    if (f.slot_u) return false;
+    if (f.slot_q == 0 && f.direct == 0) return true;
+    if (sv->grid_ms <= 0 || sv->slot_ms <= 0 || sv->gaps < 16) return false;
+    const double rho = sv->grid_ms / sv->gap_ms;
+    sv->grid_regime = sv->grid_regime ? rho <= 1.0 : rho < 0.7;
+    return sv->grid_regime && f.remain + (double)(f.direct + 1) * sv->grid_ms < sv->slot_ms;
+}
+
+// An update goes to a kernel of its own only while nothing is in flight on
+// the slot path: a running slot kernel would otherwise have to be stopped and
+// drained for it.
+static bool server_route_direct_update(const ddp_server *sv)
+{
+    if (!sv->grid_on) return false;
+    const InFlight f = server_in_flight(sv);
+    return f.slot_q == 0 && f.slot_u == 0;
 }
 
 extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *o,
@@ -697,7 +849,7 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
         sv->gplan.cap = (int32_t)std::max<int64_t>(sv->gplan.cap, (8LL * g->N + sv->gplan.K - 1) / sv->gplan.K);
         if ((rc = ddp_ga_alloc_owned(ctx, g->N, sv->gplan, &sv->gstate))) return fail(rc);
         hipError_t ge = hipMalloc((void **)&sv->gstamp, 64);
-        if (ge == hipSuccess) ge = hipEventCreateWithFlags(&sv->grid_ev, hipEventDisableTiming);
+        if (ge == hipSuccess) ge = hipEventCreateWithFlags(&sv->direct_ev, hipEventDisableTiming);
         if (ge != hipSuccess) return fail(ddp_hip_err(ctx, ge, "grid-mode state", __FILE__, __LINE__));
         sv->grid_on = true;
     }
@@ -753,7 +905,7 @@ extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h,
     hal((void **)&sv->arena_h, (void **)&sv->arena_d, sv->arena_cap * 4);
     hal((void **)&sv->flags_h, (void **)&sv->flags_d, 64);
     sv->delivered.assign(sv->ring, 0);
-    sv->grid_of.assign(sv->ring, 0);
+    sv->direct_of.assign(sv->ring, 0);
     sv->tags.assign(sv->ring, 0);
     sv->types.assign(sv->ring, 0);
     sv->submit_ms.assign(sv->ring, 0.0);
@@ -791,6 +943,7 @@ extern "C" int ddp_server_stats(const ddp_server *sv, ddp_server_stats_t *out)
     if (!sv || !out) return DDP_ERR_ARG;
     out->requests = sv->requests;
     out->grid_queries = sv->grid_queries;
+    out->direct_updates = sv->direct_updates;
     out->launches = (int32_t)sv->gen;
     out->compactions = sv->compactions;
     return DDP_OK;
@@ -800,22 +953,39 @@ static void publish(ddp_server *sv, const Req &r, uint64_t tag, int32_t type)
 {
     const uint32_t k = sv->submitted & (sv->ring - 1);
     sv->ring_h[k] = r;
-    sv->grid_of[k] = r.grid ? 1 : 0;
+    sv->direct_of[k] = r.direct ? 1 : 0;
     sv->requests += 1;
-    sv->grid_queries += r.grid ? 1 : 0;
+    sv->grid_queries += (r.direct && type == DDP_REQ_QUERY) ? 1 : 0;
+    sv->direct_updates += (r.direct && type == DDP_REQ_UPDATE) ? 1 : 0;
     Res *res = sv->res_h + k;
     // cleared to the ticket itself: never equal to ticket + 1, even when it wraps
     __atomic_store_n(&res->seq, sv->submitted, __ATOMIC_RELAXED);
     sv->tags[k] = tag;
     sv->types[k] = type;
     sv->submit_ms[k] = ms_now(sv);
+    if (sv->last_submit_ms >= 0) {
+        const double wgt = std::max(0.02, 1.0 / (double)++sv->gaps);
+        sv->gap_ms += wgt * ((sv->submit_ms[k] - sv->last_submit_ms) - sv->gap_ms);
+    }
+    sv->last_submit_ms = sv->submit_ms[k];
     ++sv->submitted;
     __atomic_store_n(&sv->flags_h[0], sv->submitted, __ATOMIC_RELEASE);
 }
 
-// A query submitted while nothing is in flight: grid mode.  The entry is
-// published (its result word cleared) before any of its kernels is queued,
-// so the result they write cannot be overwritten.
+// A direct request none of whose kernels could be queued: its result says so.
+static void direct_failed(ddp_server *sv, uint32_t ticket, int32_t version, int32_t status)
+{
+    Res *res = sv->res_h + (ticket & (sv->ring - 1));
+    res->cost = DDP_INF_COST;
+    res->status = status;
+    res->version = version;
+    res->t0 = res->t1 = 0;
+    __atomic_store_n(&res->seq, ticket + 1, __ATOMIC_RELEASE);
+}
+
+// A query routed to grid mode.  The entry is published (its result word
+// cleared) before any of its kernels is queued, so the result they write
+// cannot be overwritten.  Grid queries queue on the stream one after another.
 static int server_grid_query(ddp_server *sv, Req r, uint64_t tag)
 {
     ddp_ctx *ctx = sv->ctx;
@@ -829,7 +999,7 @@ static int server_grid_query(ddp_server *sv, Req r, uint64_t tag)
     gv.relax_min = sv->base_relax_min && r.relax_min;
     // an idle serve kernel would hold the stream for idle_ms: tell it to stop
     if (sv->running) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
-    r.grid = 1;
+    r.direct = 1;
     const uint32_t ticket = sv->submitted;
     publish(sv, r, tag, DDP_REQ_QUERY);
     hipLaunchKernelGGL(serve_grid_begin, dim3(1), dim3(64), 0, ctx->stream, sv->gstamp);
@@ -839,16 +1009,27 @@ static int server_grid_query(ddp_server *sv, Req r, uint64_t tag)
                            sv->res_d + (ticket & (sv->ring - 1)), ticket, r.version);
     const hipError_t e = rc ? hipSuccess : hipGetLastError();
     if (rc || e != hipSuccess) {
-        // nothing will write this entry's result: fail it from here
-        Res *res = sv->res_h + (ticket & (sv->ring - 1));
-        res->cost = DDP_INF_COST;
-        res->status = 8;
-        res->version = r.version;
-        res->t0 = res->t1 = 0;
-        __atomic_store_n(&res->seq, ticket + 1, __ATOMIC_RELEASE);
+        direct_failed(sv, ticket, r.version, 8);
         return rc ? rc : ddp_hip_err(ctx, e, "grid-mode query", __FILE__, __LINE__);
     }
-    DDP_HIP(ctx, hipEventRecord(sv->grid_ev, ctx->stream));
+    DDP_HIP(ctx, hipEventRecord(sv->direct_ev, ctx->stream));
+    return DDP_OK;
+}
+
+// An update applied while no slot runs: one workgroup for just this entry.
+static int server_direct_update(ddp_server *sv, uint32_t ticket, int32_t version)
+{
+    ddp_ctx *ctx = sv->ctx;
+    ServeArgs a = server_args(sv);
+    void *args[] = {&a, &ticket};
+    const void *fn =
+        sv->eng == SENG_BIDI ? (const void *)serve_update_kernel<true> : (const void *)serve_update_kernel<false>;
+    hipError_t e = hipLaunchKernel(fn, dim3(1), dim3(1024), args, 0, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(sv->direct_ev, ctx->stream);
+    if (e != hipSuccess) {
+        direct_failed(sv, ticket, version, SERR_WAIT);
+        return ddp_hip_err(ctx, e, "direct update", __FILE__, __LINE__);
+    }
     return DDP_OK;
 }
 
@@ -864,7 +1045,7 @@ extern "C" int ddp_server_submit_query(ddp_server *sv, int32_t s, int32_t t, uin
     r.t = ddp_to_dev(sv->g, t);
     r.version = sv->version;
     r.relax_min = sv->version < sv->zero_w_version;  // the base graph's own minimum is in a.g
-    if (sv->grid_on && r.s != r.t && server_quiet(sv)) return server_grid_query(sv, r, tag);
+    if (r.s != r.t && server_route_grid(sv)) return server_grid_query(sv, r, tag);
     publish(sv, r, tag, DDP_REQ_QUERY);
     return server_ensure(sv);
 }
@@ -994,8 +1175,12 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
     for (size_t k = 0; k < q.size(); ++k) std::memcpy(sv->arena_h + pos + dels.size() + 4 * k, q[k].data(), 16);
     const size_t roff = pos + dels.size() + 4 * q.size();
     for (size_t k = 0; k < rq.size(); ++k) std::memcpy(sv->arena_h + roff + 2 * k, rq[k].data(), 8);
+    // With nothing in flight on the slot path, the update is applied by a
+    // kernel of its own: no slot launch, so later queries can stay in grid mode.
+    const bool direct = server_route_direct_update(sv);
     Req r{};
     r.type = DDP_REQ_UPDATE;
+    r.direct = direct ? 1 : 0;
     r.version = sv->version + 1;
     r.del_off = (int32_t)pos;
     r.del_n = (int32_t)(dels.size() / 2);
@@ -1006,7 +1191,10 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
     sv->arena_marks.emplace_back(sv->submitted, pos, pos + need);
     sv->log_used += (int32_t)q.size();
     sv->version += 1;
+    const uint32_t ticket = sv->submitted;
+    if (direct && sv->running) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);  // idle: stop it
     publish(sv, r, tag, DDP_REQ_UPDATE);
+    if (direct) return server_direct_update(sv, ticket, r.version);
     return server_ensure(sv);
 }
 
@@ -1031,6 +1219,10 @@ extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t
         o.done_ms = ms_now(sv);
         o.device_ms = (double)(r->t1 - r->t0) * 1e-5;
         sv->delivered[k & mask] = 1;
+        if (o.type == DDP_REQ_QUERY && o.status == 0) {  // for the grid-mode routing
+            double &avg = sv->direct_of[k & mask] ? sv->grid_ms : sv->slot_ms;
+            avg = avg > 0 ? 0.75 * avg + 0.25 * o.device_ms : o.device_ms;
+        }
     }
     while (sv->collected != sv->submitted && sv->delivered[sv->collected & mask]) {
         sv->delivered[sv->collected & mask] = 0;
@@ -1067,7 +1259,7 @@ extern "C" void ddp_server_destroy(ddp_server *sv)
     hipSetDevice(sv->device);
     if (sv->flags_h) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
     if (sv->running && sv->done_ev) hipEventSynchronize(sv->done_ev);
-    if (sv->grid_ev) hipEventSynchronize(sv->grid_ev);  // a grid-mode query still running
+    if (sv->direct_ev) hipEventSynchronize(sv->direct_ev);  // a grid-mode query still running
     server_free(sv);
     delete sv;
 }

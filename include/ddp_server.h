@@ -42,15 +42,21 @@ typedef struct {
     int32_t ring;           /* request ring entries (power of two); 0 = 65536 */
     int32_t max_inserts;    /* capacity of the inserted-edge log; 0 = 1<<20.  An update
                                that does not fit triggers ddp_server_compact first */
-    int32_t idle_ms;        /* the kernel exits after this long without work; 0 = 2000 */
+    int32_t idle_ms;        /* the kernel exits once a slot has waited this long for work
+                               while no slot runs a request; 0 = 2000 */
     int32_t engine;         /* DDP_SERVER_AUTO / _HEAPS / _PILES (as DDP_BATCH_*) */
-    int32_t grid_k;         /* 0 = off.  K > 0: a query submitted while no other request
-                               is in flight is searched in grid mode (ddp_astar's
-                               DDP_ASTAR_GRID with K lane heaps over the whole GPU) instead
-                               of one slot; requests arriving meanwhile start in slots
-                               after it.  Worth it where one query is a large share of the
-                               graph (a full search of a power-law graph): alone on the
-                               GPU it finishes many times sooner than in one slot */
+    int32_t grid_k;         /* 0 = off.  K > 0: low-load queries are searched in grid mode
+                               (ddp_astar's DDP_ASTAR_GRID, K lane heaps over the whole GPU)
+                               instead of one slot: a query goes there while no request is
+                               in flight on the slot path, if no grid query is in flight
+                               either or if the grid queries in flight plus this one are
+                               expected to finish before one query in a slot would (moving
+                               averages of finished queries' device times).  Grid queries
+                               run one after another with the GPU to themselves; requests
+                               sent to slots meanwhile start after them.  Worth it where
+                               one query is a large share of the graph (a full search of a
+                               power-law graph), which alone on the GPU finishes many
+                               times sooner than in one slot */
 } ddp_server_opts_t;
 
 /* Queue discipline of a query slot, as for batches (include/ddp_batch.h):
@@ -90,6 +96,8 @@ int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_
 typedef struct {
     int64_t requests;       /* submitted so far */
     int64_t grid_queries;   /* of them, queries searched in grid mode (grid_k) */
+    int64_t direct_updates; /* of them, updates applied by a kernel of their own while
+                               queries ran in grid mode (grid_k) */
     int32_t launches;       /* serve-kernel launches (the first request, and relaunches
                                after an idle exit, a grid-mode query or a compaction) */
     int32_t compactions;    /* inserted-edge log compactions */
