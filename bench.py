@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 METRIC = "exact-cost A* queries/sec + p50/p99 latency, batched vs one-at-a-time GA*"
 HBM_PEAK_GBS = 8000.0
 ENGINE_NAMES = {0: "piles", 1: "heaps", 2: "piles", 3: "bidi"}
+MODE_NAMES = {0: "auto", 1: "grid", 2: "workgroup", 3: "piles", 4: "bidi"}
 
 
 def log(*a):
@@ -245,14 +246,21 @@ def run_c4(args):
                          "expansions": s_e["expansions"], "edges": s_e["edges"]}
     log(f"engines: {engines}")
 
-    # the same queries, one at a time through the single-query GA* kernel
+    # the same queries, one at a time through the single-query GA* kernels,
+    # in the mode and shape the single-query tuner picks on 16 of them
     rng = np.random.default_rng(1)
     sample = np.sort(rng.choice(len(src), size=min(args.single_sample, len(src)), replace=False))
     single_lat = []
+    one = {}
+    if len(sample):
+        from ddp_amd import autotune
+        one = autotune.tune_astar(ctx, dg, src[sample], dst[sample], heur=ddp_amd.H_EUCLID, sample=16)
+        log(f"single-query tuner: {one}")
     ts = time.perf_counter()
     for i in sample:
         q0 = time.perf_counter()
-        r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_EUCLID, K=4096)
+        r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_EUCLID, K=one["K"], block=one["block"],
+                          heap_cap=one["heap_cap"], lds_cap=one["lds_cap"], mode=one["mode"])
         single_lat.append((time.perf_counter() - q0) * 1e3)
         if r.cost != costs[i]:
             raise SystemExit(f"single-query GA* disagrees with the batch on query {i}: {r.cost} vs {costs[i]}")
@@ -272,7 +280,9 @@ def run_c4(args):
     res["engines"] = engines
     if len(sample):
         res["one_at_a_time"] = {"queries_per_s": round(single_qps, 1), "sample": int(len(sample)),
-                                "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": 4096}
+                                "p50_ms": pct(single_lat, 50), "p99_ms": pct(single_lat, 99), "K": one["K"],
+                                "block": one["block"], "mode": MODE_NAMES.get(one["mode"], one["mode"]),
+                                "chosen_by": "ddp_tune_astar on 16 of the sampled queries"}
         res["speedup_vs_one_at_a_time"] = round(value / single_qps, 2)
     res.update({
         "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
@@ -543,15 +553,28 @@ def run_c5(args):
     if args.c5_batch_only:
         ctx.close()
         return {"batched": batched}
-    lat1 = []
-    for i in range(min(args.single_sample, 8)):
-        q0 = time.perf_counter()
-        r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_ZERO, K=8192,
-                          heap_cap=8192)
-        lat1.append((time.perf_counter() - q0) * 1e3)
-        if r.cost != cost_b[i]:
-            raise SystemExit(f"C5 query {i}: single {r.cost} batch {cost_b[i]}")
+    # one at a time: grid mode (a single query is a full search of the giant
+    # component; one workgroup takes 0.7-1 s), at the K that is fastest on the
+    # sample; every query's cost checked against the batch
+    lat1, one_k, by_k = [], 0, {}
+    nsingle = min(args.single_sample, 8)
+    for K in (8192, 16384, 32768):
+        cap = max(2 * hg.N // K + 256, (8 * hg.N + K - 1) // K)  # 8N queued entries, as the server plans
+        lk = []
+        for i in range(nsingle):
+            q0 = time.perf_counter()
+            r = ddp_amd.astar(ctx, dg, int(src[i]), int(dst[i]), heur=ddp_amd.H_ZERO, K=K, heap_cap=cap,
+                              mode=ddp_amd.MODE_GRID)
+            lk.append((time.perf_counter() - q0) * 1e3)
+            if r.cost != cost_b[i]:
+                raise SystemExit(f"C5 query {i}: single (K={K}) {r.cost} batch {cost_b[i]}")
+        if lk:
+            by_k[K] = round(float(np.mean(lk)), 2)
+            if not lat1 or np.mean(lk) < np.mean(lat1):
+                lat1, one_k = lk, K
+    log(f"C5 one at a time, mean ms by K: {by_k}")
     single_qps = 1000.0 / float(np.mean(lat1)) if lat1 else None
+    grid_k = args.c5_grid_k or one_k or 32768
     # served: closed loop (every request at once) for the capacity, then open
     # loop at fractions of it for the latency curve
     o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, 0.0, args.c5_requests, ups, 11,
@@ -574,12 +597,12 @@ def run_c5(args):
     for f in args.c5_low_load:
         qps = round(f * cap, 1)
         pair = {"offered_qps": qps}
-        for name, gk in (("slots", 0), ("grid", args.c5_grid_k)):
+        for name, gk in (("slots", 0), ("grid", grid_k)):
             o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_low_requests, ups,
                                                13, heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine,
                                                grid_k=gk)
             pair[name] = served_summary(o_, ty, qi, el, info, qps, cost_b)
-        pair["grid_k"] = args.c5_grid_k
+        pair["grid_k"] = grid_k
         low.append(pair)
         log(f"C5 low load {pair}")
     best_batched = max(v["queries_per_s"] for v in batched.values())
@@ -594,7 +617,8 @@ def run_c5(args):
     res.update({"p50_ms": lat_at["p50_ms"], "p99_ms": lat_at["p99_ms"], "latency_at_qps": lat_at["offered_qps"],
                 "sweep": sweep, "low_load": low, "batched": batched, "batched_queries_per_s": best_batched,
                 "one_at_a_time": ({"queries_per_s": round(single_qps, 3), "p50_ms": pct(lat1, 50),
-                                   "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": 8192} if lat1 else None),
+                                   "p99_ms": pct(lat1, 99), "sample": len(lat1), "K": one_k, "mode": "grid",
+                                   "mean_ms_by_K": by_k} if lat1 else None),
                 "batched_over_one_at_a_time": round(best_batched / single_qps, 2) if single_qps else None,
                 "served_over_one_at_a_time": round(cap / single_qps, 2) if single_qps else None})
     ctx.close()
@@ -674,8 +698,10 @@ def main():
     ap.add_argument("--c5-low-load", type=float, nargs="*", default=[0.025, 0.05],
                     help="c5: fractions of the closed-loop capacity for the low-load points, each served with "
                          "and without grid mode")
-    ap.add_argument("--c5-low-requests", type=int, default=64, help="c5: requests per low-load point")
-    ap.add_argument("--c5-grid-k", type=int, default=8192, help="c5: grid-mode lanes of the low-load points")
+    ap.add_argument("--c5-low-requests", type=int, default=128,
+                    help="c5: requests per low-load point (the router settles after its first 20-30)")
+    ap.add_argument("--c5-grid-k", type=int, default=0,
+                    help="c5: grid-mode lanes of the low-load points (0 = the one-at-a-time leg's fastest K)")
     ap.add_argument("--no-extra", action="store_true",
                     help="c4 only: skip the C3 and C5 legs that the default line carries as extra keys")
     args = ap.parse_args()

@@ -750,10 +750,10 @@ static InFlight server_in_flight(const ddp_server *sv)
 //    would.  "Room" has hysteresis: grid mode is entered while a grid query
 //    takes under 70% of the mean gap between submissions and left once it
 //    takes more than the whole gap.  The mean is long-run (the running mean
-//    of the first 50 gaps, then a moving average of weight 1/50) and counts
-//    only after 16 gaps, so Poisson noise seldom crosses either threshold:
-//    each flip costs the queries that straddle it (a slot query waits for the
-//    grid queue, a grid query for the running slots).
+//    of the first 50 gaps, then a moving average of weight 1/50), so Poisson
+//    noise seldom crosses either threshold: each flip costs the queries that
+//    straddle it (a slot query waits for the grid queue, a grid query for the
+//    running slots).
 // Routing a query to grid mode stops the serve kernel (its running slots
 // finish first): requests sent to slots later wait for the grid queue.
 static bool server_route_grid(ddp_server *sv)
@@ -762,7 +762,7 @@ static bool server_route_grid(ddp_server *sv)
     const InFlight f = server_in_flight(sv);
     if (f.slot_u) return false;
     if (f.slot_q == 0 && f.direct == 0) return true;
-    if (sv->grid_ms <= 0 || sv->slot_ms <= 0 || sv->gaps < 16) return false;
+    if (sv->grid_ms <= 0 || sv->slot_ms <= 0 || sv->gap_ms <= 0) return false;
     const double rho = sv->grid_ms / sv->gap_ms;
     sv->grid_regime = sv->grid_regime ? rho <= 1.0 : rho < 0.7;
     return sv->grid_regime && f.remain + (double)(f.direct + 1) * sv->grid_ms < sv->slot_ms;
