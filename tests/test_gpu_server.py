@@ -206,7 +206,8 @@ def test_server_config4_full_size(ctx):
 
 
 @pytest.mark.timeout(1200)
-def test_server_config5_served(ctx):
+@pytest.mark.parametrize("grid_k,qps", [(0, 150.0), (32768, 10.0)], ids=["slots", "grid_low_load"])
+def test_server_config5_served(ctx, grid_k, qps):
     """C5's served workload on its own graph: R-MAT scale 23 (8,388,608
     nodes, 134,217,728 edges, a/b/c/d = 0.57/0.19/0.19/0.05, seed 5),
     w in [1,1000], h = 0.  Poisson arrivals of point queries between connected
@@ -214,7 +215,10 @@ def test_server_config5_served(ctx):
     Every request succeeds and is bound to the version of its submission
     position; on every version, sampled queries equal the oracle's Dijkstra on
     the oracle-applied graph, and every served cost equals the batch kernel's
-    on that version's graph, whose paths are checked edge by edge."""
+    on that version's graph, whose paths are checked edge by edge.  Served
+    at 150 requests/s in slots, and at 10/s with grid mode for low-load
+    queries (most of them then run with the whole GPU, updates by their own
+    kernel)."""
     from helpers import path_cost, rmat_updates
     hg = ddp_amd.HostGraph.rmat(23, 16 * (1 << 23), 0.57, 0.19, 0.19, 1, 1000, 5)
     say("c5 served: graph", hg.N, hg.E)
@@ -232,12 +236,16 @@ def test_server_config5_served(ctx):
     for k, p in enumerate(pos):
         ups[p] = ub[k]
     rng = np.random.default_rng(55)
-    arrival = np.cumsum(rng.exponential(1000.0 / 150.0, n))  # 150 requests/s offered
+    arrival = np.cumsum(rng.exponential(1000.0 / qps, n))
     dg = ddp_amd.DeviceGraph(ctx, hg, reorder=True)
-    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO)
-    say("c5 served: replay", sv.info())
+    sv = ddp_amd.Server(ctx, dg, heur=ddp_amd.H_ZERO, grid_k=grid_k)
+    say("c5 served: replay", sv.info(), "grid_k", grid_k, "offered/s", qps)
     res = sv.replay(arrival, types, qs, qt, ups)
+    st = sv.stats()
+    say("c5 served: stats", st)
     sv.close()
+    if grid_k:
+        assert st["grid_queries"] > len(src) // 2, st
     dg.close()
     assert [r["tag"] for r in res] == list(range(n)) and all(r["status"] == 0 for r in res), \
         [r for r in res if r["status"]][:3]
