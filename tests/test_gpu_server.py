@@ -475,9 +475,13 @@ def grid_mode_session(ctx, hg, dg, heur, engine, block, grid_k, src, dst, ub, ma
             v += 1
             query(1000 * b + 8)
         settle()
-        if b < len(ub) and b % 2 == 1:  # with nothing in flight
+        if b < len(ub) and b % 2 == 1:  # with nothing in flight: applied by its own kernel
             assert sv.submit_update(ub[b], 10**6 + b) == 0
             v += 1
+            # at once a query on the new version, while that kernel may still be
+            # queued behind a stopping serve kernel: it must not be claimed
+            # there before the version exists
+            query(1000 * b + 9)
             settle()
     st = sv.stats()
     say("  grid mode session: stats", st)
