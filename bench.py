@@ -578,7 +578,8 @@ def run_c5(args):
     # served: closed loop (every request at once) for the capacity, then open
     # loop at fractions of it for the latency curve
     o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, 0.0, args.c5_requests, ups, 11,
-                                       heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine)
+                                       heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine,
+                                       block=args.c5_serve_block)
     closed = served_summary(o_, ty, qi, el, info, 0.0, cost_b)
     log(f"C5 served closed loop {closed}")
     cap = closed["queries_per_s"]
@@ -586,7 +587,8 @@ def run_c5(args):
     sweep = [closed]
     for qps in offered:
         o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_sweep_requests, ups, 12,
-                                           heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine)
+                                           heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine,
+                                           block=args.c5_serve_block)
         sweep.append(served_summary(o_, ty, qi, el, info, qps, cost_b))
         log(f"C5 served {sweep[-1]}")
     under = [x for x in sweep if 0 < x["offered_qps"] < cap] or sweep
@@ -600,7 +602,7 @@ def run_c5(args):
         for name, gk in (("slots", 0), ("grid", grid_k)):
             o_, ty, qi, el, info = serve_trace(ctx, dg, ddp_amd.H_ZERO, src, dst, qps, args.c5_low_requests, ups,
                                                13, heap_cap=args.c5_heap_cap, engine=args.c5_serve_engine,
-                                               grid_k=gk)
+                                               block=args.c5_serve_block, grid_k=gk)
             pair[name] = served_summary(o_, ty, qi, el, info, qps, cost_b)
         pair["grid_k"] = grid_k
         low.append(pair)
@@ -695,6 +697,7 @@ def main():
     ap.add_argument("--c5-batch-only", action="store_true", help="c5: the batched launches only (for profiling)")
     ap.add_argument("--c5-engine", default="", help="c5: batch engine to run (piles or bidi; default both)")
     ap.add_argument("--c5-serve-engine", type=int, default=0, help="c5: serving engine (0 = library default)")
+    ap.add_argument("--c5-serve-block", type=int, default=0, help="c5: lanes per serving slot (0 = library default)")
     ap.add_argument("--c5-low-load", type=float, nargs="*", default=[0.025, 0.05],
                     help="c5: fractions of the closed-loop capacity for the low-load points, each served with "
                          "and without grid mode")
