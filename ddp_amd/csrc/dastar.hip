@@ -45,18 +45,40 @@ using clk = std::chrono::steady_clock;
 // 11-bit digits.  An update batch holds thousands of pairs; this is several
 // times faster than a comparison sort on them and keeps equal nodes in input
 // order, which the insert lists rely on.
-void sort_by_node(std::vector<std::pair<int32_t, int32_t>> &v, int32_t n)
+// Stable LSD radix sort of m (key, value) pairs by key (keys < n), 11 bits a
+// pass; the last pass writes them interleaved (key, value) into out.  tk and
+// tv are scratch kept between batches.
+void radix_pairs(const int32_t *key, const int32_t *val, int32_t m, int32_t n, int32_t *out,
+                 std::vector<int32_t> &tk, std::vector<int32_t> &tv)
 {
-    if (v.size() < 2) return;
-    std::vector<std::pair<int32_t, int32_t>> tmp(v.size());
     int bits = 1;
     while (bits < 31 && (1ll << bits) < (int64_t)n) ++bits;
-    for (int sh = 0; sh < bits; sh += 11) {
+    const int passes = (bits + 10) / 11;
+    tk.resize(2 * (size_t)m);
+    tv.resize(2 * (size_t)m);
+    const int32_t *ck = key, *cv = val;
+    int32_t *bk[2] = {tk.data(), tk.data() + m}, *bv[2] = {tv.data(), tv.data() + m};
+    for (int p = 0; p < passes; ++p) {
+        const int sh = 11 * p;
         uint32_t cnt[2049] = {0};
-        for (const auto &p : v) cnt[(((uint32_t)p.first >> sh) & 2047u) + 1]++;
+        for (int32_t i = 0; i < m; ++i) cnt[(((uint32_t)ck[i] >> sh) & 2047u) + 1]++;
         for (int i = 0; i < 2048; ++i) cnt[i + 1] += cnt[i];
-        for (const auto &p : v) tmp[cnt[((uint32_t)p.first >> sh) & 2047u]++] = p;
-        v.swap(tmp);
+        if (p == passes - 1) {
+            for (int32_t i = 0; i < m; ++i) {
+                const uint32_t j = cnt[((uint32_t)ck[i] >> sh) & 2047u]++;
+                out[2 * j] = ck[i];
+                out[2 * j + 1] = cv[i];
+            }
+        } else {
+            int32_t *nk = bk[p & 1], *nv = bv[p & 1];
+            for (int32_t i = 0; i < m; ++i) {
+                const uint32_t j = cnt[((uint32_t)ck[i] >> sh) & 2047u]++;
+                nk[j] = ck[i];
+                nv[j] = cv[i];
+            }
+            ck = nk;
+            cv = nv;
+        }
     }
 }
 float ms_since(clk::time_point t0) { return std::chrono::duration<float, std::milli>(clk::now() - t0).count(); }
@@ -695,6 +717,7 @@ struct ddp_dastar {
     size_t c_work = 0;
     int32_t *h_work = nullptr;  // pinned host staging of d_work (an asynchronous upload)
     size_t c_hwork = 0;
+    std::vector<int32_t> ku, kv, kx, tk, tv;  // host scratch of a batch's insert grouping
     int32_t epoch = 0;
     // search
     GaGridPlan P{};
@@ -1111,38 +1134,24 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     const auto t0 = clk::now();
     const int32_t N = d->N;
     ddp_dastar_stats_t S{};
+    int32_t nd = 0;
     for (int32_t i = 0; i < n; ++i) {
         const ddp_update_t &l = lines[i];
         DDP_CHECK(l.flag == 0 || l.flag == 1, ctx, DDP_ERR_ARG, "update line %d: flag %d", i, l.flag);
         DDP_CHECK(l.u >= 0 && l.u < N && l.v >= 0 && l.v < N, ctx, DDP_ERR_ARG, "update line %d: edge %d->%d outside [0,%d)",
                   i, l.u, l.v, N);
+        nd += l.flag == 0;
     }
-    // ---- 1. host, O(batch): split the lines; group new inserts by source and by target
-    std::vector<int32_t> pairs, ins;
-    std::vector<std::pair<int32_t, int32_t>> fg, rg;  // (key, log entry), insert order kept
-    for (int32_t i = 0; i < n; ++i) {
-        const ddp_update_t &l = lines[i];
-        if (l.flag == 0) {
-            ++S.deletes;
-            pairs.push_back(l.u);
-            pairs.push_back(l.v);
-        } else {
-            const int32_t x = d->log_used + S.inserts;
-            ++S.inserts;
-            ins.push_back(l.u);
-            ins.push_back(l.v);
-            ins.push_back((int32_t)l.w);
-            fg.emplace_back(l.u, x);
-            rg.emplace_back(l.v, x);
-        }
-    }
-    sort_by_node(fg, N);
-    sort_by_node(rg, N);
-    int rc = log_reserve(d, d->log_used + S.inserts);
+    S.deletes = nd;
+    S.inserts = n - nd;
+    const int32_t ni = S.inserts;
+    int rc = log_reserve(d, d->log_used + ni);
     if (rc) return rc;
-    // device work list: [pairs | ins | fg | rg], staged in pinned host memory
+    // ---- 1. host, O(batch), written straight into the pinned device work
+    // list [pairs | ins | fg | rg]: the delete pairs, the new inserts, and the
+    // inserts' log entries grouped by source and by target (insert order kept)
     // (every earlier batch synchronised before returning, so it is free)
-    const size_t nwork = pairs.size() + ins.size() + 4 * fg.size();
+    const size_t nwork = 2 * (size_t)nd + 7 * (size_t)ni;
     if (d->c_hwork < nwork || !d->h_work) {
         if (d->h_work) DDP_HIP(ctx, hipHostFree(d->h_work));
         d->h_work = nullptr;
@@ -1151,16 +1160,32 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
         DDP_HIP(ctx, hipHostMalloc((void **)&d->h_work, c * sizeof(int32_t)));
         d->c_hwork = c;
     }
-    {
-        int32_t *o = d->h_work;
-        o = std::copy(pairs.begin(), pairs.end(), o);
-        o = std::copy(ins.begin(), ins.end(), o);
-        for (const auto &x : fg) *o++ = x.first, *o++ = x.second;
-        for (const auto &x : rg) *o++ = x.first, *o++ = x.second;
+    int32_t *o_pairs = d->h_work, *o_ins = o_pairs + 2 * (size_t)nd, *o_fg = o_ins + 3 * (size_t)ni,
+            *o_rg = o_fg + 2 * (size_t)ni;
+    d->ku.resize(ni);
+    d->kv.resize(ni);
+    d->kx.resize(ni);
+    for (int32_t i = 0, a = 0, b = 0; i < n; ++i) {
+        const ddp_update_t &l = lines[i];
+        if (l.flag == 0) {
+            o_pairs[2 * a] = l.u;
+            o_pairs[2 * a + 1] = l.v;
+            ++a;
+        } else {
+            o_ins[3 * b] = l.u;
+            o_ins[3 * b + 1] = l.v;
+            o_ins[3 * b + 2] = (int32_t)l.w;
+            d->ku[b] = l.u;
+            d->kv[b] = l.v;
+            d->kx[b] = d->log_used + b;
+            ++b;
+        }
     }
+    radix_pairs(d->ku.data(), d->kx.data(), ni, N, o_fg, d->tk, d->tv);
+    radix_pairs(d->kv.data(), d->kx.data(), ni, N, o_rg, d->tk, d->tv);
     if ((rc = dev_upload(ctx, &d->d_work, &d->c_work, (const int32_t *)d->h_work, nwork))) return rc;
-    const int32_t *w_pairs = d->d_work, *w_ins = w_pairs + pairs.size(), *w_fg = w_ins + ins.size(),
-                  *w_rg = w_fg + 2 * fg.size();
+    const int32_t *w_pairs = d->d_work, *w_ins = w_pairs + 2 * (size_t)nd, *w_fg = w_ins + 3 * (size_t)ni,
+                  *w_rg = w_fg + 2 * (size_t)ni;
     const int T = 256;
     auto grid = [&](size_t k) { return dim3((unsigned)std::max<size_t>(1, (k + T - 1) / T)); };
     // ---- 2. device: deletes (they never see this batch's inserts), then the appends
@@ -1168,7 +1193,7 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     uint32_t rctrl[RC_WORDS] = {0};
     DDP_HIP(ctx, hipEventRecord(d->ev_r0, ctx->stream));
     DDP_HIP(ctx, hipMemsetAsync(d->d_rctrl, 0, RC_WORDS * 4, ctx->stream));
-    const int32_t npairs = (int32_t)(pairs.size() / 2), nins = S.inserts;
+    const int32_t npairs = nd, nins = ni;
     if (npairs)
         hipLaunchKernelGGL(apply_deletes, grid(npairs), dim3(T), 0, ctx->stream, w_pairs, npairs, fwd_view(d),
                            d->g->col, d->g->rcol, (const int32_t *)d->g->roff, (const int32_t *)d->d_reid, d->L,
