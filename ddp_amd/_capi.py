@@ -76,7 +76,7 @@ class DAStats(ctypes.Structure):
                 ("invalidated", c_int32), ("repulled", c_int32), ("pushes", c_int32), ("iterations", c_int32),
                 ("expansions", c_int64), ("edges", c_int64), ("improvements", c_int64), ("host_ms", c_float),
                 ("repair_ms", c_float), ("search_ms", c_float), ("total_ms", c_float), ("compacted", c_int32),
-                ("compact_ms", c_float)]
+                ("compact_ms", c_float), ("pile_refills", c_int64), ("pile_scanned", c_int64)]
 
 
 class ServerOpts(ctypes.Structure):

@@ -895,7 +895,8 @@ static int dastar_compact(ddp_dastar *d)
 // plus the repair pushes counted in control slot 2; the dormant pile it leaves
 // becomes the next open set.
 // rctrl_out: also copies the repair counters back, in the same host round trip.
-static int pile_run(ddp_dastar *d, int32_t *iters, float *kms, uint32_t *rctrl_out = nullptr)
+static int pile_run(ddp_dastar *d, int32_t *iters, float *kms, uint32_t *rctrl_out = nullptr,
+                    unsigned long long *stats_out = nullptr)
 {
     ddp_ctx *ctx = d->ctx;
     DaPileArgs a{};
@@ -942,6 +943,8 @@ static int pile_run(ddp_dastar *d, int32_t *iters, float *kms, uint32_t *rctrl_o
     DDP_HIP(ctx, hipMemcpyAsync(res, d->pres, 16, hipMemcpyDeviceToHost, ctx->stream));
     if (rctrl_out)
         DDP_HIP(ctx, hipMemcpyAsync(rctrl_out, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (stats_out)  // the search counters in the same round trip
+        DDP_HIP(ctx, hipMemcpyAsync(stats_out, d->S.stats, 64, hipMemcpyDeviceToHost, ctx->stream));
     DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     DDP_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1212,6 +1215,8 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     S.host_ms = ms_since(t0);
     int32_t iters = 0;
     float kms = 0;
+    unsigned long long cnt[8] = {0};  // the search counters (expanded, edges, improved, stale, rebuilds, refills, scanned)
+    bool have_cnt = false;
     if (mode == DDP_DASTAR_RECOMPUTE) {
         if ((rc = fresh_search(d, &iters, &kms))) return rc;
         DDP_HIP(ctx, hipMemcpyAsync(rctrl, d->d_rctrl, RC_WORDS * 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1237,8 +1242,9 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
         // ---- 6. resume GA* from the retained queues plus the repair pushes
         if (d->piles && d->s != d->t) {
             // the pile search reads F(t) on the device: one host round trip
-            // for the repair counters and the search together
-            if ((rc = pile_run(d, &iters, &kms, rctrl))) return rc;
+            // for the repair counters, the search and its counters together
+            if ((rc = pile_run(d, &iters, &kms, rctrl, cnt))) return rc;
+            have_cnt = true;
         } else {
             uint64_t tword = 0;
             DDP_HIP(ctx, hipMemcpyAsync(&tword, d->S.gp + d->t, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1267,11 +1273,15 @@ extern "C" int ddp_dastar_apply(ddp_dastar *d, const ddp_update_t *lines, int32_
     d->live_log += nins - (int64_t)rctrl[RC_DEL_LOG];
     S.iterations = iters;
     S.search_ms = kms;
-    unsigned long long cnt[4];
-    if ((rc = ddp_ga_stats(ctx, d->S, cnt))) return rc;
+    if (!have_cnt) {
+        DDP_HIP(ctx, hipMemcpyAsync(cnt, d->S.stats, 64, hipMemcpyDeviceToHost, ctx->stream));
+        DDP_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     S.expansions = (int64_t)cnt[0];
     S.edges = (int64_t)cnt[1];
     S.improvements = (int64_t)cnt[2];
+    S.pile_refills = (int64_t)cnt[5];
+    S.pile_scanned = (int64_t)cnt[6];
     // list walks grow with the log: fold it into the rows once it is long
     if (d->compact_at > 0 && d->log_used >= d->compact_at) {
         const auto tc = clk::now();

@@ -48,6 +48,8 @@ typedef struct {
     float total_ms;
     int32_t compacted;          /* 1 if the batch ended with a log compaction */
     float compact_ms;           /* its time (included in total_ms) */
+    int64_t pile_refills;       /* pile engine: near-ring refills of the resumed search */
+    int64_t pile_scanned;       /* pile engine: far-pile entries read by those refills */
 } ddp_dastar_stats_t;
 
 /* Uploads the graph (forward + transposed rows), runs the initial GA* from s
