@@ -10,7 +10,7 @@
 //      of u->v as in DA_star/GA_merge.cu:1019-1037) become tombstones in HBM;
 //   2. if a deleted u->v was v's parent edge, v's whole parent subtree is
 //      invalidated (found by walking children: y with parent(y) == x, level
-//      by level in one launch over 64 workgroups), which replaces the
+//      by level in one launch over 16 workgroups), which replaces the
 //      ancestor walks and serial cycle fixing of
 //      DA_star/DA_dyn_insert_del_nocycle.cu:376-483, 1023-1134;
 //   3. each invalidated node re-pulls its best value from live in-edges of
@@ -358,11 +358,14 @@ __device__ __forceinline__ bool walk_barrier(uint32_t *bar, uint32_t nblocks, ui
 // level goes to workgroup i % kWalkGrid, so even a level of a few hundred
 // nodes spreads over kWalkGrid CUs: one workgroup walking alone was bound
 // by its CU's gather rate (about 35 ns per node, 42 us for a level of 1,200
-// nodes).  The roots are the rctrl[RC_NEXT] draws apply_deletes made into
-// buf0; later levels alternate between buf1 and buf0 (each at most N nodes:
-// every node is claimed once per epoch).  On exit rctrl[RC_WALKED] holds the
-// number of levels, and RC_WALK_ERR is set if a barrier timed out.
-constexpr int kWalkGrid = 64, kWalkGridBlock = 256;
+// nodes).  Sixteen workgroups (4,096 lanes) still take a C3 level in one
+// pass and their barrier is cheaper than 32's or 64's: the repair's device
+// time per C3 batch was 0.250, 0.259 and 0.296 ms with 16, 32 and 64.  The
+// roots are the rctrl[RC_NEXT] draws apply_deletes made into buf0; later
+// levels alternate between buf1 and buf0 (each at most N nodes: every node is
+// claimed once per epoch).  On exit rctrl[RC_WALKED] holds the number of
+// levels, and RC_WALK_ERR is set if a barrier timed out.
+constexpr int kWalkGrid = 16, kWalkGridBlock = 256;
 
 __global__ void __launch_bounds__(kWalkGridBlock)
     subtree_walk(GraphView G, InsLog L, const uint64_t *gp, int32_t *mark, int32_t epoch, int32_t *buf0,
