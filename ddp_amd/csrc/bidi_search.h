@@ -208,6 +208,10 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
     L.gu[lane] = rs.gu;
     L.node[lane] = u;
     const bool use_own = total <= 4 * B * R;
+    // a round of at most one edge per lane is latency-bound: relax without the
+    // word read, the atomicMin's returned word telling whether it improved
+    // (heuristics without loads; as relax_window's DIRECT)
+    const bool direct = (HK == DDP_H_ZERO || HK == DDP_H_MANHATTAN) && relax_min && total <= B;
     if (use_own) {
         const int32_t to = deg < kOwnRun ? run0 + deg : run0 + kOwnRun;
         for (int32_t x = run0; x < to; ++x) L.own[x] = (uint16_t)lane;
@@ -265,7 +269,7 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
             if (VERS && dv[r] <= D.ver) c[r] = -1;  // deleted at or before this version
             if (c[r] >= 0) {
                 hin[r] = heur_load<HK>(H, c[r]);
-                cur[r] = ld_gp(pick2(D.gp, dr[r]) + c[r]);
+                if (!direct) cur[r] = ld_gp(pick2(D.gp, dr[r]) + c[r]);
             }
         }
         // stage 3: candidates and one atomic per improving edge
@@ -279,7 +283,7 @@ __device__ __forceinline__ void expand_bidi(const BidiSide &D, const HeurView &H
                 const int32_t pc = pot_from<HK>(H, c[r], hin[r]);
                 const uint32_t cand = L.gu[jj[r]] + wt[r] + (uint32_t)(dr[r] ? -pc : pc);
                 want[r] = ((uint64_t)cand << 32) | (uint32_t)L.node[jj[r]];
-                tried[r] = (uint32_t)(cur[r] >> 32) > cand;
+                tried[r] = direct || (uint32_t)(cur[r] >> 32) > cand;
             }
         }
         __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every load above was consumed

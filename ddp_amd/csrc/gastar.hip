@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
         }
         // -- expand and push
         GridPush<B> push{a.pbuf + (int64_t)(it & 1) * a.pcap, ctrl + cs * GA_SLOT_WORDS + GA_S_PUSHCNT, a.pcap};
-        expand_block<B, HK>(a.g, a.h, a.gp, a.t, u, fu, L, C, push);
+        expand_block<B, HK, true>(a.g, a.h, a.gp, a.t, u, fu, L, C, push);
         err_local |= push.err;
         const uint32_t wt = wave_min_u32(push.tmin);
         const uint32_t wp = wave_min_u32(push.pushmin);
@@ -329,7 +329,8 @@ __global__ void __launch_bounds__(B) ga_wg_kernel(WgArgs a)
     } else {
         if constexpr (WM == WM_PILES) {
             const PilePtrs P{a.home, a.home + a.ncap, a.home + a.ncap + a.fcap, a.ncap, a.fcap, s_heap, a.lring};
-            slot_search_piles<B, HK>(a.g, a.h, a.gp, a.touched, P, a.N, a.s, a.t, a.maxit, a.delta0, S, C, stale);
+            slot_search_piles<B, HK, pile_entries_per_lane<B>(), false, true>(a.g, a.h, a.gp, a.touched, P, a.N, a.s,
+                                                                              a.t, a.maxit, a.delta0, S, C, stale);
         } else {
             slot_search<B, HK>(a.g, a.h, a.gp, a.touched, H, s_heap, a.N, a.s, a.t, a.maxit, S, C, stale);
         }

@@ -73,11 +73,18 @@ constexpr int expand_unroll()
 // issued stage by stage (heuristic inputs and (f, parent) words, then one
 // atomic per improving edge), and improvements go to the push policy.  Fb
 // as in expand_rows.
-template <int R, int HK, class GuOf, class ParOf, class Push>
+//
+// DIRECT (atomicMin relaxation, heuristics that need no load): no word is
+// read first; every edge that can still beat Fb issues its atomicMin, whose
+// returned word says whether it improved.  A small round is a chain of
+// dependent round trips, and this takes one out of it; a large round keeps the
+// read, which spares the atomics of the many edges that cannot improve.
+template <int R, int HK, bool DIRECT = false, class GuOf, class ParOf, class Push>
 __device__ __forceinline__ void relax_window(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
                                              int32_t (&c)[R], const uint32_t (&wt)[R], GuOf gu_of, ParOf par_of,
                                              uint32_t Fb, ExpandCounters &C, Push &push)
 {
+    static_assert(!DIRECT || HK == DDP_H_ZERO || HK == DDP_H_MANHATTAN, "DIRECT: heuristics without loads only");
     // stage 2: heuristic inputs and the current (f, parent) words, all
     // loads issued before the first heuristic is computed.  hin, cur and
     // got below are only written and read by lanes whose edge takes part:
@@ -92,7 +99,7 @@ __device__ __forceinline__ void relax_window(const GraphView &G, const HeurView 
                 c[r] = -1;  // cannot beat the bound: no word read
         if (c[r] >= 0) {  // negative id = deleted edge
             hin[r] = heur_load<HK>(H, c[r]);
-            cur[r] = ld_gp(gp + c[r]);
+            if constexpr (!DIRECT) cur[r] = ld_gp(gp + c[r]);
         }
     }
     int32_t hc[R];
@@ -108,13 +115,14 @@ __device__ __forceinline__ void relax_window(const GraphView &G, const HeurView 
         if (c[r] >= 0 && (HK != DDP_H_ARRAY || hc[r] >= 0)) {  // h = -1: cannot reach t
             const uint32_t cand = gu_of(r) + wt[r] + (uint32_t)hc[r];
             want[r] = ((uint64_t)cand << 32) | (uint32_t)par_of(r);
-            tried[r] = (uint32_t)(cur[r] >> 32) > cand;
+            if constexpr (DIRECT) tried[r] = true;
+            else tried[r] = (uint32_t)(cur[r] >> 32) > cand;
         }
     }
     // Every load above has returned by now (the comparisons used them);
     // saying so keeps the compiler from waiting again before each CAS.
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    const bool by_min = G.relax_min != 0;  // uniform
+    const bool by_min = DIRECT || G.relax_min != 0;  // uniform
     if (by_min) {
         // one atomicMin per improving edge: the smallest (f, parent) of
         // the round wins with no retry round trips
@@ -122,7 +130,7 @@ __device__ __forceinline__ void relax_window(const GraphView &G, const HeurView 
         for (int r = 0; r < R; ++r)
             if (tried[r]) got[r] = (uint64_t)atomicMin((unsigned long long *)(gp + c[r]),
                                                        (unsigned long long)want[r]);
-    } else {
+    } else if constexpr (!DIRECT) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (tried[r])
@@ -139,9 +147,16 @@ __device__ __forceinline__ void relax_window(const GraphView &G, const HeurView 
         key[r] = 0;
         if (tried[r]) {
             const uint32_t cand = (uint32_t)(want[r] >> 32);
-            bool ok = by_min ? (uint32_t)(got[r] >> 32) > cand : got[r] == cur[r];
-            uint64_t prev = by_min ? got[r] : cur[r];
-            if (!ok && !by_min) {
+            bool ok;
+            uint64_t prev;
+            if constexpr (DIRECT) {
+                ok = (uint32_t)(got[r] >> 32) > cand;
+                prev = got[r];
+            } else {
+                ok = by_min ? (uint32_t)(got[r] >> 32) > cand : got[r] == cur[r];
+                prev = by_min ? got[r] : cur[r];
+            }
+            if (!DIRECT && !ok && !by_min) {
                 uint64_t now = got[r];
                 while ((uint32_t)(now >> 32) > cand) {
                     const uint64_t old = atomicCAS((unsigned long long *)(gp + c[r]), (unsigned long long)now,
@@ -192,7 +207,7 @@ __device__ __forceinline__ void relax_window(const GraphView &G, const HeurView 
 // (f, parent) word is read: it cannot lead to a cheaper path to t, and the
 // word's random 8-byte read costs a whole cache line.  Callers that keep
 // searching for a later, larger bound (a resumed DA* search) pass UINT32_MAX.
-template <int B, int HK, int P, class Push, bool FINAL_SYNC = true>
+template <int B, int HK, int P, class Push, bool FINAL_SYNC = true, bool DIRECT_OK = false>
 __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
                                             const int32_t (&u)[P], const RowSpan (&rs)[P], ExpandLds<B * P> &L,
                                             ExpandCounters &C, Push &push, uint32_t Fb = 0xFFFFFFFFu)
@@ -231,6 +246,12 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
     // power-law graphs) skip the hints: refilling them per window cost two
     // barriers each and measured slower than searching.
     const bool use_own = total <= 4 * B * R;
+    // a round of at most one edge per lane is latency-bound: relax without the
+    // word read (atomicMin relaxation, heuristics without loads; relax_window).
+    // Only where the caller allows it (DIRECT_OK): the second copy of the
+    // relaxation costs registers that the 1,024-lane batch and serve kernels
+    // do not have to spare, and their rounds are mostly large.
+    const bool direct = DIRECT_OK && G.relax_min != 0 && total <= B;
     if (use_own) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
@@ -291,9 +312,14 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (G.dver && dv[r] <= G.ver) c[r] = -1;  // deleted at or before this version
-        relax_window<R, HK>(
-            G, H, gp, t, c, wt, [&](int r) { return L.gu[jj[r]]; }, [&](int r) { return L.node[jj[r]]; }, Fb, C,
-            push);
+        auto gu_of = [&](int r) { return L.gu[jj[r]]; };
+        auto par_of = [&](int r) { return L.node[jj[r]]; };
+        if constexpr (DIRECT_OK && (HK == DDP_H_ZERO || HK == DDP_H_MANHATTAN)) {
+            if (direct) relax_window<R, HK, true>(G, H, gp, t, c, wt, gu_of, par_of, Fb, C, push);
+            else relax_window<R, HK>(G, H, gp, t, c, wt, gu_of, par_of, Fb, C, push);
+        } else {
+            relax_window<R, HK>(G, H, gp, t, c, wt, gu_of, par_of, Fb, C, push);
+        }
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -342,12 +368,12 @@ __device__ __forceinline__ void expand_rows(const GraphView &G, const HeurView &
     if constexpr (FINAL_SYNC) __syncthreads();
 }
 
-template <int B, int HK, class Push>
+template <int B, int HK, bool DIRECT_OK = false, class Push>
 __device__ __forceinline__ void expand_block(const GraphView &G, const HeurView &H, uint64_t *gp, int32_t t,
                                              int32_t u, uint32_t fu, ExpandLds<B> &L, ExpandCounters &C,
                                              Push &push)
 {
     const int32_t uu[1] = {u};
     const RowSpan rs[1] = {row_span<HK>(G, H, u, fu)};
-    expand_rows<B, HK, 1>(G, H, gp, t, uu, rs, L, C, push);
+    expand_rows<B, HK, 1, Push, true, DIRECT_OK>(G, H, gp, t, uu, rs, L, C, push);
 }

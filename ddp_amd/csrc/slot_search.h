@@ -399,7 +399,7 @@ constexpr uint32_t kThinRing = 64;
 // to Z's dormant pile instead of being dropped (see PileResume).  On return
 // the dormant pile holds every open entry left: the search's open set for the
 // next resume.  A far-pile rebuild re-queues every node with a finite word.
-template <int B, int HK, int W = pile_entries_per_lane<B>(), bool RESUME = false>
+template <int B, int HK, int W = pile_entries_per_lane<B>(), bool RESUME = false, bool DIRECT_OK = false>
 __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp, int32_t *touched,
                                   const PilePtrs &P, int32_t N, int32_t s, int32_t t, int32_t maxit,
                                   uint32_t delta0, SlotLds<B, B * W> &S, ExpandCounters &C, uint32_t &stale,
@@ -661,7 +661,7 @@ __device__ void slot_search_piles(const GraphView &G, HeurView hv, uint64_t *gp,
         }
         PilePush<B, B * W, RESUME> push{&P, far, &S, touched, Z};
         // a resumed search keeps every relaxation: a later batch may raise F(t)
-        expand_rows<B, HK, W, PilePush<B, B * W, RESUME>, false>(G, hv, gp, t, u, rs, S.L, C, push,
+        expand_rows<B, HK, W, PilePush<B, B * W, RESUME>, false, DIRECT_OK>(G, hv, gp, t, u, rs, S.L, C, push,
                                                                   RESUME ? 0xFFFFFFFFu : F);
         if (push.err) atomicOr(&S.err, push.err);
         __syncthreads();  // the round's pushes and errors are visible to the next round
