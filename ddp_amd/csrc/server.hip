@@ -20,13 +20,15 @@
 // Every device wait is bounded; idle workgroups exit after idle_ms and the host
 // relaunches the kernel when new requests arrive.
 //
-// Grid mode at low load (opts.grid_k): a query submitted while no other
-// request is in flight is searched by the single-query GA* grid kernel (K lane
-// heaps over the whole GPU, gastar.hip) instead of one slot.  Its ring entry
-// is marked so that the serve kernel skips it, and its search runs on the
-// server's stream after any serve kernel (an idle one is told to stop) and
-// before the next launch; a one-lane kernel then writes its result into the
-// ring like a slot would.
+// Grid mode at low load (opts.grid_k): a query that server_route_grid sends
+// there (nothing else in flight, or arrivals slow enough for grid mode to
+// keep up) is searched by the single-query GA* grid kernel (K lane heaps over
+// the whole GPU, gastar.hip) instead of one slot.  Its ring entry is marked
+// so that the serve kernel skips it, and its search runs on the server's
+// stream after any serve kernel (which is told to stop) and before the next
+// launch; a one-lane kernel then writes its result into the ring like a slot
+// would.  Updates submitted while nothing runs on the slot path are applied
+// the same way, by a kernel of their own.
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -572,7 +574,7 @@ struct ddp_server {
     hipEvent_t done_ev = nullptr;
     bool running = false;
     uint32_t gen = 0;              // number of the newest launch
-    // grid mode for queries submitted while nothing is in flight
+    // grid mode for low-load queries (server_route_grid)
     bool grid_on = false;
     GaGridPlan gplan{};
     GaGridState gstate{};
