@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(B) ga_grid_kernel(GridArgs a)
         }
         // -- expand and push
         GridPush<B> push{a.pbuf + (int64_t)(it & 1) * a.pcap, ctrl + cs * GA_SLOT_WORDS + GA_S_PUSHCNT, a.pcap};
-        expand_block<B, HK, true>(a.g, a.h, a.gp, a.t, u, fu, L, C, push);
+        expand_block<B, HK>(a.g, a.h, a.gp, a.t, u, fu, L, C, push);
         err_local |= push.err;
         const uint32_t wt = wave_min_u32(push.tmin);
         const uint32_t wp = wave_min_u32(push.pushmin);
