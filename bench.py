@@ -555,10 +555,11 @@ def run_c5(args):
         return {"batched": batched}
     # one at a time: grid mode (a single query is a full search of the giant
     # component; one workgroup takes 0.7-1 s), at the K that is fastest on the
-    # sample; every query's cost checked against the batch
+    # sample (default blocks: 256 lanes below K = 65,536, 1,024 from there);
+    # every query's cost checked against the batch
     lat1, one_k, by_k = [], 0, {}
     nsingle = min(args.single_sample, 8)
-    for K in (8192, 16384, 32768):
+    for K in (8192, 16384, 32768, 65536):
         cap = max(2 * hg.N // K + 256, (8 * hg.N + K - 1) // K)  # 8N queued entries, as the server plans
         lk = []
         for i in range(nsingle):

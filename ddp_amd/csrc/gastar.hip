@@ -427,7 +427,11 @@ int ddp_ga_plan(ddp_ctx *ctx, const ddp_graph *g, const ddp_astar_opts_t *o, GaG
         }
         return DDP_OK;
     }
-    if (B == 0) B = K >= 4096 ? 256 : 64;
+    // the widest grids on wide workgroups: fewer workgroups make each
+    // device-wide barrier cheaper.  One R-MAT scale-23 search: K = 65,536 took
+    // 33 ms on 64 workgroups of 1,024 lanes and 42 ms on 256 of 256; K = 32,768
+    // took 37 ms on 128 of 256 lanes and 41 ms on 32 of 1,024.
+    if (B == 0) B = K >= 65536 ? 1024 : K >= 4096 ? 256 : 64;
     DDP_CHECK(B == 64 || B == 256 || B == 1024, ctx, DDP_ERR_ARG,
               "block must be 64, 256 or 1024 (got %d)", B);
     K = ((K + B - 1) / B) * B;

@@ -197,9 +197,9 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
     // workgroup mode with 256..1024 queues, pile mode with 512/1024 lanes, the
     // search from both ends with 256..1024 lanes (closed-form heuristics; on a
     // graph without transposed rows ddp_astar refuses it before any launch and
-    // the candidate drops out like any that fails), grid mode with 1024..32768
+    // the candidate drops out like any that fails), grid mode with 1024..65536
     // queues (a full search of a large power-law graph wants the most lanes:
-    // K = 32,768 took 38 ms where 8,192 took 66 on R-MAT scale 23)
+    // K = 65,536 took 30 ms where 8,192 took 66 on R-MAT scale 23)
     struct Cand {
         int32_t K, B, mode;
     };
@@ -212,6 +212,7 @@ extern "C" int ddp_tune_astar(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, in
         for (int32_t B : {64, 256}) cands.push_back({K, B, DDP_ASTAR_GRID});
     for (int32_t K : {16384, 32768})
         for (int32_t B : {256, 1024}) cands.push_back({K, B, DDP_ASTAR_GRID});
+    cands.push_back({65536, 1024, DDP_ASTAR_GRID});
     auto time_opts = [&](const ddp_astar_opts_t &o) {
         double total = 0;
         for (size_t i = 0; i < s.size(); ++i) {

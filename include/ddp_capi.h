@@ -162,7 +162,8 @@ int ddp_bfs_hops_noalloc(ddp_ctx *ctx, const ddp_graph *g, int32_t t,
 
 typedef struct {
     int32_t K;          /* number of priority queues (one per lane); 0 = 1024 */
-    int32_t block;      /* lanes per workgroup; grid mode: 64, 256 or 1024; 0 = auto */
+    int32_t block;      /* lanes per workgroup; grid mode: 64, 256 or 1024; 0 = auto
+                           (grid mode: 64 below K = 4096, 256 below 65536, 1024 from there) */
     int32_t heap_cap;   /* entries per queue (LDS part + HBM spill); 0 = auto */
     int32_t lds_cap;    /* heap entries per queue kept in LDS; 0 = auto */
     int32_t max_iters;  /* iteration limit; 0 = auto (bounded) */

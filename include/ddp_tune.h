@@ -30,7 +30,7 @@ int ddp_tune_batch(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, int32_t nq, c
 /* Times single-query GA* in workgroup mode (K 256..1024), pile mode (K 512,
  * 1024), bidirectional mode (K 256..1024; closed-form heuristics, graphs with
  * transposed rows) and grid mode (K 1024..8192 at block 64/256, 16384 and
- * 32768 at block 256/1024) on up to `sample`
+ * 32768 at block 256/1024, 65536 at block 1024) on up to `sample`
  * of the queries, then the winner with 8, 16 and 32 LDS heap levels per queue
  * (grid and workgroup modes); writes the fastest options (K, block, mode,
  * heap_cap, lds_cap) and its mean ms. */
