@@ -333,10 +333,12 @@ def run_c1(args):
     h, _ = dg.bfs_hops(t, reverse=True, scale=minw, store_as_h=True)
     r, wall, kms = time_single(ctx, dg, s, t, ddp_amd.H_ARRAY, 1024, args.steps, args.warmup)
     other = {}
-    for name, mode, heur in (("workgroup_mode", ddp_amd.MODE_WORKGROUP, ddp_amd.H_ARRAY),
-                             ("pile_mode", ddp_amd.MODE_PILES, ddp_amd.H_ARRAY),
-                             ("bidi_mode_h0", ddp_amd.MODE_BIDI, ddp_amd.H_ZERO)):
-        ro, wall_o, kms_o = time_single(ctx, dg, s, t, heur, 1024, args.steps, args.warmup, mode=mode)
+    for name, mode, heur, blk in (("grid_block256", ddp_amd.MODE_GRID, ddp_amd.H_ARRAY, 256),
+                                  ("grid_block1024", ddp_amd.MODE_GRID, ddp_amd.H_ARRAY, 1024),
+                                  ("workgroup_mode", ddp_amd.MODE_WORKGROUP, ddp_amd.H_ARRAY, 0),
+                                  ("pile_mode", ddp_amd.MODE_PILES, ddp_amd.H_ARRAY, 0),
+                                  ("bidi_mode_h0", ddp_amd.MODE_BIDI, ddp_amd.H_ZERO, 0)):
+        ro, wall_o, kms_o = time_single(ctx, dg, s, t, heur, 1024, args.steps, args.warmup, block=blk, mode=mode)
         if ro.cost != r.cost:
             raise SystemExit(f"C1: {name} {ro.cost} != grid mode {r.cost}")
         so = ro.stats
