@@ -35,6 +35,7 @@
 #include "../../include/ddp_dastar.h"
 #include "compact.h"
 #include "gastar.h"
+#include "host_sort.h"
 #include "slot_search.h"
 
 namespace {
@@ -45,42 +46,6 @@ using clk = std::chrono::steady_clock;
 // 11-bit digits.  An update batch holds thousands of pairs; this is several
 // times faster than a comparison sort on them and keeps equal nodes in input
 // order, which the insert lists rely on.
-// Stable LSD radix sort of m (key, value) pairs by key (keys < n), 11 bits a
-// pass; the last pass writes them interleaved (key, value) into out.  tk and
-// tv are scratch kept between batches.
-void radix_pairs(const int32_t *key, const int32_t *val, int32_t m, int32_t n, int32_t *out,
-                 std::vector<int32_t> &tk, std::vector<int32_t> &tv)
-{
-    int bits = 1;
-    while (bits < 31 && (1ll << bits) < (int64_t)n) ++bits;
-    const int passes = (bits + 10) / 11;
-    tk.resize(2 * (size_t)m);
-    tv.resize(2 * (size_t)m);
-    const int32_t *ck = key, *cv = val;
-    int32_t *bk[2] = {tk.data(), tk.data() + m}, *bv[2] = {tv.data(), tv.data() + m};
-    for (int p = 0; p < passes; ++p) {
-        const int sh = 11 * p;
-        uint32_t cnt[2049] = {0};
-        for (int32_t i = 0; i < m; ++i) cnt[(((uint32_t)ck[i] >> sh) & 2047u) + 1]++;
-        for (int i = 0; i < 2048; ++i) cnt[i + 1] += cnt[i];
-        if (p == passes - 1) {
-            for (int32_t i = 0; i < m; ++i) {
-                const uint32_t j = cnt[((uint32_t)ck[i] >> sh) & 2047u]++;
-                out[2 * j] = ck[i];
-                out[2 * j + 1] = cv[i];
-            }
-        } else {
-            int32_t *nk = bk[p & 1], *nv = bv[p & 1];
-            for (int32_t i = 0; i < m; ++i) {
-                const uint32_t j = cnt[((uint32_t)ck[i] >> sh) & 2047u]++;
-                nk[j] = ck[i];
-                nv[j] = cv[i];
-            }
-            ck = nk;
-            cv = nv;
-        }
-    }
-}
 float ms_since(clk::time_point t0) { return std::chrono::duration<float, std::milli>(clk::now() - t0).count(); }
 
 enum : int { RC_NEXT = 0, RC_MARKED = 1, RC_ROOTS = 2, RC_REPULLED = 3, RC_PUSHES = 4, RC_WORDS = 16 };

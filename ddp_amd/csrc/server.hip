@@ -43,6 +43,7 @@
 #include "bidi_search.h"
 #include "compact.h"
 #include "gastar.h"
+#include "serve_route.h"
 
 namespace {
 
@@ -582,10 +583,7 @@ struct ddp_server {
     hipEvent_t direct_ev = nullptr;  // after the newest direct request's kernels
     std::vector<uint8_t> direct_of;  // per ring entry: handled by kernels of its own
     int64_t requests = 0, grid_queries = 0, direct_updates = 0;
-    double grid_ms = 0, slot_ms = 0;  // moving averages of finished queries' device times (0: none yet)
-    double gap_ms = 0, last_submit_ms = -1;  // long-run mean gap between submissions
-    int64_t gaps = 0;
-    bool grid_regime = false;  // arrivals leave grid mode room (with hysteresis)
+    RouteState route;  // what the grid-mode routing goes by (serve_route.h)
 };
 
 static double ms_now(const ddp_server *sv)
@@ -711,18 +709,10 @@ static int server_ensure(ddp_server *sv)
     return DDP_OK;
 }
 
-// Requests without a result yet: queries and updates on the slot path, and
-// requests handled by kernels of their own (grid-mode queries, updates applied
-// directly); *remain: the longest a slot-path query is expected to run on,
-// from its submission and the slots' moving average.
-struct InFlight {
-    int64_t slot_q = 0, slot_u = 0, direct = 0;
-    double remain = 0;
-};
-
-static InFlight server_in_flight(const ddp_server *sv)
+// Requests without a result yet (serve_route.h).
+static RouteInFlight server_in_flight(const ddp_server *sv)
 {
-    InFlight f;
+    RouteInFlight f;
     const uint32_t mask = sv->ring - 1;
     const double now = ms_now(sv);
     for (uint32_t k = sv->collected; k != sv->submitted; ++k) {
@@ -734,50 +724,23 @@ static InFlight server_in_flight(const ddp_server *sv)
             ++f.slot_u;
         } else {
             ++f.slot_q;
-            f.remain = std::max(f.remain, sv->submit_ms[k & mask] + sv->slot_ms - now);
+            f.remain = std::max(f.remain, sv->submit_ms[k & mask] + sv->route.slot_ms - now);
         }
     }
     return f;
 }
 
-// Grid mode for the next query?
-//  * never while an update is in flight on the slot path: the grid search
-//    reads the graph without waiting for versions, and that update might not
-//    be applied before it runs;
-//  * always when nothing is in flight: alone on the GPU a query finishes
-//    sooner in grid mode (a caller sets grid_k only where that holds);
-//  * otherwise only with measurements to go by, while arrivals leave grid
-//    mode room, and if the slot queries still running plus the grid queries
-//    in flight plus this one are expected to finish before one slot query
-//    would.  "Room" has hysteresis: grid mode is entered while a grid query
-//    takes under 70% of the mean gap between submissions and left once it
-//    takes more than the whole gap.  The mean is long-run (the running mean
-//    of the first 50 gaps, then a moving average of weight 1/50), so Poisson
-//    noise seldom crosses either threshold: each flip costs the queries that
-//    straddle it (a slot query waits for the grid queue, a grid query for the
-//    running slots).
-// Routing a query to grid mode stops the serve kernel (its running slots
-// finish first): requests sent to slots later wait for the grid queue.
+// Grid mode for the next query (route_grid).  Routing a query there stops the
+// serve kernel (its running slots finish first): requests sent to slots later
+// wait for the grid queue.
 static bool server_route_grid(ddp_server *sv)
 {
-    if (!sv->grid_on) return false;
-    const InFlight f = server_in_flight(sv);
-    if (f.slot_u) return false;
-    if (f.slot_q == 0 && f.direct == 0) return true;
-    if (sv->grid_ms <= 0 || sv->slot_ms <= 0 || sv->gap_ms <= 0) return false;
-    const double rho = sv->grid_ms / sv->gap_ms;
-    sv->grid_regime = sv->grid_regime ? rho <= 1.0 : rho < 0.7;
-    return sv->grid_regime && f.remain + (double)(f.direct + 1) * sv->grid_ms < sv->slot_ms;
+    return sv->grid_on && route_grid(sv->route, server_in_flight(sv));
 }
 
-// An update goes to a kernel of its own only while nothing is in flight on
-// the slot path: a running slot kernel would otherwise have to be stopped and
-// drained for it.
 static bool server_route_direct_update(const ddp_server *sv)
 {
-    if (!sv->grid_on) return false;
-    const InFlight f = server_in_flight(sv);
-    return f.slot_q == 0 && f.slot_u == 0;
+    return sv->grid_on && route_direct_update(server_in_flight(sv));
 }
 
 extern "C" int ddp_server_create(ddp_ctx *ctx, const ddp_graph *g, ddp_heur_t h, const ddp_server_opts_t *o,
@@ -965,11 +928,7 @@ static void publish(ddp_server *sv, const Req &r, uint64_t tag, int32_t type)
     sv->tags[k] = tag;
     sv->types[k] = type;
     sv->submit_ms[k] = ms_now(sv);
-    if (sv->last_submit_ms >= 0) {
-        const double wgt = std::max(0.02, 1.0 / (double)++sv->gaps);
-        sv->gap_ms += wgt * ((sv->submit_ms[k] - sv->last_submit_ms) - sv->gap_ms);
-    }
-    sv->last_submit_ms = sv->submit_ms[k];
+    route_note_submit(sv->route, sv->submit_ms[k]);
     ++sv->submitted;
     __atomic_store_n(&sv->flags_h[0], sv->submitted, __ATOMIC_RELEASE);
 }
@@ -1221,10 +1180,8 @@ extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t
         o.done_ms = ms_now(sv);
         o.device_ms = (double)(r->t1 - r->t0) * 1e-5;
         sv->delivered[k & mask] = 1;
-        if (o.type == DDP_REQ_QUERY && o.status == 0) {  // for the grid-mode routing
-            double &avg = sv->direct_of[k & mask] ? sv->grid_ms : sv->slot_ms;
-            avg = avg > 0 ? 0.75 * avg + 0.25 * o.device_ms : o.device_ms;
-        }
+        if (o.type == DDP_REQ_QUERY && o.status == 0)  // for the grid-mode routing
+            route_note_query(sv->route, sv->direct_of[k & mask] != 0, o.device_ms);
     }
     while (sv->collected != sv->submitted && sv->delivered[sv->collected & mask]) {
         sv->delivered[sv->collected & mask] = 0;
