@@ -544,3 +544,47 @@ def test_server_grid_mode_euclid(ctx):
         d = cache[(v, s)][t]
         assert r["status"] == 0 and r["version"] == v and r["cost"] == int(d), (tag, r, d)
     assert st["grid_queries"] >= 3 * (len(ub) + 1), st
+
+
+def maze_updates(hg, width, nb, per, seed):
+    """Update batches that keep the Manhattan heuristic admissible: deletes of
+    existing edges, inserts between any two cells weighted at least their
+    Manhattan distance."""
+    rng = np.random.default_rng(seed)
+    o = np.append(hg.off, len(hg.col)).astype(np.int64)
+    deg = np.diff(o)
+    out = []
+    for _ in range(nb):
+        us = rng.integers(0, hg.N, 4 * per)
+        us = us[deg[us] > 0][:per]
+        lines = [(0, int(u), int(hg.col[o[u] + rng.integers(0, deg[u])]), 0) for u in us]
+        for a, b in zip(rng.integers(0, hg.N, per), rng.integers(0, hg.N, per)):
+            md = abs(int(a) // width - int(b) // width) + abs(int(a) % width - int(b) % width)
+            lines.append((1, int(a), int(b), md + int(rng.integers(0, 3))))
+        out.append(lines)
+    return out
+
+
+def test_server_grid_mode_manhattan(ctx):
+    """Grid mode aims the Manhattan heuristic at each query's own target cell
+    (row and column from the grid width) on a maze with updates that keep it
+    admissible; every cost equals Dijkstra's on its version."""
+    width = 256
+    hg = ddp_amd.HostGraph.grid_maze(width, 200, 300, 57)
+    dg = ddp_amd.DeviceGraph(ctx, hg)
+    dg.set_grid(width)
+    src, dst = hg.reachable_pairs(40, 58)
+    ub = maze_updates(hg, width, 3, 30, 59)
+    got, plan, st = grid_mode_session(ctx, hg, dg, ddp_amd.H_MANHATTAN, ddp_amd.ENGINE_PILES, 1024, 2048, src, dst,
+                                      ub, 0)
+    dg.close()
+    vers = versions(hg, ub)
+    cache = {}
+    for tag, (v, s, t) in plan.items():
+        r = got[tag]
+        if (v, s) not in cache:
+            cache[(v, s)], _ = pyoracle.dijkstra(*vers[v], s)
+        d = cache[(v, s)][t]
+        assert r["status"] == 0 and r["version"] == v, (tag, r)
+        assert r["cost"] == (int(d) if d >= 0 else ddp_amd.INF_COST), (tag, r, d)
+    assert st["grid_queries"] >= 4 * (len(ub) + 1), st
