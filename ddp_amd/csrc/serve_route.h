@@ -47,6 +47,13 @@ inline void route_note_query(RouteState &r, bool grid, double device_ms)
 //    be applied before it runs;
 //  * always when nothing is in flight: alone on the GPU a query finishes
 //    sooner in grid mode (a caller sets grid_k only where that holds);
+//  * before any query has finished in a slot, while nothing runs on the
+//    slot path and at most one grid query is in flight: with no slot time to
+//    weigh the queue against, a query may wait behind one grid query (where
+//    grid mode pays, one query alone finishes many times sooner there than in
+//    a slot).  Sending it to a slot instead would cost a full slot time, and
+//    the queries after it would follow, since a grid query waits for the
+//    running slots;
 //  * otherwise only with measurements to go by, while arrivals leave grid
 //    mode room, and if the slot queries still running plus the grid queries
 //    in flight plus this one are expected to finish before one slot query
@@ -60,7 +67,8 @@ inline bool route_grid(RouteState &r, const RouteInFlight &f)
 {
     if (f.slot_u) return false;
     if (f.slot_q == 0 && f.direct == 0) return true;
-    if (r.grid_ms <= 0 || r.slot_ms <= 0 || r.gap_ms <= 0) return false;
+    if (r.slot_ms <= 0) return f.slot_q == 0 && f.direct <= 1;
+    if (r.grid_ms <= 0 || r.gap_ms <= 0) return false;
     const double rho = r.grid_ms / r.gap_ms;
     r.grid_regime = r.grid_regime ? rho <= 1.0 : rho < 0.7;
     return r.grid_regime && f.remain + (double)(f.direct + 1) * r.grid_ms < r.slot_ms;

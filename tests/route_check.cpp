@@ -49,11 +49,23 @@ int main()
         expect(!route_grid(r, flight(0, 1, 0)), "slot update in flight: slots");
         expect(!route_grid(r, flight(3, 1, 2)), "slot update among others: slots");
     }
-    {  // something in flight but no measurements yet: slots
+    {  // before any slot-path query has finished: behind at most one grid query
         RouteState r;
-        expect(!route_grid(r, flight(0, 0, 1)), "grid query in flight, no measurements: slots");
+        expect(route_grid(r, flight(0, 0, 1)), "one grid query in flight, no measurements: grid");
+        expect(!route_grid(r, flight(0, 0, 2)), "two grid queries in flight, no slot average: slots");
+        expect(!route_grid(r, flight(1, 0, 0)), "a slot query running, no slot average: slots");
+        expect(!route_grid(r, flight(1, 0, 1)), "slot query and grid query, no slot average: slots");
         route_note_query(r, true, 33.0);
-        expect(!route_grid(r, flight(0, 0, 1)), "no slot average yet: slots");
+        for (int i = 0; i < 5; ++i) route_note_submit(r, i * 10.0);  // arrivals far above grid capacity
+        expect(route_grid(r, flight(0, 0, 1)), "no slot average yet, high load: still one queued");
+        expect(!route_grid(r, flight(0, 0, 2)), "no slot average yet, high load: not two");
+    }
+    {  // something in flight, a slot average but no grid or gap measurement: slots
+        RouteState r;
+        route_note_query(r, false, 700.0);
+        expect(!route_grid(r, flight(0, 0, 1)), "slot average only: slots");
+        route_note_query(r, true, 33.0);
+        expect(!route_grid(r, flight(0, 0, 1)), "no gaps yet: slots");
     }
     {  // low load: queue behind the grid queries while they finish before a slot query would
         RouteState r = measured(100.0);  // rho = 0.33
