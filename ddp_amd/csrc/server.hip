@@ -715,6 +715,7 @@ static RouteInFlight server_in_flight(const ddp_server *sv)
     RouteInFlight f;
     const uint32_t mask = sv->ring - 1;
     const double now = ms_now(sv);
+    double youngest = 0;
     for (uint32_t k = sv->collected; k != sv->submitted; ++k) {
         if (sv->delivered[k & mask] || __atomic_load_n(&sv->res_h[k & mask].seq, __ATOMIC_ACQUIRE) == k + 1)
             continue;
@@ -724,9 +725,10 @@ static RouteInFlight server_in_flight(const ddp_server *sv)
             ++f.slot_u;
         } else {
             ++f.slot_q;
-            f.remain = std::max(f.remain, sv->submit_ms[k & mask] + sv->route.slot_ms - now);
+            youngest = std::max(youngest, sv->submit_ms[k & mask]);
         }
     }
+    f.slot_age = f.slot_q ? now - youngest : 0;
     return f;
 }
 

@@ -16,13 +16,13 @@ static void expect(bool ok, const char *what)
     }
 }
 
-static RouteInFlight flight(int64_t slot_q, int64_t slot_u, int64_t direct, double remain = 0)
+static RouteInFlight flight(int64_t slot_q, int64_t slot_u, int64_t direct, double slot_age = 0)
 {
     RouteInFlight f;
     f.slot_q = slot_q;
     f.slot_u = slot_u;
     f.direct = direct;
-    f.remain = remain;
+    f.slot_age = slot_age;
     return f;
 }
 
@@ -49,23 +49,32 @@ int main()
         expect(!route_grid(r, flight(0, 1, 0)), "slot update in flight: slots");
         expect(!route_grid(r, flight(3, 1, 2)), "slot update among others: slots");
     }
-    {  // before any slot-path query has finished: behind at most one grid query
+    {  // no grid time or no arrival rate yet: behind at most one grid query, nothing on the slot path
         RouteState r;
         expect(route_grid(r, flight(0, 0, 1)), "one grid query in flight, no measurements: grid");
-        expect(!route_grid(r, flight(0, 0, 2)), "two grid queries in flight, no slot average: slots");
-        expect(!route_grid(r, flight(1, 0, 0)), "a slot query running, no slot average: slots");
-        expect(!route_grid(r, flight(1, 0, 1)), "slot query and grid query, no slot average: slots");
-        route_note_query(r, true, 33.0);
-        for (int i = 0; i < 5; ++i) route_note_submit(r, i * 10.0);  // arrivals far above grid capacity
-        expect(route_grid(r, flight(0, 0, 1)), "no slot average yet, high load: still one queued");
-        expect(!route_grid(r, flight(0, 0, 2)), "no slot average yet, high load: not two");
-    }
-    {  // something in flight, a slot average but no grid or gap measurement: slots
-        RouteState r;
+        expect(!route_grid(r, flight(0, 0, 2)), "two grid queries in flight, no measurements: slots");
+        expect(!route_grid(r, flight(1, 0, 0, 500.0)), "a slot query running, no measurements: slots");
+        expect(!route_grid(r, flight(1, 0, 1, 500.0)), "slot query and grid query, no measurements: slots");
         route_note_query(r, false, 700.0);
-        expect(!route_grid(r, flight(0, 0, 1)), "slot average only: slots");
+        expect(route_grid(r, flight(0, 0, 1)), "slot average only: still behind one grid query");
         route_note_query(r, true, 33.0);
-        expect(!route_grid(r, flight(0, 0, 1)), "no gaps yet: slots");
+        expect(route_grid(r, flight(0, 0, 1)), "no gaps yet: behind one grid query");
+        expect(!route_grid(r, flight(0, 0, 2)), "no gaps yet: not two");
+    }
+    {  // grid time and arrival rate measured, no slot query finished yet
+        RouteState r;
+        for (int i = 0; i < 40; ++i) route_note_submit(r, i * 100.0);
+        route_note_query(r, true, 33.0);  // rho = 0.33
+        expect(route_grid(r, flight(0, 0, 1)), "no slot average, one grid query ahead: grid");
+        expect(!route_grid(r, flight(0, 0, 2)), "no slot average, two grid queries ahead: slots");
+        expect(route_grid(r, flight(1, 0, 0, 50.0)), "youngest slot query 50 ms old: 33 < 50, grid");
+        expect(!route_grid(r, flight(1, 0, 0, 20.0)), "youngest slot query 20 ms old: 33 >= 20, slots");
+        expect(route_grid(r, flight(3, 0, 2, 200.0)), "two grid queries ahead, 200 ms old: 99 < 200, grid");
+        RouteState h;
+        for (int i = 0; i < 40; ++i) route_note_submit(h, i * 10.0);
+        route_note_query(h, true, 33.0);  // rho = 3.3
+        expect(!route_grid(h, flight(0, 0, 1)), "arrivals outpace grid mode: slots");
+        expect(!route_grid(h, flight(1, 0, 0, 500.0)), "arrivals outpace grid mode, old slot query: slots");
     }
     {  // low load: queue behind the grid queries while they finish before a slot query would
         RouteState r = measured(100.0);  // rho = 0.33
@@ -73,8 +82,12 @@ int main()
         expect(r.grid_regime, "rho 0.33 enters the grid regime");
         expect(route_grid(r, flight(0, 0, 19)), "20 x 33 = 660 < 700: grid");
         expect(!route_grid(r, flight(0, 0, 21)), "22 x 33 = 726 >= 700: slots");
-        expect(route_grid(r, flight(1, 0, 0, 600.0)), "a slot query ending in 600 ms: 633 < 700, grid");
-        expect(!route_grid(r, flight(1, 0, 0, 680.0)), "a slot query ending in 680 ms: 713 >= 700, slots");
+        expect(route_grid(r, flight(1, 0, 0, 100.0)), "a slot query 100 ms old, ending in 600 ms: 633 < 700, grid");
+        expect(!route_grid(r, flight(1, 0, 0, 20.0)), "a slot query 20 ms old, ending in 680 ms: 713 >= 700, slots");
+        expect(route_grid(r, flight(1, 0, 2, 100.0)), "two grid queries and a 100 ms old slot query: 99 < 100, grid");
+        expect(!route_grid(r, flight(1, 0, 3, 100.0)), "three grid queries and a 100 ms old slot query: 132 >= 100, slots");
+        expect(route_grid(r, flight(1, 0, 20, 2000.0)), "a slot query past the average: 21 x 33 = 693 < 700, grid");
+        expect(!route_grid(r, flight(1, 0, 21, 2000.0)), "a slot query past the average: 726 >= 700, slots");
     }
     {  // hysteresis: enter below 0.7, leave above 1.0
         RouteState r = measured(40.0);  // rho = 33 / 40 = 0.825
