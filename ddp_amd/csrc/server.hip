@@ -57,6 +57,8 @@ struct Req {       // pinned, written by the host
     int32_t relax_min;       // queries: every weight at this version is >= 1
     int32_t direct;          // handled by kernels of its own (a grid-mode query, or an update
                              // applied while no slot runs), skipped by the serve kernel
+    int32_t stop_gen;        // direct entries: the launch told to stop when this entry was
+                             // published (0: none ran); that launch claims nothing from here on
 };
 
 struct Res {       // pinned, written by the device
@@ -246,7 +248,10 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
         // applied (entries handled by kernels of their own are skipped at
         // once), so no slot holds a claim while it waits: the version may come
         // from an update kernel queued behind this launch, which a stop or an
-        // idle close then lets run.
+        // idle close then lets run.  A launch stopped for such an entry (its
+        // stop_gen) claims nothing from there on: requests published after it
+        // run after its kernels, in the next launch, instead of keeping this
+        // one alive while that entry's kernels wait behind it.
         if (lane == 0) {
             uint32_t ticket = 0;
             bool have = false;
@@ -264,6 +269,9 @@ __global__ void __launch_bounds__(B) serve_kernel(ServeArgs a)
                     __atomic_thread_fence(__ATOMIC_ACQUIRE);
                     const Req *hp = a.ring + (h & a.ring_mask);
                     ready = sys_ld_i(&hp->direct) != 0;
+                    // this launch was stopped for this entry: what follows it
+                    // starts after the entry's own kernels, in a later launch
+                    if (ready && (uint32_t)sys_ld_i(&hp->stop_gen) == a.gen) break;
                     if (!ready) {
                         const int32_t ver = sys_ld_i(&hp->version);
                         const uint32_t need =
@@ -961,8 +969,10 @@ static int server_grid_query(ddp_server *sv, Req r, uint64_t tag)
     gv.ver = r.version;
     gv.relax_min = sv->base_relax_min && r.relax_min;
     // an idle serve kernel would hold the stream for idle_ms: tell it to stop
+    // (its running slots finish their requests and claim none after this one)
     if (sv->running) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
     r.direct = 1;
+    r.stop_gen = sv->running ? (int32_t)sv->gen : 0;
     const uint32_t ticket = sv->submitted;
     publish(sv, r, tag, DDP_REQ_QUERY);
     hipLaunchKernelGGL(serve_grid_begin, dim3(1), dim3(64), 0, ctx->stream, sv->gstamp);
@@ -1155,7 +1165,10 @@ extern "C" int ddp_server_submit_update(ddp_server *sv, const ddp_update_t *line
     sv->log_used += (int32_t)q.size();
     sv->version += 1;
     const uint32_t ticket = sv->submitted;
-    if (direct && sv->running) __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);  // idle: stop it
+    if (direct && sv->running) {  // idle: stop it
+        __atomic_store_n(&sv->flags_h[1], sv->gen, __ATOMIC_RELEASE);
+        r.stop_gen = (int32_t)sv->gen;
+    }
     publish(sv, r, tag, DDP_REQ_UPDATE);
     if (direct) return server_direct_update(sv, ticket, r.version);
     return server_ensure(sv);
