@@ -1,6 +1,7 @@
 // Routing of the server's low-load grid mode (server.hip): pure host-side
-// functions of what the host knows -- the requests in flight, moving averages
-// of finished queries' device times, and the gaps between submissions.  No
+// functions of what the host knows -- the requests in flight, a moving average
+// of finished grid-mode queries' device times, and the gaps between
+// submissions.  No
 // HIP dependency, so the policy is also built and checked on the CPU.
 #pragma once
 
@@ -17,7 +18,7 @@ struct RouteInFlight {
 };
 
 struct RouteState {
-    double grid_ms = 0, slot_ms = 0;         // moving averages of finished queries' device times (0: none yet)
+    double grid_ms = 0;                      // moving average of grid-mode queries' device times (0: none yet)
     double gap_ms = 0, last_submit_ms = -1;  // long-run mean gap between submissions
     int64_t gaps = 0;
     bool grid_regime = false;                // arrivals leave grid mode room (with hysteresis)
@@ -34,39 +35,37 @@ inline void route_note_submit(RouteState &r, double now_ms)
     r.last_submit_ms = now_ms;
 }
 
-// A query finished without error in device_ms, in grid mode or in a slot.
-inline void route_note_query(RouteState &r, bool grid, double device_ms)
+// A grid-mode query finished without error in device_ms.
+inline void route_note_grid(RouteState &r, double device_ms)
 {
-    double &avg = grid ? r.grid_ms : r.slot_ms;
-    avg = avg > 0 ? 0.75 * avg + 0.25 * device_ms : device_ms;
+    r.grid_ms = r.grid_ms > 0 ? 0.75 * r.grid_ms + 0.25 * device_ms : device_ms;
 }
 
-// Grid mode for the next query?  A query routed there waits for every
-// running slot query (the serve kernel is stopped and its slots finish first)
-// and for the grid queries ahead of it, then takes one grid time G; in a slot
-// it would take one slot time S from now.
+// Grid mode for the next query?  Grid-mode queries run one after another on
+// the server's stream, after the slot queries already running: routing one
+// there stops the serve kernel, whose slots finish their requests and claim
+// nothing published after it.  So a query sent to a slot while grid queries
+// are queued starts after them as well.  With G a grid query's time and S a
+// slot query's (many G where grid_k is set):
 //  * never while an update is in flight on the slot path: the grid search
 //    reads the graph without waiting for versions, and that update might not
 //    be applied before it runs;
 //  * always when nothing is in flight: alone on the GPU a query finishes
-//    sooner in grid mode (a caller sets grid_k only where that holds, and
-//    where it pays one query alone finishes many times sooner there);
-//  * before G and the arrival rate are measured, only behind at most one grid
-//    query and with nothing on the slot path;
-//  * otherwise while arrivals leave grid mode room and the grid queue plus
-//    this query, (direct + 1) G, ends before a slot query would.  With slot
-//    queries running, the youngest of them ends about S after its submission,
-//    so the test is (direct + 1) G < S - (S - slot_age) = slot_age, which
-//    needs no S (capped at S once that query has run longer than S); with none
-//    running it is (direct + 1) G < S, or, before any slot query has finished,
-//    at most one grid query ahead.  A query sent to a slot for want of S would
-//    cost a full slot time and hold up every grid query after it.
-//    "Room" has hysteresis: grid mode is entered while a grid query takes
-//    under 70% of the mean gap between submissions and left once it takes
-//    more than the whole gap, so Poisson noise in the long-run mean seldom
-//    crosses either threshold: each flip costs the queries that straddle it
-//    (a slot query waits for the grid queue, a grid query for the running
-//    slots).
+//    sooner in grid mode (a caller sets grid_k only where that holds);
+//  * before G and the arrival rate are measured, only with nothing on the
+//    slot path and at most one grid query ahead;
+//  * otherwise only while arrivals leave grid mode room, and then
+//    - with grid queries queued: in a slot the query would wait for the same
+//      queue and then take S > G;
+//    - with slot queries running and no grid query queued: it waits for them,
+//      and the youngest ends about S after its submission, so it finishes at
+//      about S - slot_age + G, against S in a slot: grid once slot_age > G.
+//    "Room" has hysteresis: grid mode is entered while G is under 70% of the
+//    mean gap between submissions and left once it exceeds the whole gap (a
+//    single queue of grid queries then keeps up with arrivals), and Poisson
+//    noise in the long-run mean seldom crosses either threshold: each flip
+//    costs the queries that straddle it (a slot query waits for the grid
+//    queue, a grid query for the running slots).
 inline bool route_grid(RouteState &r, const RouteInFlight &f)
 {
     if (f.slot_u) return false;
@@ -75,9 +74,8 @@ inline bool route_grid(RouteState &r, const RouteInFlight &f)
     const double rho = r.grid_ms / r.gap_ms;
     r.grid_regime = r.grid_regime ? rho <= 1.0 : rho < 0.7;
     if (!r.grid_regime) return false;
-    const double queue = (double)(f.direct + 1) * r.grid_ms;
-    if (f.slot_q) return queue < (r.slot_ms > 0 ? std::min(f.slot_age, r.slot_ms) : f.slot_age);
-    return r.slot_ms > 0 ? queue < r.slot_ms : f.direct <= 1;
+    if (f.direct) return true;
+    return f.slot_age > r.grid_ms;
 }
 
 // An update goes to a kernel of its own only while nothing is in flight on

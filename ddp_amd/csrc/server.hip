@@ -1195,8 +1195,8 @@ extern "C" int ddp_server_poll(ddp_server *sv, ddp_server_result_t *out, int32_t
         o.done_ms = ms_now(sv);
         o.device_ms = (double)(r->t1 - r->t0) * 1e-5;
         sv->delivered[k & mask] = 1;
-        if (o.type == DDP_REQ_QUERY && o.status == 0)  // for the grid-mode routing
-            route_note_query(sv->route, sv->direct_of[k & mask] != 0, o.device_ms);
+        if (o.type == DDP_REQ_QUERY && o.status == 0 && sv->direct_of[k & mask])  // for the grid-mode routing
+            route_note_grid(sv->route, o.device_ms);
     }
     while (sv->collected != sv->submitted && sv->delivered[sv->collected & mask]) {
         sv->delivered[sv->collected & mask] = 0;

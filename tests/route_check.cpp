@@ -26,13 +26,12 @@ static RouteInFlight flight(int64_t slot_q, int64_t slot_u, int64_t direct, doub
     return f;
 }
 
-// grid 33 ms, slot 700 ms, a submission every gap ms (the running mean is exact)
+// grid 33 ms, a submission every gap ms (the running mean is exact)
 static RouteState measured(double gap)
 {
     RouteState r;
     for (int i = 0; i < 40; ++i) route_note_submit(r, i * gap);
-    route_note_query(r, true, 33.0);
-    route_note_query(r, false, 700.0);
+    route_note_grid(r, 33.0);
     return r;
 }
 
@@ -48,6 +47,7 @@ int main()
         RouteState r = measured(100.0);
         expect(!route_grid(r, flight(0, 1, 0)), "slot update in flight: slots");
         expect(!route_grid(r, flight(3, 1, 2)), "slot update among others: slots");
+        expect(!route_grid(r, flight(0, 1, 0, 5000.0)), "slot update, old slot query: slots");
     }
     {  // no grid time or no arrival rate yet: behind at most one grid query, nothing on the slot path
         RouteState r;
@@ -55,39 +55,33 @@ int main()
         expect(!route_grid(r, flight(0, 0, 2)), "two grid queries in flight, no measurements: slots");
         expect(!route_grid(r, flight(1, 0, 0, 500.0)), "a slot query running, no measurements: slots");
         expect(!route_grid(r, flight(1, 0, 1, 500.0)), "slot query and grid query, no measurements: slots");
-        route_note_query(r, false, 700.0);
-        expect(route_grid(r, flight(0, 0, 1)), "slot average only: still behind one grid query");
-        route_note_query(r, true, 33.0);
+        route_note_grid(r, 33.0);
         expect(route_grid(r, flight(0, 0, 1)), "no gaps yet: behind one grid query");
         expect(!route_grid(r, flight(0, 0, 2)), "no gaps yet: not two");
+        RouteState g;
+        for (int i = 0; i < 5; ++i) route_note_submit(g, i * 100.0);
+        expect(route_grid(g, flight(0, 0, 1)), "no grid time yet: behind one grid query");
+        expect(!route_grid(g, flight(0, 0, 2)), "no grid time yet: not two");
     }
-    {  // grid time and arrival rate measured, no slot query finished yet
-        RouteState r;
-        for (int i = 0; i < 40; ++i) route_note_submit(r, i * 100.0);
-        route_note_query(r, true, 33.0);  // rho = 0.33
-        expect(route_grid(r, flight(0, 0, 1)), "no slot average, one grid query ahead: grid");
-        expect(!route_grid(r, flight(0, 0, 2)), "no slot average, two grid queries ahead: slots");
-        expect(route_grid(r, flight(1, 0, 0, 50.0)), "youngest slot query 50 ms old: 33 < 50, grid");
-        expect(!route_grid(r, flight(1, 0, 0, 20.0)), "youngest slot query 20 ms old: 33 >= 20, slots");
-        expect(route_grid(r, flight(3, 0, 2, 200.0)), "two grid queries ahead, 200 ms old: 99 < 200, grid");
-        RouteState h;
-        for (int i = 0; i < 40; ++i) route_note_submit(h, i * 10.0);
-        route_note_query(h, true, 33.0);  // rho = 3.3
+    {  // in the grid regime: grid whenever grid queries are queued (a slot query would wait for them)
+        RouteState r = measured(100.0);  // rho = 0.33
+        expect(route_grid(r, flight(0, 0, 1)), "one grid query queued: grid");
+        expect(r.grid_regime, "rho 0.33 enters the grid regime");
+        expect(route_grid(r, flight(0, 0, 40)), "forty queued: a slot query would wait for them too, grid");
+        expect(route_grid(r, flight(3, 0, 2, 1.0)), "grid queries queued, a young slot query: grid");
+    }
+    {  // in the grid regime, slot queries running and no grid query queued: grid once the youngest is older than G
+        RouteState r = measured(100.0);
+        expect(route_grid(r, flight(1, 0, 0, 50.0)), "youngest slot query 50 ms old: 50 > 33, grid");
+        expect(!route_grid(r, flight(1, 0, 0, 20.0)), "youngest slot query 20 ms old: 20 <= 33, slots");
+        expect(!route_grid(r, flight(4, 0, 0, 33.0)), "exactly G old: slots");
+        expect(route_grid(r, flight(4, 0, 0, 900.0)), "long-running slot queries: grid");
+    }
+    {  // out of the regime: slots unless nothing is in flight
+        RouteState h = measured(10.0);  // rho = 3.3
         expect(!route_grid(h, flight(0, 0, 1)), "arrivals outpace grid mode: slots");
         expect(!route_grid(h, flight(1, 0, 0, 500.0)), "arrivals outpace grid mode, old slot query: slots");
-    }
-    {  // low load: queue behind the grid queries while they finish before a slot query would
-        RouteState r = measured(100.0);  // rho = 0.33
-        expect(route_grid(r, flight(0, 0, 2)), "two grid queries queued: grid");
-        expect(r.grid_regime, "rho 0.33 enters the grid regime");
-        expect(route_grid(r, flight(0, 0, 19)), "20 x 33 = 660 < 700: grid");
-        expect(!route_grid(r, flight(0, 0, 21)), "22 x 33 = 726 >= 700: slots");
-        expect(route_grid(r, flight(1, 0, 0, 100.0)), "a slot query 100 ms old, ending in 600 ms: 633 < 700, grid");
-        expect(!route_grid(r, flight(1, 0, 0, 20.0)), "a slot query 20 ms old, ending in 680 ms: 713 >= 700, slots");
-        expect(route_grid(r, flight(1, 0, 2, 100.0)), "two grid queries and a 100 ms old slot query: 99 < 100, grid");
-        expect(!route_grid(r, flight(1, 0, 3, 100.0)), "three grid queries and a 100 ms old slot query: 132 >= 100, slots");
-        expect(route_grid(r, flight(1, 0, 20, 2000.0)), "a slot query past the average: 21 x 33 = 693 < 700, grid");
-        expect(!route_grid(r, flight(1, 0, 21, 2000.0)), "a slot query past the average: 726 >= 700, slots");
+        expect(!h.grid_regime, "regime off at rho 3.3");
     }
     {  // hysteresis: enter below 0.7, leave above 1.0
         RouteState r = measured(40.0);  // rho = 33 / 40 = 0.825
@@ -108,10 +102,10 @@ int main()
     }
     {  // moving averages
         RouteState r;
-        route_note_query(r, false, 800.0);
-        expect(r.slot_ms == 800.0 && r.grid_ms == 0.0, "first slot query sets its average");
-        route_note_query(r, false, 400.0);
-        expect(std::fabs(r.slot_ms - 700.0) < 1e-9, "then 0.75 old + 0.25 new");
+        route_note_grid(r, 40.0);
+        expect(r.grid_ms == 40.0, "first grid query sets its average");
+        route_note_grid(r, 20.0);
+        expect(std::fabs(r.grid_ms - 35.0) < 1e-9, "then 0.75 old + 0.25 new");
         RouteState g;
         route_note_submit(g, 5.0);
         expect(g.gap_ms == 0.0 && g.gaps == 0, "one submission: no gap yet");
