@@ -48,15 +48,15 @@ typedef struct {
     int32_t grid_k;         /* 0 = off.  K > 0: low-load queries are searched in grid mode
                                (ddp_astar's DDP_ASTAR_GRID, K lane heaps over the whole GPU)
                                instead of one slot: a query goes there while no update is
-                               in flight on the slot path, if nothing else is in flight
-                               or if, at a submission rate that leaves grid mode room, the
-                               running slot queries and the grid queries ahead plus this
-                               one are expected to finish before one query in a slot
-                               would (moving averages of finished queries' device times,
-                               the age of the running slot queries).  Grid queries run one
-                               after another with the GPU to themselves, after the slot
-                               queries already running; requests sent to slots meanwhile
-                               start after them.  Worth it where
+                               in flight on the slot path, if nothing else is in flight,
+                               or, while grid mode alone keeps up with submissions (a
+                               moving average of grid-mode queries' device times against
+                               the mean gap between submissions), if grid queries are
+                               queued or the youngest running slot query is older than
+                               one grid-mode query takes.  Grid queries run one after
+                               another with the GPU to themselves, after the slot queries
+                               already running; requests sent to slots meanwhile start
+                               after them.  Worth it where
                                one query is a large share of the graph (a full search of a
                                power-law graph), which alone on the GPU finishes many
                                times sooner than in one slot */
