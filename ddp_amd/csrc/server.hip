@@ -25,9 +25,10 @@
 // keep up) is searched by the single-query GA* grid kernel (K lane heaps over
 // the whole GPU, gastar.hip) instead of one slot.  Its ring entry is marked
 // so that the serve kernel skips it, and its search runs on the server's
-// stream after any serve kernel (which is told to stop) and before the next
-// launch; a one-lane kernel then writes its result into the ring like a slot
-// would.  Updates submitted while nothing runs on the slot path are applied
+// stream after any serve kernel (which is told to stop: its running slots
+// finish their requests and claim nothing published after the entry) and
+// before the next launch; a one-lane kernel then writes its result into the
+// ring like a slot would.  Updates submitted while nothing runs on the slot path are applied
 // the same way, by a kernel of their own.
 #include <algorithm>
 #include <array>
