@@ -11,11 +11,17 @@
 // Requests without a result yet: queries and updates on the slot path, and
 // requests handled by kernels of their own (grid-mode queries, updates applied
 // directly); slot_age: how long ago the most recently submitted slot-path
-// query in flight was submitted (ms; 0 when slot_q is 0).
+// query in flight was submitted (ms; 0 when slot_q is 0); slots_open: a
+// serve launch is running and has not been told to stop.
 struct RouteInFlight {
     int64_t slot_q = 0, slot_u = 0, direct = 0;
     double slot_age = 0;
+    bool slots_open = false;
 };
+
+// Gaps between submissions measured before the arrival rate counts: the
+// first few gaps of a Poisson stream are too noisy to set the regime by.
+constexpr int64_t kRouteMinGaps = 8;
 
 struct RouteState {
     double grid_ms = 0;                      // moving average of grid-mode queries' device times (0: none yet)
@@ -52,7 +58,7 @@ inline void route_note_grid(RouteState &r, double device_ms)
 //    be applied before it runs;
 //  * always when nothing is in flight: alone on the GPU a query finishes
 //    sooner in grid mode (a caller sets grid_k only where that holds);
-//  * before G and the arrival rate are measured, only with nothing on the
+//  * before G and kRouteMinGaps gaps are measured, only with nothing on the
 //    slot path and at most one grid query ahead;
 //  * otherwise only while arrivals leave grid mode room, and then
 //    - with grid queries queued: in a slot the query would wait for the same
@@ -70,7 +76,7 @@ inline bool route_grid(RouteState &r, const RouteInFlight &f)
 {
     if (f.slot_u) return false;
     if (f.slot_q == 0 && f.direct == 0) return true;
-    if (r.grid_ms <= 0 || r.gap_ms <= 0) return f.slot_q == 0 && f.direct <= 1;
+    if (r.grid_ms <= 0 || r.gaps < kRouteMinGaps) return f.slot_q == 0 && f.direct <= 1;
     const double rho = r.grid_ms / r.gap_ms;
     r.grid_regime = r.grid_regime ? rho <= 1.0 : rho < 0.7;
     if (!r.grid_regime) return false;
@@ -78,7 +84,12 @@ inline bool route_grid(RouteState &r, const RouteInFlight &f)
     return f.slot_age > r.grid_ms;
 }
 
-// An update goes to a kernel of its own only while nothing is in flight on
-// the slot path: a running slot kernel would otherwise have to be stopped and
-// drained for it.
-inline bool route_direct_update(const RouteInFlight &f) { return f.slot_q == 0 && f.slot_u == 0; }
+// An update goes to a kernel of its own unless an update is pending on the
+// slot path (versions are applied in order) or an open serve launch runs slot
+// queries (it would have to be stopped and drained for it).  Once the launch
+// has been stopped for grid work, a slot update would wait for that queue and
+// a relaunch, and while it is pending no query could go to grid mode.
+inline bool route_direct_update(const RouteInFlight &f)
+{
+    return f.slot_u == 0 && (f.slot_q == 0 || !f.slots_open);
+}

@@ -738,6 +738,7 @@ static RouteInFlight server_in_flight(const ddp_server *sv)
         }
     }
     f.slot_age = f.slot_q ? now - youngest : 0;
+    f.slots_open = sv->running && __atomic_load_n(&sv->flags_h[1], __ATOMIC_RELAXED) != sv->gen;
     return f;
 }
 

@@ -62,6 +62,14 @@ int main()
         for (int i = 0; i < 5; ++i) route_note_submit(g, i * 100.0);
         expect(route_grid(g, flight(0, 0, 1)), "no grid time yet: behind one grid query");
         expect(!route_grid(g, flight(0, 0, 2)), "no grid time yet: not two");
+        RouteState few;  // a grid time but fewer than kRouteMinGaps gaps: the rate does not count yet
+        for (int i = 0; i < 8; ++i) route_note_submit(few, i * 10.0);  // 7 gaps, rho 3.3
+        route_note_grid(few, 33.0);
+        expect(route_grid(few, flight(0, 0, 1)), "7 gaps: still behind one grid query");
+        expect(!route_grid(few, flight(0, 0, 2)), "7 gaps: not two");
+        route_note_submit(few, 80.0);  // the 8th gap: rho 3.3 counts
+        expect(!route_grid(few, flight(0, 0, 1)), "8 gaps at rho 3.3: slots");
+        expect(!few.grid_regime, "8 gaps at rho 3.3: no regime");
     }
     {  // in the grid regime: grid whenever grid queries are queued (a slot query would wait for them)
         RouteState r = measured(100.0);  // rho = 0.33
@@ -94,11 +102,21 @@ int main()
         expect(!route_grid(q, flight(0, 0, 1)), "rho 1.1 leaves");
         expect(!q.grid_regime, "regime off after rho 1.1");
     }
-    {  // updates go to their own kernel only with nothing on the slot path
+    {  // updates go to their own kernel unless a slot update is pending or an open launch runs slot queries
+        RouteInFlight open_q = flight(1, 0, 0), stopped_q = flight(3, 0, 2), pend = flight(0, 1, 0);
+        open_q.slots_open = true;
+        pend.slots_open = false;
         expect(route_direct_update(flight(0, 0, 0)), "idle: direct update");
+        RouteInFlight idle_open = flight(0, 0, 0);
+        idle_open.slots_open = true;
+        expect(route_direct_update(idle_open), "an open launch with no slot query: direct update");
         expect(route_direct_update(flight(0, 0, 5)), "grid queries queued: direct update");
-        expect(!route_direct_update(flight(1, 0, 0)), "slot query running: slot update");
-        expect(!route_direct_update(flight(0, 1, 0)), "slot update pending: slot update");
+        expect(!route_direct_update(open_q), "an open launch runs a slot query: slot update");
+        expect(route_direct_update(stopped_q), "slot queries in a stopped launch: direct update");
+        expect(!route_direct_update(pend), "slot update pending: slot update (versions in order)");
+        RouteInFlight pend_open = flight(2, 1, 0);
+        pend_open.slots_open = true;
+        expect(!route_direct_update(pend_open), "slot update pending in an open launch: slot update");
     }
     {  // moving averages
         RouteState r;

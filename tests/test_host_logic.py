@@ -1,6 +1,7 @@
 """Host logic of the library built for the CPU with g++ from the same headers
 the library compiles: the server's grid-mode routing policy
-(ddp_amd/csrc/serve_route.h) and DA*'s insert grouping
+(ddp_amd/csrc/serve_route.h), its rules and its behaviour under load against a
+queueing model of the server, and DA*'s insert grouping
 (ddp_amd/csrc/host_sort.h)."""
 import os
 import shutil
@@ -25,6 +26,13 @@ def build_and_run(tmp_path, name, sanitize):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_serve_route_policy(tmp_path):
     build_and_run(tmp_path, "route_check", sanitize=False)
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_serve_route_under_load(tmp_path):
+    # the policy against a queueing model of the server (Poisson arrivals,
+    # stopped launches, relaunches behind the grid queue)
+    build_and_run(tmp_path, "route_sim", sanitize=False)
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
