@@ -28,8 +28,9 @@
 // stream after any serve kernel (which is told to stop: its running slots
 // finish their requests and claim nothing published after the entry) and
 // before the next launch; a one-lane kernel then writes its result into the
-// ring like a slot would.  Updates submitted while nothing runs on the slot path are applied
-// the same way, by a kernel of their own.
+// ring like a slot would.  Updates are applied the same way, by a kernel of
+// their own, unless an update is pending on the slot path or an open launch
+// runs slot queries (serve_route.h).
 #include <algorithm>
 #include <array>
 #include <atomic>

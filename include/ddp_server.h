@@ -99,8 +99,9 @@ int ddp_server_info(const ddp_server *sv, int32_t *slots, int32_t *block, int32_
 typedef struct {
     int64_t requests;       /* submitted so far */
     int64_t grid_queries;   /* of them, queries searched in grid mode (grid_k) */
-    int64_t direct_updates; /* of them, updates applied by a kernel of their own while
-                               queries ran in grid mode (grid_k) */
+    int64_t direct_updates; /* of them, updates applied by a kernel of their own (grid_k):
+                               no update pending on the slot path and no open launch
+                               running slot queries */
     int32_t launches;       /* serve-kernel launches (the first request, and relaunches
                                after an idle exit, a grid-mode query or a compaction) */
     int32_t compactions;    /* inserted-edge log compactions */
